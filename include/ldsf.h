@@ -1,0 +1,213 @@
+/*
+ * ldsf.h -- C-ABI of the MI355X (gfx950) Behler symmetry-function path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (LDSF_OK == 0) and never throws; ldsf_last_error() returns the calling
+ * thread's last message.  Host-pointer entry points keep the reference's
+ * argument lists and its "+=" semantics into caller-owned buffers
+ * (src/SymmetryFunctions.cpp:192,253; python_bindings/python_bindings.cpp:99,
+ * 124,147,176,205,241,276 zero them first).  Device-pointer entry points take
+ * a plan built once per neighbour list and a hipStream_t passed as void*.
+ *
+ * Layouts (identical to the reference):
+ *   species[n_total]            int32, species code of every row (padding too)
+ *   number_of_neighs[n_atoms]   int32, neighbour count of each centre
+ *   neighbor_list[sum]          int32, global rows, centres concatenated
+ *   coordinates[n_total][3]     float64, row-major
+ *   desc[n_atoms][width]        float64, row-major, +=
+ *   dE_dzeta[n_atoms][width]    float64
+ *   d_coordinates[n_total][3]   float64, +=, padding rows included
+ */
+#ifndef LDSF_H_
+#define LDSF_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDSF_API __attribute__((visibility("default")))
+
+/* status codes */
+#define LDSF_OK 0
+#define LDSF_ERR_ARG 1             /* bad argument (C++ layer: std::invalid_argument) */
+#define LDSF_ERR_DEVICE 2          /* HIP runtime error / no device */
+#define LDSF_ERR_UNSUPPORTED 3     /* parameter set or size not supported */
+#define LDSF_ERR_FILE 4            /* parameter file I/O or parse (std::runtime_error) */
+#define LDSF_ERR_NEIGHBOR 5        /* neighbour-list build failure (collision, huge box) */
+#define LDSF_ERR_SINGULAR_CELL 6   /* padding: cell matrix not invertible */
+#define LDSF_ERR_NOMEM 7           /* device allocation failed */
+#define LDSF_ERR_KIND 8            /* descriptor kind other than SymmetryFunctions */
+
+/* precision forms */
+#define LDSF_FP64 0       /* plain form: FP64 throughout */
+#define LDSF_FP32_BULK 1  /* FP32 exp/cos/pow terms; FP64 geometry and accumulation */
+
+typedef struct ldsf_sf ldsf_sf;          /* a SymmetryFunctions parameter set */
+typedef struct ldsf_plan ldsf_plan;      /* device-resident neighbour-list plan */
+typedef struct ldsf_neighlist ldsf_neighlist;
+typedef struct ldsf_padding ldsf_padding;
+
+LDSF_API const char *ldsf_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Parameter sets
+ * ------------------------------------------------------------------------ */
+
+/* SymmetryFunctions(species, cutoff_function, cutoff_matrix, types, sizes,
+ * params) (src/SymmetryFunctions.cpp:429-450), reached from
+ * DescriptorKind::initDescriptor's SF overload (src/Descriptors.cpp:579-590).
+ * cutoff_matrix is n_species x n_species row-major; type_names are "g1".."g5";
+ * sizes is [rows0, cols0, rows1, cols1, ...]; params is flat row-major.
+ * species_resolved = 1 selects the per-neighbour-species G1-G3 blocks and
+ * per-unordered-species-pair G4/G5 blocks (build extension; 0 = reference). */
+LDSF_API int ldsf_sf_create(ldsf_sf **out, int n_species, const char *const *species_names,
+                            const double *cutoff_matrix, int n_types,
+                            const char *const *type_names, const int *sizes,
+                            const double *params, int species_resolved);
+
+/* SymmetryFunctions(std::string&) -> initFromFile (src/SymmetryFunctions.cpp:262-407).
+ * Species are sorted as the reference's std::set sorts them. */
+LDSF_API int ldsf_sf_create_from_file(ldsf_sf **out, const char *path);
+
+LDSF_API void ldsf_sf_destroy(ldsf_sf *sf);
+
+/* DescriptorKind::width (include/Descriptors.hpp:184) */
+LDSF_API int ldsf_sf_width(const ldsf_sf *sf);
+LDSF_API int ldsf_sf_num_species(const ldsf_sf *sf);
+/* species name k (sorted order for file init); NULL if out of range */
+LDSF_API const char *ldsf_sf_species_name(const ldsf_sf *sf, int k);
+/* default precision of the reference-signature entry points below */
+LDSF_API int ldsf_sf_set_precision(ldsf_sf *sf, int precision);
+
+/* ------------------------------------------------------------------------
+ * Reference-signature entry points (host pointers)
+ * ------------------------------------------------------------------------ */
+
+/* Descriptor::compute (include/Descriptors.hpp:130-132, src/Descriptors.cpp:121-136) */
+LDSF_API int ldsf_compute(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
+                          const int *number_of_neighs, const double *coordinates, double *desc);
+
+/* Descriptor::compute_single_atom (Descriptors.hpp:163-165, Descriptors.cpp:355-365) */
+LDSF_API int ldsf_compute_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
+                                      const int *neighbor_list, int number_of_neighs,
+                                      const double *coordinates, double *desc);
+
+/* Descriptor::compute_batch (Descriptors.hpp:147-149, Descriptors.cpp:138-163) */
+LDSF_API int ldsf_compute_batch(ldsf_sf *sf, int n_configurations, const int *n_atoms,
+                                const int *config_ptr, const int *species,
+                                const int *neighbor_list, const int *number_of_neighs,
+                                const double *coordinates, double *desc);
+
+/* Descriptor::gradient (Descriptors.hpp:58-60, Descriptors.cpp:165-190).
+ * d_coordinates += J^T dE_dzeta.  desc is accepted for signature parity and
+ * is not read or written (the reference's callers pass a throwaway copy). */
+LDSF_API int ldsf_gradient(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
+                           const int *number_of_neighs, const double *coordinates,
+                           double *d_coordinates, double *desc, const double *dE_dzeta);
+
+/* Descriptor::gradient_single_atom (Descriptors.hpp:97-99, Descriptors.cpp:368-395) */
+LDSF_API int ldsf_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
+                                       const int *neighbor_list, int number_of_neighs,
+                                       const double *coordinates, double *d_coordinates,
+                                       double *desc, const double *dE_dzeta);
+
+/* Descriptor::gradient_batch (Descriptors.hpp:78-80, Descriptors.cpp:263-292) */
+LDSF_API int ldsf_gradient_batch(ldsf_sf *sf, int n_configurations, const int *n_atoms,
+                                 const int *config_ptr, const int *species,
+                                 const int *neighbor_list, const int *number_of_neighs,
+                                 const double *coordinates, double *d_coordinates, double *desc,
+                                 const double *dE_dzeta);
+
+/* Descriptor::jacobian (Descriptors.hpp:117-118, Descriptors.cpp:454-486):
+ * J_coordinates[width*n_atoms][3*n_total_atoms] += d desc / d coords. */
+LDSF_API int ldsf_jacobian(ldsf_sf *sf, int n_atoms, int n_total_atoms, const int *species,
+                           const int *neighbor_list, const int *number_of_neighs,
+                           const double *coordinates, double *J_coordinates);
+
+/* Descriptor::num_gradient_single_atom (Descriptors.hpp:101-103,
+ * Descriptors.cpp:488-518, include/finite_difference.hpp:46-143):
+ * d_coordinates[0:3] += sum_w dE_dzeta[w] d desc_index[w] / d x_index, by the
+ * order-8 central stencil with h = (551.25 eps)^(1/9); the descriptor
+ * evaluations run on the GPU.  n_atoms is the number of coordinate rows. */
+LDSF_API int ldsf_num_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms,
+                                           const int *species, const int *neighbor_list,
+                                           int number_of_neighs, const double *coordinates,
+                                           double *d_coordinates, const double *dE_dzeta);
+
+/* ------------------------------------------------------------------------
+ * Device-pointer entry points (no reference counterpart: the reference has
+ * no device memory).  A plan holds the CSR offsets of the neighbour list, its
+ * transpose (for the atomic-free adjoint gather) and scratch, on the current
+ * HIP device.  All device pointers must be on that device; work is ordered on
+ * `stream` (a hipStream_t, NULL = default stream).
+ * ------------------------------------------------------------------------ */
+
+/* Plan over centres centers[0..n_centers) (device int32; NULL = rows
+ * 0..n_centers-1), with n_total coordinate rows.  Synchronises `stream` once
+ * (to size the launch from the largest neighbour count). */
+LDSF_API int ldsf_plan_create(ldsf_plan **out, const ldsf_sf *sf, int n_centers,
+                              const int *d_centers, int n_total, const int *d_species,
+                              const int *d_neighbor_list, const int *d_number_of_neighs,
+                              void *stream);
+
+/* Plan over the concatenated configurations of compute_batch/gradient_batch:
+ * n_atoms[] and config_ptr[] are HOST arrays (Descriptors.cpp:154-161). */
+LDSF_API int ldsf_plan_create_batch(ldsf_plan **out, const ldsf_sf *sf, int n_configurations,
+                                    const int *n_atoms, const int *config_ptr, int n_total,
+                                    const int *d_species, const int *d_neighbor_list,
+                                    const int *d_number_of_neighs, void *stream);
+
+LDSF_API void ldsf_plan_destroy(ldsf_plan *plan);
+LDSF_API int ldsf_plan_num_centers(const ldsf_plan *plan);
+LDSF_API long ldsf_plan_num_edges(const ldsf_plan *plan);
+LDSF_API int ldsf_plan_max_neighbors(const ldsf_plan *plan);
+
+/* d_desc[n_centers][width] += descriptors of the plan's centres. */
+LDSF_API int ldsf_plan_compute(ldsf_plan *plan, const double *d_coordinates, double *d_desc,
+                               int precision, void *stream);
+
+/* d_dcoords[n_total][3] += J^T d_dE_dzeta (dE_dzeta is [n_centers][width]). */
+LDSF_API int ldsf_plan_gradient(ldsf_plan *plan, const double *d_coordinates,
+                                const double *d_dE_dzeta, double *d_dcoords, int precision,
+                                void *stream);
+
+/* Which kernel family a plan uses: 0 = general (any parameter set), 1 = wave-
+ * per-centre LDS kernels.  Setting 0 forces the general kernels. */
+LDSF_API int ldsf_plan_kernel_family(const ldsf_plan *plan);
+LDSF_API int ldsf_plan_set_kernel_family(ldsf_plan *plan, int family);
+
+/* ------------------------------------------------------------------------
+ * Input-layout producers (python_bindings/neighbor/neighbor_list.h)
+ * ------------------------------------------------------------------------ */
+
+/* nbl_create_paddings (neighbor_list.cpp:283-418) */
+LDSF_API int ldsf_nbl_create_paddings(int n_particles, double cutoff, const double *cell,
+                                      const int *pbc, const double *coords, const int *species,
+                                      ldsf_padding **out);
+LDSF_API int ldsf_padding_count(const ldsf_padding *p);
+LDSF_API int ldsf_padding_copy(const ldsf_padding *p, double *coords, int *species, int *image);
+LDSF_API void ldsf_padding_free(ldsf_padding *p);
+
+/* nbl_initialize / nbl_build / nbl_clean (neighbor_list.cpp:61-243) */
+LDSF_API ldsf_neighlist *ldsf_nbl_create(void);
+LDSF_API void ldsf_nbl_free(ldsf_neighlist *nl);
+LDSF_API int ldsf_nbl_build(ldsf_neighlist *nl, int n_particles, const double *coords,
+                            double influence_distance, int n_cutoffs, const double *cutoffs,
+                            const int *need_neighbors);
+/* nbl_get_neigh (neighbor_list.cpp:246-280), KIM GetNeighbor signature; returns 1 on error */
+LDSF_API int ldsf_nbl_get_neigh(void const *data_object, int n_cutoffs, double const *cutoffs,
+                                int neighbor_list_index, int particle_number,
+                                int *number_of_neighbors, int const **neighbors_of_particle);
+LDSF_API int ldsf_nbl_num_lists(const ldsf_neighlist *nl);
+LDSF_API double ldsf_nbl_list_cutoff(const ldsf_neighlist *nl, int k);
+LDSF_API int ldsf_nbl_num_particles(const ldsf_neighlist *nl);
+LDSF_API long ldsf_nbl_total(const ldsf_neighlist *nl, int k, int n_query);
+/* NeighborList.get_numneigh_and_neighlist_1D (python_package/libdescriptor/neighbor.py:108-154) */
+LDSF_API int ldsf_nbl_flatten(const ldsf_neighlist *nl, int k, int n_query, int *numneigh,
+                              int *neighlist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LDSF_H_ */

@@ -1,0 +1,178 @@
+"""ctypes binding of the device-pointer half of the C-ABI (include/ldsf.h):
+plans over device-resident neighbour lists and the forward / adjoint launches
+on a caller's HIP stream.  Used by the torch autograd Function, the batched
+server and the benchmark.  Raises if libldsf.so is not built.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libldsf.so")
+
+LDSF_OK = 0
+FP64 = 0
+FP32_BULK = 1
+
+_lib = None
+
+# every function the header declares, with (restype, argtypes)
+_vp, _i, _d, _l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+_PROTOS = {
+    "ldsf_last_error": (ctypes.c_char_p, []),
+    "ldsf_sf_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _i]),
+    "ldsf_sf_create_from_file": (_i, [_vp, ctypes.c_char_p]),
+    "ldsf_sf_destroy": (None, [_vp]),
+    "ldsf_sf_width": (_i, [_vp]),
+    "ldsf_sf_num_species": (_i, [_vp]),
+    "ldsf_sf_species_name": (ctypes.c_char_p, [_vp, _i]),
+    "ldsf_sf_set_precision": (_i, [_vp, _i]),
+    "ldsf_compute": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_compute_single_atom": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp]),
+    "ldsf_compute_batch": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_gradient_single_atom": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "ldsf_gradient_batch": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_jacobian": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_num_gradient_single_atom": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "ldsf_plan_create": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp]),
+    "ldsf_plan_create_batch": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "ldsf_plan_destroy": (None, [_vp]),
+    "ldsf_plan_num_centers": (_i, [_vp]),
+    "ldsf_plan_num_edges": (_l, [_vp]),
+    "ldsf_plan_max_neighbors": (_i, [_vp]),
+    "ldsf_plan_compute": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_gradient": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_kernel_family": (_i, [_vp]),
+    "ldsf_plan_set_kernel_family": (_i, [_vp, _i]),
+    "ldsf_nbl_create_paddings": (_i, [_i, _d, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_padding_count": (_i, [_vp]),
+    "ldsf_padding_copy": (_i, [_vp, _vp, _vp, _vp]),
+    "ldsf_padding_free": (None, [_vp]),
+    "ldsf_nbl_create": (_vp, []),
+    "ldsf_nbl_free": (None, [_vp]),
+    "ldsf_nbl_build": (_i, [_vp, _i, _vp, _d, _i, _vp, _vp]),
+    "ldsf_nbl_get_neigh": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp]),
+    "ldsf_nbl_num_lists": (_i, [_vp]),
+    "ldsf_nbl_list_cutoff": (_d, [_vp, _i]),
+    "ldsf_nbl_num_particles": (_i, [_vp]),
+    "ldsf_nbl_total": (_l, [_vp, _i, _i]),
+    "ldsf_nbl_flatten": (_i, [_vp, _i, _i, _vp, _vp]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libdescriptor_amd: libldsf.so is not built; run "
+                              "`python -m libdescriptor_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class LdsfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ldsf error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != LDSF_OK:
+        raise LdsfError(rc, lib().ldsf_last_error().decode(errors="replace"))
+
+
+def _ptr(t):
+    """Data pointer of a torch tensor or a numpy array (None -> NULL)."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+class Plan:
+    """A device plan (ldsf_plan) over device-resident int32 species,
+    neighbour list and neighbour counts.  Keeps references to the tensors it
+    borrows so they outlive the plan."""
+
+    def __init__(self, sf_handle, n_total, species, neighlist, numneigh, centers=None,
+                 n_atoms=None, config_ptr=None, stream=None):
+        import numpy as np
+        self._keep = (species, neighlist, numneigh, centers)
+        out = ctypes.c_void_p()
+        st = _stream_ptr(stream)
+        if n_atoms is not None:
+            na = np.ascontiguousarray(n_atoms, dtype=np.int32)
+            cp = np.ascontiguousarray(config_ptr, dtype=np.int32)
+            check(lib().ldsf_plan_create_batch(ctypes.byref(out), sf_handle, len(na), na.ctypes.data,
+                                               cp.ctypes.data, n_total, _ptr(species),
+                                               _ptr(neighlist), _ptr(numneigh), st))
+        else:
+            n_centers = int(numneigh.numel()) if hasattr(numneigh, "numel") else len(numneigh)
+        # the C-ABI trusts the neighbour list to hold sum(numneigh) entries
+        total = int(numneigh.sum()) if n_centers else 0
+        n_nl = int(neighlist.numel()) if hasattr(neighlist, "numel") else len(neighlist)
+        n_sp = int(species.numel()) if hasattr(species, "numel") else len(species)
+        if total > n_nl or n_sp < n_total:
+            raise ValueError("neighbour list shorter than sum(numneigh), or species shorter "
+                             "than n_total")
+        if n_atoms is None:
+            check(lib().ldsf_plan_create(ctypes.byref(out), sf_handle, n_centers, _ptr(centers),
+                                         n_total, _ptr(species), _ptr(neighlist), _ptr(numneigh),
+                                         st))
+        self.handle = out.value
+        self.n_total = n_total
+
+    @property
+    def n_centers(self):
+        return lib().ldsf_plan_num_centers(self.handle)
+
+    @property
+    def n_edges(self):
+        return lib().ldsf_plan_num_edges(self.handle)
+
+    @property
+    def max_neighbors(self):
+        return lib().ldsf_plan_max_neighbors(self.handle)
+
+    @property
+    def kernel_family(self):
+        return lib().ldsf_plan_kernel_family(self.handle)
+
+    @kernel_family.setter
+    def kernel_family(self, fam):
+        check(lib().ldsf_plan_set_kernel_family(self.handle, int(fam)))
+
+    def compute(self, coords, desc, precision=FP64, stream=None):
+        check(lib().ldsf_plan_compute(self.handle, _ptr(coords), _ptr(desc), int(precision),
+                                      _stream_ptr(stream)))
+
+    def gradient(self, coords, dE_dzeta, dcoords, precision=FP64, stream=None):
+        check(lib().ldsf_plan_gradient(self.handle, _ptr(coords), _ptr(dE_dzeta), _ptr(dcoords),
+                                       int(precision), _stream_ptr(stream)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.ldsf_plan_destroy(h)
+            self.handle = None
+
+
+def _stream_ptr(stream):
+    """hipStream_t of a torch stream (default: torch's current stream)."""
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return torch.cuda.current_stream().cuda_stream
+        except ImportError:
+            pass
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream

@@ -1,0 +1,60 @@
+"""PyTorch-ROCm autograd for the symmetry-function descriptor.
+
+``SymmetryFunctionsFn.apply(coords, plan, precision)`` returns the
+(n_centers, width) descriptor tensor; its backward is the hand-written HIP
+adjoint (d coords = J^T d desc), running on torch's current stream, with the
+padding rows of coords included as in the reference's gradient
+(python_bindings/python_bindings.cpp:235,252).  The same Function serves one
+configuration and a batch of uneven configurations (one persistent launch
+over all centres of a batch plan).
+"""
+import torch
+
+from . import _capi
+
+
+class SymmetryFunctionsFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, coords, plan, width, precision=_capi.FP64):
+        if coords.dtype != torch.float64 or not coords.is_cuda:
+            raise TypeError("coords must be a float64 tensor on the GPU")
+        coords = coords.contiguous()
+        desc = torch.zeros((plan.n_centers, width), dtype=torch.float64, device=coords.device)
+        plan.compute(coords, desc, precision)
+        ctx.save_for_backward(coords)
+        ctx.plan = plan
+        ctx.precision = precision
+        return desc
+
+    @staticmethod
+    def backward(ctx, grad_desc):
+        (coords,) = ctx.saved_tensors
+        g = grad_desc.to(torch.float64).contiguous()
+        dcoords = torch.zeros_like(coords)
+        ctx.plan.gradient(coords, g, dcoords, ctx.precision)
+        return dcoords, None, None, None
+
+
+class SymmetryFunctionsLayer(torch.nn.Module):
+    """Descriptor layer over a fixed neighbour list held on the device.
+
+    ``descriptor`` is a DescriptorKind from init_descriptor /
+    init_symmetry_functions.  For a batch pass ``n_atoms`` and ``config_ptr``
+    (host int arrays) as compute_batch takes them."""
+
+    def __init__(self, descriptor, species, neighlist, numneigh, n_atoms=None, config_ptr=None,
+                 precision=_capi.FP64, device="cuda"):
+        super().__init__()
+        dev = torch.device(device)
+        self.descriptor = descriptor
+        self.width = descriptor.width
+        self.precision = precision
+        self.species = torch.as_tensor(species, dtype=torch.int32).to(dev).contiguous()
+        self.neighlist = torch.as_tensor(neighlist, dtype=torch.int32).to(dev).contiguous()
+        self.numneigh = torch.as_tensor(numneigh, dtype=torch.int32).to(dev).contiguous()
+        self.plan = _capi.Plan(descriptor._handle, int(self.species.numel()), self.species,
+                               self.neighlist, self.numneigh, n_atoms=n_atoms,
+                               config_ptr=config_ptr)
+
+    def forward(self, coords):
+        return SymmetryFunctionsFn.apply(coords, self.plan, self.width, self.precision)

@@ -1,0 +1,672 @@
+// Host runtime of the C-ABI: per-device parameter tables, neighbour-list
+// plans, and the reference-signature host-pointer entry points, which stage
+// their arguments in device memory and run the same plans.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ldsf.h"
+#include "ldsf_internal.h"
+#include "sf_kernels.h"
+
+namespace {
+
+int hip_fail(hipError_t e, const char *where)
+{
+    return ldsf::fail(e == hipErrorOutOfMemory ? LDSF_ERR_NOMEM : LDSF_ERR_DEVICE,
+                      std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, where)                            \
+    do {                                                \
+        hipError_t e__ = (expr);                        \
+        if (e__ != hipSuccess) return hip_fail(e__, where); \
+    } while (0)
+
+// device buffer that frees itself
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void **)&p, sizeof(T) * count);
+    }
+    T *release_ownership()
+    {
+        T *q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
+};
+
+ldsf::WaveShape wave_shape_of(const ldsf_sf *sf, bool *ok)
+{
+    ldsf::WaveShape sh{};
+    const int ns = (int)sf->species.size();
+    sh.n2 = (int)sf->k2.size();
+    sh.n3 = (int)sf->k3.size();
+    sh.nsb = sf->species_resolved ? ns : 1;
+    sh.npb = sf->species_resolved ? ns * (ns + 1) / 2 : 1;
+    bool good = true;
+    for (int k : sf->k2) good = good && k == ldsf::G2;
+    for (int k : sf->k3) good = good && k == ldsf::G4;
+    // slots must be laid out block by block: one g2 block then one g4 block
+    if (sf->kind.size() > 2) good = false;
+    if (sf->kind.size() == 2 && !(sf->kind[0] == ldsf::G2 && sf->kind[1] == ldsf::G4)) good = false;
+    if (sf->kind.size() == 1 && sf->kind[0] != ldsf::G2 && sf->kind[0] != ldsf::G4) good = false;
+    *ok = good && ldsf::wave_supported(sh, ns);
+    return sh;
+}
+
+}  // namespace
+
+ldsf_sf::~ldsf_sf()
+{
+    for (auto &d : dev) {
+        if (d.mem) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(d.device);
+            (void)hipFree(d.mem);
+            (void)hipSetDevice(cur);
+        }
+    }
+}
+
+namespace ldsf {
+
+int device_table(const ldsf_sf *csf, SFTab *out)
+{
+    auto *sf = const_cast<ldsf_sf *>(csf);
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    for (auto &d : sf->dev)
+        if (d.device == device) {
+            *out = d.tab;
+            return LDSF_OK;
+        }
+    // one allocation: doubles first, then ints, each array 8-byte aligned
+    const int ns = (int)sf->species.size();
+    const size_t n2 = sf->k2.size(), n3 = sf->k3.size();
+    const size_t nd = ns * ns + 2 * n2 + 4 * n3;
+    const size_t ni = 3 * n2 + 4 * n3;
+    std::vector<double> hd;
+    hd.reserve(nd);
+    std::vector<int> hi;
+    hi.reserve(ni + 1);
+    auto put_d = [&](const std::vector<double> &v) { size_t o = hd.size(); hd.insert(hd.end(), v.begin(), v.end()); return o; };
+    auto put_i = [&](const std::vector<int> &v) { size_t o = hi.size(); hi.insert(hi.end(), v.begin(), v.end()); return o; };
+    size_t o_rc = put_d(sf->rcut), o_a2 = put_d(sf->a2), o_b2 = put_d(sf->b2), o_z3 = put_d(sf->zeta3),
+           o_l3 = put_d(sf->lam3), o_e3 = put_d(sf->eta3), o_c3 = put_d(sf->coef3);
+    size_t o_k2 = put_i(sf->k2), o_s2 = put_i(sf->slot2), o_t2 = put_i(sf->str2), o_k3 = put_i(sf->k3),
+           o_iz = put_i(sf->izeta3), o_s3 = put_i(sf->slot3), o_t3 = put_i(sf->str3);
+    size_t bytes = sizeof(double) * (hd.size() + 1) + sizeof(int) * (hi.size() + 2);
+    void *mem = nullptr;
+    HIP_TRY(hipMalloc(&mem, bytes), "hipMalloc(parameter table)");
+    double *dd = (double *)mem;
+    int *di = (int *)(dd + hd.size() + 1);
+    hipError_t e = hipSuccess;
+    if (!hd.empty()) e = hipMemcpy(dd, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hi.empty()) e = hipMemcpy(di, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(mem);
+        return hip_fail(e, "upload parameter table");
+    }
+    SFTab t{};
+    t.n_species = ns;
+    t.width = sf->width;
+    t.n2 = (int)n2;
+    t.n3 = (int)n3;
+    t.species_resolved = sf->species_resolved;
+    t.rc = dd + o_rc;
+    t.a2 = dd + o_a2;
+    t.b2 = dd + o_b2;
+    t.zeta3 = dd + o_z3;
+    t.lam3 = dd + o_l3;
+    t.eta3 = dd + o_e3;
+    t.coef3 = dd + o_c3;
+    t.k2 = di + o_k2;
+    t.slot2 = di + o_s2;
+    t.str2 = di + o_t2;
+    t.k3 = di + o_k3;
+    t.izeta3 = di + o_iz;
+    t.slot3 = di + o_s3;
+    t.str3 = di + o_t3;
+    ldsf_sf::Dev d;
+    d.device = device;
+    d.mem = mem;
+    d.tab = t;
+    sf->dev.push_back(d);
+    *out = t;
+    return LDSF_OK;
+}
+
+}  // namespace ldsf
+
+struct ldsf_plan {
+    const ldsf_sf *sf = nullptr;
+    int device = 0;
+    int n_centers = 0;
+    int n_total = 0;
+    long n_edges = 0;
+    int max_nn = 0;
+    int family = 0;
+    bool wave_ok = false;
+    ldsf::WaveShape shape{};
+    const int *species = nullptr;  // borrowed
+    const int *nl = nullptr;       // borrowed
+    int *centers = nullptr;        // owned (NULL = identity)
+    int *offs = nullptr;
+    int *in_start = nullptr;
+    int *in_edges = nullptr;
+    double *edge = nullptr;
+    double *self = nullptr;
+
+    ~ldsf_plan()
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(device);
+        for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
+                        (void *)edge, (void *)self})
+            if (p) (void)hipFree(p);
+        (void)hipSetDevice(cur);
+    }
+};
+
+namespace {
+
+int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_centers, int n_total,
+               const int *d_species, const int *d_nl, const int *d_numneigh, hipStream_t s)
+{
+    DevBuf<int> centers_guard;
+    centers_guard.p = owned_centers;
+    if (!sf) return ldsf::fail(LDSF_ERR_ARG, "plan_create: null descriptor");
+    if (n_centers < 0 || n_total < 0 || (n_total > 0 && !d_species) ||
+        (n_centers > 0 && !d_numneigh))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_create: bad arguments");
+    ldsf::SFTab tab;
+    if (int rc = ldsf::device_table(sf, &tab)) return rc;
+    auto plan = new ldsf_plan;
+    std::unique_ptr<ldsf_plan> guard(plan);
+    HIP_TRY(hipGetDevice(&plan->device), "hipGetDevice");
+    plan->sf = sf;
+    plan->n_centers = n_centers;
+    plan->n_total = n_total;
+    plan->species = d_species;
+    plan->nl = d_nl;
+    plan->centers = centers_guard.release_ownership();
+
+    DevBuf<int> offs, scratch_i, count;
+    HIP_TRY(offs.alloc(n_centers + 1), "hipMalloc(offsets)");
+    HIP_TRY(scratch_i.alloc(4), "hipMalloc(flags)");
+    HIP_TRY(hipMemsetAsync(scratch_i.p, 0, sizeof(int) * 4, s), "hipMemsetAsync");
+    HIP_TRY(ldsf::launch_plan_validate(n_centers, plan->centers, n_total, d_numneigh, d_species,
+                                       tab.n_species, scratch_i.p, s), "validate");
+    size_t tmp_bytes = 0, tmp2 = 0;
+    HIP_TRY(ldsf::launch_scan_int(n_centers, d_numneigh, offs.p, nullptr, &tmp_bytes, s), "scan size");
+    HIP_TRY(ldsf::launch_scan_int(n_total, d_numneigh, offs.p, nullptr, &tmp2, s), "scan size");
+    tmp_bytes = std::max(tmp_bytes, tmp2) + 256;
+    DevBuf<char> tmp;
+    HIP_TRY(tmp.alloc(tmp_bytes), "hipMalloc(scan scratch)");
+    // a negative count would make the scan meaningless; it is caught below
+    HIP_TRY(ldsf::launch_scan_int(n_centers, d_numneigh, offs.p, tmp.p, &tmp_bytes, s), "scan");
+    HIP_TRY(ldsf::launch_max_int(n_centers, d_numneigh, scratch_i.p + 1, s), "max");
+    int host_flags[4] = {0, 0, 0, 0};
+    int total = 0;
+    HIP_TRY(hipMemcpyAsync(host_flags, scratch_i.p, sizeof(int) * 4, hipMemcpyDeviceToHost, s), "copy flags");
+    HIP_TRY(hipMemcpyAsync(&total, offs.p + n_centers, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (host_flags[0] & 2) return ldsf::fail(LDSF_ERR_ARG, "negative number_of_neighs entry");
+    if (host_flags[0] & 8) return ldsf::fail(LDSF_ERR_ARG, "centre row out of range");
+    if (host_flags[0] & 4) return ldsf::fail(LDSF_ERR_ARG, "species code out of range");
+    if (total < 0) return ldsf::fail(LDSF_ERR_UNSUPPORTED, "more than 2^31-1 neighbour-list entries");
+    if (total > 0 && !d_nl) return ldsf::fail(LDSF_ERR_ARG, "plan_create: null neighbour list");
+    plan->n_edges = total;
+    plan->max_nn = host_flags[1];
+
+    // transposed list: count, scan, scatter, sort buckets
+    HIP_TRY(count.alloc(n_total + 1), "hipMalloc(counts)");
+    HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
+    HIP_TRY(hipMemsetAsync(scratch_i.p, 0, sizeof(int), s), "hipMemsetAsync");
+    HIP_TRY(ldsf::launch_edge_targets_count(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
+                                            count.p, scratch_i.p, s), "count");
+    HIP_TRY(hipMemcpyAsync(host_flags, scratch_i.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flags");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (host_flags[0] & 1) return ldsf::fail(LDSF_ERR_ARG, "neighbour index out of range [0, n_total)");
+    DevBuf<int> in_start, in_edges;
+    HIP_TRY(in_start.alloc(n_total + 1), "hipMalloc(in_start)");
+    HIP_TRY(in_edges.alloc(plan->n_edges + n_centers), "hipMalloc(in_edges)");
+    HIP_TRY(ldsf::launch_scan_int(n_total, count.p, in_start.p, tmp.p, &tmp_bytes, s), "scan");
+    HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
+    HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, in_start.p,
+                                           count.p, in_edges.p, s), "fill");
+    HIP_TRY(ldsf::launch_sort_buckets(n_total, in_start.p, in_edges.p, s), "sort");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    plan->offs = offs.release_ownership();
+    plan->in_start = in_start.release_ownership();
+    plan->in_edges = in_edges.release_ownership();
+    plan->shape = wave_shape_of(sf, &plan->wave_ok);
+    plan->family = plan->wave_ok ? 1 : 0;
+    *out = guard.release();
+    return LDSF_OK;
+}
+
+int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, hipStream_t s)
+{
+    ldsf::SFTab tab;
+    if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
+    hipError_t e;
+    if (plan->family == 1 && plan->wave_ok)
+        e = ldsf::launch_fwd_wave(tab, plan->shape, precision, plan->n_centers, plan->centers,
+                                  plan->offs, plan->nl, plan->species, x, desc, plan->max_nn, s);
+    else
+        e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
+                                     plan->nl, plan->species, x, desc, s);
+    HIP_TRY(e, "forward kernel launch");
+    return LDSF_OK;
+}
+
+int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dcoords,
+                  int precision, hipStream_t s)
+{
+    ldsf::SFTab tab;
+    if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
+    if (!plan->edge && plan->n_edges > 0)
+        HIP_TRY(hipMalloc((void **)&plan->edge, sizeof(double) * 3 * plan->n_edges), "hipMalloc(edge)");
+    if (!plan->self && plan->n_centers > 0)
+        HIP_TRY(hipMalloc((void **)&plan->self, sizeof(double) * 3 * plan->n_centers), "hipMalloc(self)");
+    hipError_t e;
+    if (plan->family == 1 && plan->wave_ok)
+        e = ldsf::launch_adj_wave(tab, plan->shape, precision, plan->n_centers, plan->centers,
+                                  plan->offs, plan->nl, plan->species, x, dE, plan->edge,
+                                  plan->self, plan->max_nn, s);
+    else
+        e = ldsf::launch_adj_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
+                                     plan->nl, plan->species, x, dE, plan->edge, plan->self, s);
+    HIP_TRY(e, "adjoint kernel launch");
+    HIP_TRY(ldsf::launch_gather(plan->n_total, plan->in_start, plan->in_edges, plan->n_edges,
+                                plan->edge, plan->self, dcoords, s), "gather launch");
+    return LDSF_OK;
+}
+
+// ---- host-pointer entry points: stage, run a plan, copy back ----
+
+struct HostCall {
+    const ldsf_sf *sf;
+    int n_centers;
+    std::vector<int> centers;  // empty = identity
+    const int *species;
+    const int *nl;
+    const int *numneigh;
+    const double *coords;
+    int n_rows = 0;
+    long n_edges = 0;
+};
+
+int host_prepare(HostCall &c)
+{
+    if (!c.sf) return ldsf::fail(LDSF_ERR_ARG, "null descriptor");
+    if (c.n_centers < 0) return ldsf::fail(LDSF_ERR_ARG, "negative atom count");
+    if (c.n_centers > 0 && (!c.species || !c.numneigh || !c.coords))
+        return ldsf::fail(LDSF_ERR_ARG, "null input array");
+    long total = 0;
+    int rows = 0;
+    for (int t = 0; t < c.n_centers; ++t) {
+        if (c.numneigh[t] < 0) return ldsf::fail(LDSF_ERR_ARG, "negative number_of_neighs entry");
+        total += c.numneigh[t];
+        int ctr = c.centers.empty() ? t : c.centers[t];
+        if (ctr < 0) return ldsf::fail(LDSF_ERR_ARG, "negative centre index");
+        rows = std::max(rows, ctr + 1);
+    }
+    if (total > 0 && !c.nl) return ldsf::fail(LDSF_ERR_ARG, "null neighbour list");
+    if (total > 0x7fffffffL) return ldsf::fail(LDSF_ERR_UNSUPPORTED, "more than 2^31-1 neighbour-list entries");
+    for (long e = 0; e < total; ++e) {
+        if (c.nl[e] < 0) return ldsf::fail(LDSF_ERR_ARG, "negative neighbour index");
+        rows = std::max(rows, c.nl[e] + 1);
+    }
+    c.n_rows = rows;
+    c.n_edges = total;
+    return LDSF_OK;
+}
+
+// desc_io: [n_centers][width] (+=) or NULL; dE + dcoords_io for the adjoint
+int host_run(HostCall &c, double *desc_io, const double *dE, double *dcoords_io, int precision)
+{
+    if (int rc = host_prepare(c)) return rc;
+    if (c.n_centers == 0) return LDSF_OK;
+    const int W = c.sf->width;
+    hipStream_t s = nullptr;
+    DevBuf<int> d_species, d_nl, d_nn, d_centers;
+    DevBuf<double> d_x, d_desc, d_dE, d_dx;
+    HIP_TRY(d_species.alloc(c.n_rows), "hipMalloc");
+    HIP_TRY(d_x.alloc(3 * (size_t)c.n_rows), "hipMalloc");
+    HIP_TRY(d_nl.alloc(c.n_edges), "hipMalloc");
+    HIP_TRY(d_nn.alloc(c.n_centers), "hipMalloc");
+    HIP_TRY(hipMemcpy(d_species.p, c.species, sizeof(int) * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_x.p, c.coords, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+    if (c.n_edges)
+        HIP_TRY(hipMemcpy(d_nl.p, c.nl, sizeof(int) * c.n_edges, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_nn.p, c.numneigh, sizeof(int) * c.n_centers, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!c.centers.empty()) {
+        HIP_TRY(d_centers.alloc(c.n_centers), "hipMalloc");
+        HIP_TRY(hipMemcpy(d_centers.p, c.centers.data(), sizeof(int) * c.n_centers,
+                          hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    ldsf_plan *plan = nullptr;
+    int rc = plan_build(&plan, c.sf, c.n_centers, d_centers.release_ownership(), c.n_rows,
+                        d_species.p, d_nl.p, d_nn.p, s);
+    if (rc) return rc;
+    std::unique_ptr<ldsf_plan> guard(plan);
+    if (desc_io) {
+        HIP_TRY(d_desc.alloc((size_t)c.n_centers * W), "hipMalloc");
+        HIP_TRY(hipMemcpy(d_desc.p, desc_io, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
+        if ((rc = plan_compute(plan, d_x.p, d_desc.p, precision, s))) return rc;
+        HIP_TRY(hipMemcpy(desc_io, d_desc.p, sizeof(double) * c.n_centers * W, hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    if (dE) {
+        HIP_TRY(d_dE.alloc((size_t)c.n_centers * W), "hipMalloc");
+        HIP_TRY(d_dx.alloc(3 * (size_t)c.n_rows), "hipMalloc");
+        HIP_TRY(hipMemcpy(d_dE.p, dE, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
+        HIP_TRY(hipMemcpy(d_dx.p, dcoords_io, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+        if ((rc = plan_gradient(plan, d_x.p, d_dE.p, d_dx.p, precision, s))) return rc;
+        HIP_TRY(hipMemcpy(dcoords_io, d_dx.p, sizeof(double) * 3 * c.n_rows, hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return LDSF_OK;
+}
+
+std::vector<int> batch_centers(int n_conf, const int *n_atoms, const int *ptr, int *n_centers, int *rc)
+{
+    std::vector<int> c;
+    *rc = LDSF_OK;
+    long total = 0;
+    for (int k = 0; k < n_conf; ++k) {
+        if (n_atoms[k] < 0) { *rc = ldsf::fail(LDSF_ERR_ARG, "negative n_atoms entry"); return c; }
+        total += n_atoms[k];
+    }
+    if (total > 0x7fffffffL) { *rc = ldsf::fail(LDSF_ERR_UNSUPPORTED, "too many centres"); return c; }
+    c.reserve(total);
+    for (int k = 0; k < n_conf; ++k)
+        for (int j = 0; j < n_atoms[k]; ++j) c.push_back(j + ptr[k]);
+    *n_centers = (int)total;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+LDSF_API void ldsf_sf_destroy(ldsf_sf *sf) { delete sf; }
+
+LDSF_API int ldsf_plan_create(ldsf_plan **out, const ldsf_sf *sf, int n_centers,
+                              const int *d_centers, int n_total, const int *d_species,
+                              const int *d_neighbor_list, const int *d_number_of_neighs,
+                              void *stream)
+{
+    if (!out) return ldsf::fail(LDSF_ERR_ARG, "plan_create: null output");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    int *owned = nullptr;
+    if (d_centers && n_centers > 0) {
+        HIP_TRY(hipMalloc((void **)&owned, sizeof(int) * n_centers), "hipMalloc(centres)");
+        hipError_t e = hipMemcpyAsync(owned, d_centers, sizeof(int) * n_centers,
+                                      hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) { (void)hipFree(owned); return hip_fail(e, "copy centres"); }
+    }
+    return plan_build(out, sf, n_centers, owned, n_total, d_species, d_neighbor_list,
+                      d_number_of_neighs, s);
+}
+
+LDSF_API int ldsf_plan_create_batch(ldsf_plan **out, const ldsf_sf *sf, int n_configurations,
+                                    const int *n_atoms, const int *config_ptr, int n_total,
+                                    const int *d_species, const int *d_neighbor_list,
+                                    const int *d_number_of_neighs, void *stream)
+{
+    if (!out) return ldsf::fail(LDSF_ERR_ARG, "plan_create_batch: null output");
+    *out = nullptr;
+    if (n_configurations < 0 || (n_configurations > 0 && (!n_atoms || !config_ptr)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_create_batch: bad arguments");
+    int n_centers = 0, rc = 0;
+    std::vector<int> c = batch_centers(n_configurations, n_atoms, config_ptr, &n_centers, &rc);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    int *owned = nullptr;
+    if (n_centers > 0) {
+        HIP_TRY(hipMalloc((void **)&owned, sizeof(int) * n_centers), "hipMalloc(centres)");
+        hipError_t e = hipMemcpy(owned, c.data(), sizeof(int) * n_centers, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { (void)hipFree(owned); return hip_fail(e, "copy centres"); }
+    }
+    return plan_build(out, sf, n_centers, owned, n_total, d_species, d_neighbor_list,
+                      d_number_of_neighs, s);
+}
+
+LDSF_API void ldsf_plan_destroy(ldsf_plan *plan) { delete plan; }
+
+LDSF_API int ldsf_plan_num_centers(const ldsf_plan *plan) { return plan ? plan->n_centers : 0; }
+
+LDSF_API long ldsf_plan_num_edges(const ldsf_plan *plan) { return plan ? plan->n_edges : 0; }
+
+LDSF_API int ldsf_plan_max_neighbors(const ldsf_plan *plan) { return plan ? plan->max_nn : 0; }
+
+LDSF_API int ldsf_plan_kernel_family(const ldsf_plan *plan) { return plan ? plan->family : -1; }
+
+LDSF_API int ldsf_plan_set_kernel_family(ldsf_plan *plan, int family)
+{
+    if (!plan || (family != 0 && family != 1)) return ldsf::fail(LDSF_ERR_ARG, "set_kernel_family: bad arguments");
+    if (family == 1 && !plan->wave_ok)
+        return ldsf::fail(LDSF_ERR_UNSUPPORTED, "wave kernels do not cover this parameter set");
+    plan->family = family;
+    return LDSF_OK;
+}
+
+LDSF_API int ldsf_plan_compute(ldsf_plan *plan, const double *d_coordinates, double *d_desc,
+                               int precision, void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_desc)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_compute: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_compute: unknown precision");
+    return plan_compute(plan, d_coordinates, d_desc, precision, (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_plan_gradient(ldsf_plan *plan, const double *d_coordinates,
+                                const double *d_dE_dzeta, double *d_dcoords, int precision,
+                                void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta || !d_dcoords)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gradient: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gradient: unknown precision");
+    return plan_gradient(plan, d_coordinates, d_dE_dzeta, d_dcoords, precision, (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_compute(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
+                          const int *number_of_neighs, const double *coordinates, double *desc)
+{
+    if (!sf || (n_atoms > 0 && !desc)) return ldsf::fail(LDSF_ERR_ARG, "compute: bad arguments");
+    HostCall c{sf, n_atoms, {}, species, neighbor_list, number_of_neighs, coordinates};
+    return host_run(c, desc, nullptr, nullptr, sf->precision);
+}
+
+LDSF_API int ldsf_compute_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
+                                      const int *neighbor_list, int number_of_neighs,
+                                      const double *coordinates, double *desc)
+{
+    (void)n_atoms;
+    if (!sf || !desc || index < 0) return ldsf::fail(LDSF_ERR_ARG, "compute_single_atom: bad arguments");
+    int nn = number_of_neighs;
+    HostCall c{sf, 1, {index}, species, neighbor_list, &nn, coordinates};
+    return host_run(c, desc, nullptr, nullptr, sf->precision);
+}
+
+LDSF_API int ldsf_compute_batch(ldsf_sf *sf, int n_configurations, const int *n_atoms,
+                                const int *config_ptr, const int *species, const int *neighbor_list,
+                                const int *number_of_neighs, const double *coordinates, double *desc)
+{
+    if (!sf || n_configurations < 0 || (n_configurations > 0 && (!n_atoms || !config_ptr)))
+        return ldsf::fail(LDSF_ERR_ARG, "compute_batch: bad arguments");
+    int n = 0, rc = 0;
+    std::vector<int> ctr = batch_centers(n_configurations, n_atoms, config_ptr, &n, &rc);
+    if (rc) return rc;
+    if (n > 0 && !desc) return ldsf::fail(LDSF_ERR_ARG, "compute_batch: null desc");
+    HostCall c{sf, n, std::move(ctr), species, neighbor_list, number_of_neighs, coordinates};
+    return host_run(c, desc, nullptr, nullptr, sf->precision);
+}
+
+LDSF_API int ldsf_gradient(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
+                           const int *number_of_neighs, const double *coordinates,
+                           double *d_coordinates, double *desc, const double *dE_dzeta)
+{
+    (void)desc;
+    if (!sf || (n_atoms > 0 && (!d_coordinates || !dE_dzeta)))
+        return ldsf::fail(LDSF_ERR_ARG, "gradient: bad arguments");
+    HostCall c{sf, n_atoms, {}, species, neighbor_list, number_of_neighs, coordinates};
+    return host_run(c, nullptr, dE_dzeta, d_coordinates, sf->precision);
+}
+
+LDSF_API int ldsf_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
+                                       const int *neighbor_list, int number_of_neighs,
+                                       const double *coordinates, double *d_coordinates,
+                                       double *desc, const double *dE_dzeta)
+{
+    (void)n_atoms;
+    (void)desc;
+    if (!sf || !d_coordinates || !dE_dzeta || index < 0)
+        return ldsf::fail(LDSF_ERR_ARG, "gradient_single_atom: bad arguments");
+    int nn = number_of_neighs;
+    HostCall c{sf, 1, {index}, species, neighbor_list, &nn, coordinates};
+    return host_run(c, nullptr, dE_dzeta, d_coordinates, sf->precision);
+}
+
+LDSF_API int ldsf_gradient_batch(ldsf_sf *sf, int n_configurations, const int *n_atoms,
+                                 const int *config_ptr, const int *species,
+                                 const int *neighbor_list, const int *number_of_neighs,
+                                 const double *coordinates, double *d_coordinates, double *desc,
+                                 const double *dE_dzeta)
+{
+    (void)desc;
+    if (!sf || n_configurations < 0 || (n_configurations > 0 && (!n_atoms || !config_ptr)))
+        return ldsf::fail(LDSF_ERR_ARG, "gradient_batch: bad arguments");
+    int n = 0, rc = 0;
+    std::vector<int> ctr = batch_centers(n_configurations, n_atoms, config_ptr, &n, &rc);
+    if (rc) return rc;
+    if (n > 0 && (!d_coordinates || !dE_dzeta)) return ldsf::fail(LDSF_ERR_ARG, "gradient_batch: null output");
+    HostCall c{sf, n, std::move(ctr), species, neighbor_list, number_of_neighs, coordinates};
+    return host_run(c, nullptr, dE_dzeta, d_coordinates, sf->precision);
+}
+
+LDSF_API int ldsf_jacobian(ldsf_sf *sf, int n_atoms, int n_total_atoms, const int *species,
+                           const int *neighbor_list, const int *number_of_neighs,
+                           const double *coordinates, double *J_coordinates)
+{
+    if (!sf || n_atoms < 0 || n_total_atoms < 0 || (n_atoms > 0 && !J_coordinates))
+        return ldsf::fail(LDSF_ERR_ARG, "jacobian: bad arguments");
+    HostCall c{sf, n_atoms, {}, species, neighbor_list, number_of_neighs, coordinates};
+    if (int rc = host_prepare(c)) return rc;
+    if (n_atoms == 0) return LDSF_OK;
+    if (c.n_rows > n_total_atoms) return ldsf::fail(LDSF_ERR_ARG, "jacobian: neighbour index >= n_total_atoms");
+    const int W = sf->width;
+    const size_t rows = (size_t)W * n_atoms;
+    const size_t cols = 3 * (size_t)n_total_atoms;
+    DevBuf<int> d_species, d_nl, d_nn;
+    DevBuf<double> d_x, d_g, d_J;
+    HIP_TRY(d_species.alloc(c.n_rows), "hipMalloc");
+    HIP_TRY(d_x.alloc(3 * (size_t)c.n_rows), "hipMalloc");
+    HIP_TRY(d_nl.alloc(c.n_edges), "hipMalloc");
+    HIP_TRY(d_nn.alloc(n_atoms), "hipMalloc");
+    HIP_TRY(hipMemcpy(d_species.p, species, sizeof(int) * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_x.p, coordinates, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+    if (c.n_edges)
+        HIP_TRY(hipMemcpy(d_nl.p, neighbor_list, sizeof(int) * c.n_edges, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_nn.p, number_of_neighs, sizeof(int) * n_atoms, hipMemcpyHostToDevice), "hipMemcpy");
+    ldsf_plan *plan = nullptr;
+    int rc = plan_build(&plan, sf, n_atoms, nullptr, c.n_rows, d_species.p, d_nl.p, d_nn.p, nullptr);
+    if (rc) return rc;
+    std::unique_ptr<ldsf_plan> guard(plan);
+    // one-hot VJPs, one per descriptor entry (Descriptors.cpp:467-483), each
+    // into its own row of J (whose first n_rows*3 columns can be non-zero)
+    HIP_TRY(d_g.alloc(rows), "hipMalloc");
+    HIP_TRY(d_J.alloc(3 * (size_t)c.n_rows), "hipMalloc");
+    HIP_TRY(hipMemset(d_g.p, 0, sizeof(double) * rows), "hipMemset");
+    std::vector<double> row(3 * (size_t)c.n_rows);
+    const double one = 1.0, zero = 0.0;
+    for (size_t j = 0; j < rows; ++j) {
+        if (j > 0) HIP_TRY(hipMemcpy(d_g.p + j - 1, &zero, sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        HIP_TRY(hipMemcpy(d_g.p + j, &one, sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        double *Jrow = J_coordinates + j * cols;
+        HIP_TRY(hipMemcpy(d_J.p, Jrow, sizeof(double) * row.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        if ((rc = plan_gradient(plan, d_x.p, d_g.p, d_J.p, sf->precision, nullptr))) return rc;
+        HIP_TRY(hipMemcpy(Jrow, d_J.p, sizeof(double) * row.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    return LDSF_OK;
+}
+
+// numdiff::make_xph_representable (include/finite_difference.hpp:46-57)
+static double make_xph_representable(double x, double h)
+{
+    volatile double temp = x + h;
+    h = temp - x;
+    if (h == 0) h = std::nextafter(x, std::numeric_limits<double>::max()) - x;
+    return h;
+}
+
+LDSF_API int ldsf_num_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
+                                           const int *neighbor_list, int number_of_neighs,
+                                           const double *coordinates, double *d_coordinates,
+                                           const double *dE_dzeta)
+{
+    if (!sf || !coordinates || !d_coordinates || !dE_dzeta || index < 0 || index >= n_atoms)
+        return ldsf::fail(LDSF_ERR_ARG, "num_gradient_single_atom: bad arguments");
+    const int W = sf->width;
+    std::vector<double> x(coordinates, coordinates + 3 * (size_t)n_atoms);
+    std::vector<double> y(8 * (size_t)W);
+    static const int mult[8] = {1, -1, -2, 2, 3, -3, -4, 4};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int pos = index * 3 + dim;
+        double h = std::pow(551.25 * std::numeric_limits<double>::epsilon(), (double)1 / (double)9);
+        h = make_xph_representable(coordinates[pos], h);
+        for (int e = 0; e < 8; ++e) {
+            std::fill(y.begin() + e * W, y.begin() + (e + 1) * W, 0.0);
+            x[pos] = coordinates[pos] + mult[e] * h;
+            if (int rc = ldsf_compute_single_atom(sf, index, n_atoms, species, neighbor_list,
+                                                  number_of_neighs, x.data(), y.data() + e * W))
+                return rc;
+        }
+        x[pos] = coordinates[pos];
+        // stencil of finite_difference.hpp:120-128
+        for (int w = 0; w < W; ++w) {
+            double y1 = y[0 * W + w] - y[1 * W + w];
+            double y2 = y[2 * W + w] - y[3 * W + w];
+            double y3 = y[4 * W + w] - y[5 * W + w];
+            double y4 = y[6 * W + w] - y[7 * W + w];
+            double t1 = 3 * y4 / 8 + 4 * y3;
+            double t2 = 21 * y2 + 84 * y1;
+            d_coordinates[dim] += dE_dzeta[w] * ((t1 + t2) / (105 * h));
+        }
+    }
+    return LDSF_OK;
+}
+
+}  // extern "C"

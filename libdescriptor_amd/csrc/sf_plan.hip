@@ -1,0 +1,196 @@
+// Neighbour-list plans and the atomic-free adjoint gather.
+//
+// The adjoint kernels write, for every edge e of the CSR neighbour list,
+// the 3-vector dE/d r_{nl[e]} contributed by that edge's centre, and for
+// every centre t the vector dE/d r_{centre t}.  Summing these per atom needs
+// the transposed list: for each coordinate row a, the ids of the edges that
+// point at it.  It is built once per plan with an integer count, a prefix
+// sum and a scatter whose buckets are then sorted, so the gather visits
+// every row's edges in ascending id order and the result is bitwise the same
+// on every run and in every batch.  Centre self-terms are edges too (ids
+// n_edges + t), so one gather writes d_coordinates, padding rows included
+// (the layout the reference's gradient returns, python_bindings.cpp:235,252).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "sf_kernels.h"
+
+namespace ldsf {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline dim3 grid_for(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+// flags: 1 neighbour row out of range, 2 negative neighbour count,
+//        4 species code out of range, 8 centre row out of range
+__global__ void k_validate(int n_centers, const int *__restrict__ centers, int n_total,
+                           const int *__restrict__ numneigh, const int *__restrict__ species,
+                           int n_species, int *flags)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_centers) {
+        if (numneigh[t] < 0) atomicOr(flags, 2);
+        int c = centers ? centers[t] : (int)t;
+        if (c < 0 || c >= n_total) atomicOr(flags, 8);
+    }
+    if (t < n_total) {
+        int s = species[t];
+        if (s < 0 || s >= n_species) atomicOr(flags, 4);
+    }
+}
+
+__device__ __forceinline__ int edge_target(long e, long n_edges, const int *nl, const int *centers)
+{
+    if (e < n_edges) return nl[e];
+    int t = (int)(e - n_edges);
+    return centers ? centers[t] : t;
+}
+
+__global__ void k_count(long n_edges, int n_centers, const int *__restrict__ nl,
+                        const int *__restrict__ centers, int n_total, int *count, int *flags)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges + n_centers) return;
+    int a = edge_target(e, n_edges, nl, centers);
+    if (a < 0 || a >= n_total) {
+        atomicOr(flags, 1);
+        return;
+    }
+    atomicAdd(count + a, 1);
+}
+
+__global__ void k_fill(long n_edges, int n_centers, const int *__restrict__ nl,
+                       const int *__restrict__ centers, const int *__restrict__ in_start,
+                       int *cursor, int *in_edges)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges + n_centers) return;
+    int a = edge_target(e, n_edges, nl, centers);
+    int pos = atomicAdd(cursor + a, 1);
+    in_edges[in_start[a] + pos] = (int)e;
+}
+
+// insertion sort of each row's bucket (buckets hold a few tens of ids)
+__global__ void k_sort(int n_total, const int *__restrict__ in_start, int *in_edges)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_total) return;
+    int b = in_start[a], e = in_start[a + 1];
+    for (int p = b + 1; p < e; ++p) {
+        int v = in_edges[p];
+        int q = p - 1;
+        while (q >= b && in_edges[q] > v) {
+            in_edges[q + 1] = in_edges[q];
+            --q;
+        }
+        in_edges[q + 1] = v;
+    }
+}
+
+__global__ void k_gather(int n_total, const int *__restrict__ in_start,
+                         const int *__restrict__ in_edges, long n_edges,
+                         const double *__restrict__ edge, const double *__restrict__ self,
+                         double *__restrict__ dcoords)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_total) return;
+    int b = in_start[a], e = in_start[a + 1];
+    if (b == e) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int p = b; p < e; ++p) {
+        long id = in_edges[p];
+        const double *v = id < n_edges ? edge + 3 * id : self + 3 * (id - n_edges);
+        s0 += v[0];
+        s1 += v[1];
+        s2 += v[2];
+    }
+    dcoords[3 * (long)a] += s0;
+    dcoords[3 * (long)a + 1] += s1;
+    dcoords[3 * (long)a + 2] += s2;
+}
+
+__global__ void k_max(int n, const int *__restrict__ v, int *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = 0;
+    for (; i < n; i += gridDim.x * blockDim.x) m = max(m, v[i]);
+    // wave reduction, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+}  // namespace
+
+hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,
+                                const int *species, int n_species, int *flags, hipStream_t s)
+{
+    long n = n_centers > n_total ? n_centers : n_total;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_validate, grid_for(n), dim3(kBlock), 0, s, n_centers, centers, n_total,
+                       numneigh, species, n_species, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_targets_count(long n_edges, int n_centers, const int *nl, const int *centers,
+                                     int n_total, int *count, int *flags, hipStream_t s)
+{
+    long n = n_edges + n_centers;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count, grid_for(n), dim3(kBlock), 0, s, n_edges, n_centers, nl, centers,
+                       n_total, count, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, const int *centers,
+                                    const int *in_start, int *cursor, int *in_edges, hipStream_t s)
+{
+    long n = n_edges + n_centers;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, s, n_edges, n_centers, nl, centers,
+                       in_start, cursor, in_edges);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_buckets(int n_total, const int *in_start, int *in_edges, hipStream_t s)
+{
+    if (n_total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start, in_edges);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(int n_total, const int *in_start, const int *in_edges, long n_edges,
+                         const double *edge, const double *self, double *dcoords, hipStream_t s)
+{
+    if (n_total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start,
+                       in_edges, n_edges, edge, self, dcoords);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_max, dim3((unsigned)blocks), dim3(kBlock), 0, s, n, v, out);
+    return hipGetLastError();
+}
+
+// out[0] = 0, out[k] = in[0] + ... + in[k-1] for k = 1..n.  With tmp == NULL
+// only reports the scratch size in *tmp_bytes.
+hipError_t launch_scan_int(int n, const int *in, int *out, void *tmp, size_t *tmp_bytes,
+                           hipStream_t s)
+{
+    if (!tmp) {
+        *tmp_bytes = 0;
+        if (n <= 0) return hipSuccess;
+        return hipcub::DeviceScan::InclusiveSum(nullptr, *tmp_bytes, in, out + 1, n, s);
+    }
+    hipError_t err = hipMemsetAsync(out, 0, sizeof(int), s);
+    if (err != hipSuccess || n <= 0) return err;
+    return hipcub::DeviceScan::InclusiveSum(tmp, *tmp_bytes, in, out + 1, n, s);
+}
+
+}  // namespace ldsf

@@ -1,0 +1,131 @@
+"""Write the golden fixtures under tests/golden/ from the reference's own code,
+compiled from its sources into oracle/_ref/libsfref.so (oracle/build_ref.sh).
+
+Fixtures (data only: inputs and the reference's outputs):
+  c1_ref.npz      64-atom diamond Si (2x2x2 cells, a=5.431, sigma=0.05, seed 0):
+                  positions, cell, padding (coords/species/image from
+                  nbl_create_paddings), neighbour list (nbl_build order),
+                  descriptors from Descriptor::compute, the order-8 FD
+                  num_gradient_single_atom of three atoms for a seeded dE/dzeta.
+  two_species_ref.npz  a 96-atom two-species periodic system (rc = 5.0) with
+                  species-dependent cutoffs given through a parameter file,
+                  its padding/list and the reference descriptors.
+  sf_g2g4.params / sf_two_species.params  parameter files, and
+  params_ref.npz  widths and descriptors the reference's file reader yields
+                  (including its carried-over parameter vector).
+
+Run from the repo root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+
+
+def diamond(ncell, a=5.431, sigma=0.05, seed=0):
+    basis = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]], dtype=float)
+    basis = np.concatenate([basis, basis + 0.25])
+    g = np.arange(ncell)
+    cells = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    pos = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+    rng = np.random.default_rng(seed)
+    L = ncell * a
+    return np.mod(pos + rng.normal(0.0, sigma, pos.shape), L), np.eye(3) * L
+
+
+def behler():
+    eta2 = 0.001 * 500.0 ** (np.arange(8) / 7.0)
+    g2 = np.stack([eta2, np.zeros(8)], axis=1).reshape(-1)
+    g4 = np.array([[z, lam, e] for z in (1.0, 2.0, 4.0, 16.0) for lam in (1.0, -1.0)
+                   for e in (0.001, 0.01)]).reshape(-1)
+    return ["g2", "g4"], [8, 2, 16, 3], np.concatenate([g2, g4])
+
+
+def padded(pos, cell, species, rc, pbc=(1, 1, 1)):
+    pc, ps, pi = ref.create_paddings(rc, cell, np.array(pbc, np.int32), pos, species)
+    coords = np.concatenate([pos, pc])
+    allsp = np.concatenate([species, ps]).astype(np.int32)
+    need = np.r_[np.ones(len(pos)), np.zeros(len(pc))].astype(np.int32)
+    nn, nl = ref.neighlist(coords, rc, need, len(pos))
+    return coords, allsp, pi, nn, nl
+
+
+def write_params(path, species_pairs, types, sizes, params, note):
+    with open(path, "w") as f:
+        f.write(f"# {note}\n# cutoff type\ncos\n\n# number of species\n{len(species_pairs)}\n")
+        f.write("# species pairs and cutoffs\n")
+        for a, b, rc in species_pairs:
+            f.write(f"{a} {b} {rc}\n")
+        f.write(f"\n# symmetry function types\n{len(types)}\n")
+        off = 0
+        for t in range(len(types)):
+            r, c = sizes[2 * t], sizes[2 * t + 1]
+            f.write(f"{types[t]} {r} {c}\n")
+            for q in range(r):
+                f.write(" ".join(repr(float(v)) for v in params[off + q * c: off + (q + 1) * c]) + "\n")
+            off += r * c
+        f.write("\n# normalize\nfalse\n# descriptor size\n")
+        f.write(f"{sum(sizes[0::2])}\n")
+
+
+def main():
+    if not ref.available():
+        sys.exit("oracle/_ref/libsfref.so is missing: run oracle/build_ref.sh first")
+    types, sizes, params = behler()
+
+    # ---- C1 ----
+    pos, cell = diamond(2)
+    sp = np.zeros(len(pos), np.int32)
+    coords, allsp, image, nn, nl = padded(pos, cell, sp, 4.5)
+    R = ref.RefSF(["Si"], 4.5, types, sizes, params)
+    desc = R.compute(len(pos), allsp, nl, nn, coords)
+    rng = np.random.default_rng(1)
+    dE = rng.normal(size=desc.shape)
+    off = np.r_[0, np.cumsum(nn)]
+    fd_atoms = np.array([0, 5, 37], np.int32)
+    fd = np.stack([R.num_gradient_single_atom(int(i), allsp, nl[off[i]:off[i + 1]], coords, dE[i])
+                   for i in fd_atoms])
+    np.savez_compressed(os.path.join(HERE, "c1_ref.npz"), positions=pos, cell=cell,
+                        coords=coords, species=allsp, image=image, numneigh=nn, neighlist=nl,
+                        sizes=np.array(sizes, np.int32), params=params, cutoff=4.5, desc=desc,
+                        dE_dzeta=dE, fd_atoms=fd_atoms, fd=fd)
+
+    # ---- two species, species-dependent cutoffs through a file ----
+    rng = np.random.default_rng(7)
+    L = (96 / 0.05) ** (1 / 3)
+    pts = []
+    while len(pts) < 96:
+        c = rng.random(3) * L
+        if all(np.linalg.norm((c - p + L / 2) % L - L / 2) >= 1.8 for p in pts):
+            pts.append(c)
+    pos2 = np.array(pts)
+    sp2 = (rng.random(96) < 0.3).astype(np.int32)
+    coords2, allsp2, image2, nn2, nl2 = padded(pos2, np.eye(3) * L, sp2, 5.0)
+    fpath = os.path.join(HERE, "sf_two_species.params")
+    # two species -> only two pair lines are read by the reference reader;
+    # the (Al, Si) pair is left at cutoff 0.0 by its std::map default
+    write_params(fpath, [("Si", "Si", 5.0), ("Al", "Al", 4.2)], ["g2"], [8, 2],
+                 list(params[:16]), "two species, g2 only")
+    Rf = ref.RefSF(path=fpath)
+    desc2 = Rf.compute(96, allsp2, nl2, nn2, coords2)
+    # g2 + g4 file: the reference reader carries g2's values into g4's table
+    fpath3 = os.path.join(HERE, "sf_g2g4.params")
+    write_params(fpath3, [("Si", "Si", 4.5)], types, sizes, list(params), "Behler Si, g2 + g4")
+    Rg = ref.RefSF(path=fpath3)
+    desc3 = Rg.compute(len(pos), allsp, nl, nn, coords)
+    np.savez_compressed(os.path.join(HERE, "two_species_ref.npz"), positions=pos2, L=L,
+                        species0=sp2, coords=coords2, species=allsp2, image=image2,
+                        numneigh=nn2, neighlist=nl2, desc_file=desc2)
+    np.savez_compressed(os.path.join(HERE, "params_ref.npz"), width_two_species=Rf.width,
+                        width_g2g4=Rg.width, desc_g2g4_file=desc3)
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()

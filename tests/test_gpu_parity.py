@@ -1,0 +1,263 @@
+"""GPU parity: the HIP path (through the pybind module / C++ API / C-ABI, and
+through the C-ABI plan path from torch) against the oracle and the reference's
+golden outputs on the same inputs.
+
+Tolerances (written here, used below):
+  FP64 descriptors      : elementwise relative <= 1e-12
+  FP64 gradients        : max|diff| <= 1e-12 * max|oracle| (normwise; entries
+                          can cancel to ~0, so no elementwise relative bound)
+  FP32-bulk descriptors : elementwise relative <= 1e-5
+  FP32-bulk gradients   : normwise <= 1e-5
+Neighbour indexing and descriptor ordering are bit-exact by construction: the
+kernels consume the golden neighbour list and write the reference's slots.
+"""
+import numpy as np
+import pytest
+
+import libdescriptor_amd as ld
+from conftest import behler, elem_rel_err, golden, rel_err
+from oracle import orc
+from test_oracle import ALL_KINDS, cluster, full_list
+
+pytestmark = pytest.mark.gpu
+
+SF = ld.AvailableDescriptors.SymmetryFunctions
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def sf_desc(species=("Si",), rc=4.5, kinds=None, resolved=False, precision=0):
+    types, sizes, params = kinds if kinds else behler()
+    return ld.DescriptorKind.init_symmetry_functions(list(species), rc, types, sizes, list(params),
+                                                     species_resolved=resolved, precision=precision)
+
+
+def oracle_of(species=("Si",), rc=4.5, kinds=None, resolved=False):
+    types, sizes, params = kinds if kinds else behler()
+    return orc.OracleSF(list(species), rc, types, sizes, params, species_resolved=resolved)
+
+
+def test_c1_forward_matches_reference_golden(c1):
+    ds = ld.DescriptorKind.init_descriptor(SF, ["Si"], "cos", 4.5, *behler())
+    d = ld.compute(ds, 64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"])
+    assert d.shape == (64, 24)
+    assert elem_rel_err(d, c1["desc"], 1e-300) <= TOL64
+
+
+def test_c1_gradient_matches_oracle(c1):
+    ds = ld.DescriptorKind.init_descriptor(SF, ["Si"], "cos", 4.5, *behler())
+    g = c1["dE_dzeta"]
+    G = ld.gradient(ds, 64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"],
+                    c1["desc"], g)
+    O = oracle_of()
+    G0 = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"], g)
+    assert G.shape == c1["coords"].shape
+    assert rel_err(G, G0) <= TOL64
+    # padding rows carry gradient too (reference layout (n_total, 3))
+    assert np.abs(G[64:]).max() > 0
+
+
+def test_single_atom_entry_points(c1):
+    ds = ld.DescriptorKind.init_descriptor(SF, ["Si"], "cos", 4.5, *behler())
+    O = oracle_of()
+    off = np.r_[0, np.cumsum(c1["numneigh"])]
+    for i in (0, 13, 63):
+        nb = c1["neighlist"][off[i]:off[i + 1]]
+        d1 = ld.compute_single_atom(ds, i, c1["species"], nb, c1["coords"])
+        d2 = ds.compute(i, c1["species"], nb, c1["coords"])
+        assert elem_rel_err(d1, c1["desc"][i], 1e-300) <= TOL64
+        np.testing.assert_array_equal(d1, d2)
+        g = c1["dE_dzeta"][i]
+        G = ld.gradient_single_atom(ds, i, c1["species"], nb, c1["coords"], c1["desc"][i], g)
+        G0 = O.gradient_single_atom(i, c1["species"], nb, c1["coords"], g)
+        assert rel_err(G, G0) <= TOL64
+
+
+def test_reference_fd_routine_on_gpu(c1):
+    ds = ld.DescriptorKind.init_descriptor(SF, ["Si"], "cos", 4.5, *behler())
+    off = np.r_[0, np.cumsum(c1["numneigh"])]
+    for a, i in enumerate(c1["fd_atoms"]):
+        nb = c1["neighlist"][off[i]:off[i + 1]]
+        fd = ld.num_gradient_single_atom(ds, int(i), c1["species"], nb, c1["coords"],
+                                         c1["dE_dzeta"][i])
+        # descriptor differences at +-k h (h ~ 0.037 A) divided by 105 h
+        assert rel_err(fd, c1["fd"][a]) <= 1e-9
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_all_kinds_general_kernels(seed):
+    pos = cluster(14, seed)
+    nn, nl = full_list(pos, 4.0)
+    sp = np.zeros(len(pos), np.int32)
+    ds = sf_desc(rc=4.0, kinds=ALL_KINDS)
+    O = oracle_of(rc=4.0, kinds=ALL_KINDS)
+    d = ld.compute(ds, len(pos), sp, nl, nn, pos)
+    d0 = O.compute(len(pos), sp, nl, nn, pos)
+    assert elem_rel_err(d, d0, 1e-300) <= TOL64
+    g = np.random.default_rng(seed).normal(size=d.shape)
+    G = ld.gradient(ds, len(pos), sp, nl, nn, pos, d, g)
+    assert rel_err(G, O.gradient(len(pos), sp, nl, nn, pos, g)) <= TOL64
+
+
+def small_two_species(n=200, seed=3):
+    from libdescriptor_amd import workloads as w
+    pos, cell, pbc, sp = w.random_system(n, 0.05, 1.8, seed=seed)
+    return w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+
+
+def test_species_resolved_two_species():
+    L = small_two_species()
+    n = L["n_contrib"]
+    ds = sf_desc(("A", "B"), 5.0, resolved=True)
+    assert ds.width == 64
+    O = oracle_of(("A", "B"), 5.0, resolved=True)
+    d = ld.compute(ds, n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    d0 = O.compute(n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    assert elem_rel_err(d, d0, 1e-300) <= TOL64
+    g = np.random.default_rng(0).normal(size=d.shape)
+    G = ld.gradient(ds, n, L["species"], L["neighlist"], L["numneigh"], L["coords"], d, g)
+    assert rel_err(G, O.gradient(n, L["species"], L["neighlist"], L["numneigh"], L["coords"], g)) <= TOL64
+
+
+def test_batch_entry_points():
+    from libdescriptor_amd import workloads as w
+    lays = []
+    for k, n in enumerate([8, 30, 111, 17]):
+        if k % 2 == 0:
+            pos, cell, pbc, sp = w.random_system(n, 0.05, 1.8, seed=k)
+        else:
+            pos, cell, pbc, sp = w.cluster(n, 0.05, 1.8, seed=k)
+        lays.append(w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"]))
+    B = w.concat(lays)
+    ds = sf_desc(("A", "B"), 5.0, resolved=True)
+    O = oracle_of(("A", "B"), 5.0, resolved=True)
+    args = (B["n_atoms"], B["config_ptr"], B["species"], B["neighlist"], B["numneigh"], B["coords"])
+    d = ld.compute_batch(ds, *args)
+    d0 = O.compute_batch(*args)
+    assert elem_rel_err(d, d0, 1e-300) <= TOL64
+    g = np.random.default_rng(1).normal(size=d.shape)
+    G = ld.gradient_batch(ds, *args, d, g)
+    assert rel_err(G, O.gradient_batch(*args, g)) <= TOL64
+    # each configuration alone gives the same rows
+    row = 0
+    for k, L in enumerate(lays):
+        n = L["n_contrib"]
+        dk = ld.compute(ds, n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+        np.testing.assert_array_equal(dk, d[row:row + n])
+        row += n
+
+
+def test_jacobian_matches_one_hot_oracle():
+    pos = cluster(7, 5, box=5.0)
+    nn, nl = full_list(pos, 4.0)
+    sp = np.zeros(len(pos), np.int32)
+    ds = sf_desc(rc=4.0)
+    O = oracle_of(rc=4.0)
+    J = ld.jacobian(ds, len(pos), sp, nl, nn, pos)
+    W = ds.width
+    assert J.shape == (W * len(pos), pos.size)
+    for row in (0, 5, W + 3, W * len(pos) - 1):
+        g = np.zeros(W * len(pos)); g[row] = 1.0
+        ref = O.gradient(len(pos), sp, nl, nn, pos, g.reshape(len(pos), W)).reshape(-1)
+        assert rel_err(J[row], ref, floor=1e-300) <= TOL64
+
+
+def test_file_initialised_descriptors():
+    import os
+    from conftest import GOLDEN
+    t = golden("two_species_ref.npz")
+    ds = ld.DescriptorKind.init_descriptor(os.path.join(GOLDEN, "sf_two_species.params"), SF)
+    d = ld.compute(ds, 96, t["species"], t["neighlist"], t["numneigh"], t["coords"])
+    assert elem_rel_err(d, t["desc_file"], 1e-300) <= TOL64
+    c1 = golden("c1_ref.npz")
+    pr = golden("params_ref.npz")
+    ds = ld.DescriptorKind.init_descriptor(os.path.join(GOLDEN, "sf_g2g4.params"), SF)
+    d = ld.compute(ds, 64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"])
+    assert elem_rel_err(d, pr["desc_g2g4_file"], 1e-300) <= TOL64
+
+
+def test_fp32_bulk_error_bounds(c1):
+    for species, rc, resolved, L in [(("Si",), 4.5, False, None),
+                                     (("A", "B"), 5.0, True, small_two_species())]:
+        if L is None:
+            L = dict(coords=c1["coords"], species=c1["species"], neighlist=c1["neighlist"],
+                     numneigh=c1["numneigh"], n_contrib=64)
+        n = L["n_contrib"]
+        args = (n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+        d64 = ld.compute(sf_desc(species, rc, resolved=resolved), *args)
+        ds32 = sf_desc(species, rc, resolved=resolved, precision=ld.FP32_BULK)
+        d32 = ld.compute(ds32, *args)
+        assert elem_rel_err(d32, d64, 1e-30) <= TOL32
+        g = np.random.default_rng(2).normal(size=d64.shape)
+        G64 = ld.gradient(sf_desc(species, rc, resolved=resolved), *args, d64, g)
+        G32 = ld.gradient(ds32, *args, d64, g)
+        assert rel_err(G32, G64) <= TOL32
+
+
+def test_torch_plan_path_and_autograd(c1):
+    import torch
+    from libdescriptor_amd.autograd import SymmetryFunctionsLayer
+    ds = sf_desc()
+    layer = SymmetryFunctionsLayer(ds, c1["species"], c1["neighlist"], c1["numneigh"])
+    x = torch.tensor(c1["coords"], device="cuda", requires_grad=True)
+    d = layer(x)
+    ref_d = ld.compute(ds, 64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"])
+    np.testing.assert_array_equal(d.detach().cpu().numpy(), ref_d)
+    w = torch.tensor(c1["dE_dzeta"], device="cuda")
+    (d * w).sum().backward()
+    G = ld.gradient(ds, 64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"],
+                    ref_d, c1["dE_dzeta"])
+    np.testing.assert_array_equal(x.grad.cpu().numpy(), G)
+
+
+def test_gradient_is_deterministic():
+    L = small_two_species(400, seed=9)
+    n = L["n_contrib"]
+    ds = sf_desc(("A", "B"), 5.0, resolved=True)
+    args = (n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    g = np.random.default_rng(4).normal(size=(n, 64))
+    G1 = ld.gradient(ds, *args, np.zeros((n, 64)), g)
+    G2 = ld.gradient(ds, *args, np.zeros((n, 64)), g)
+    np.testing.assert_array_equal(G1, G2)
+
+
+def test_out_of_range_neighbour_is_rejected(c1):
+    import torch
+    from libdescriptor_amd import _capi
+    ds = sf_desc()
+    bad = c1["neighlist"].copy()
+    bad[5] = len(c1["coords"])  # one past the last row
+    with pytest.raises(ValueError):
+        ld.compute(ds, 64, c1["species"], bad, c1["numneigh"], c1["coords"])
+    bad[5] = -3
+    with pytest.raises(ValueError):
+        ld.gradient(ds, 64, c1["species"], bad, c1["numneigh"], c1["coords"], c1["desc"],
+                    c1["dE_dzeta"])
+    # the device plan validates indices on the GPU before any kernel reads them
+    bad[5] = len(c1["coords"]) + 7
+    t = lambda a: torch.tensor(a, dtype=torch.int32, device="cuda")  # noqa: E731
+    with pytest.raises(_capi.LdsfError):
+        _capi.Plan(ds._handle, len(c1["coords"]), t(c1["species"]), t(bad), t(c1["numneigh"]))
+    sp = c1["species"].copy()
+    sp[100] = 4  # species code beyond the parameter set
+    with pytest.raises(_capi.LdsfError):
+        _capi.Plan(ds._handle, len(c1["coords"]), t(sp), t(c1["neighlist"]), t(c1["numneigh"]))
+
+
+def test_c2_full_size_properties():
+    from libdescriptor_amd import workloads as w
+    L = w.config_c2()
+    n = L["n_contrib"]
+    ds = sf_desc()
+    args = (n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    d = ld.compute(ds, *args)
+    O = oracle_of()
+    m = 1500  # oracle on the first m centres
+    d0 = O.compute(m, L["species"], L["neighlist"], L["numneigh"][:m], L["coords"])
+    assert elem_rel_err(d[:m], d0, 1e-300) <= TOL64
+    g = np.random.default_rng(6).normal(size=d.shape)
+    G = ld.gradient(ds, *args, d, g)
+    off = np.r_[0, np.cumsum(L["numneigh"])].astype(np.int32)
+    _, G0 = O.fwd_adj_omp(n, L["species"], L["neighlist"], off, L["coords"], g, 8)
+    assert rel_err(G, G0) <= TOL64
+    assert np.abs(G.sum(axis=0)).max() <= 1e-11 * np.abs(G).max() * len(G) ** 0.5

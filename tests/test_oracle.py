@@ -1,0 +1,294 @@
+"""Pins of the oracle (oracle/sf_oracle.c) -- CPU only.
+
+forward : bit-exact against the reference's own compiled forward (oracle/_ref)
+          and its golden output; closed forms for one neighbour and for an
+          equilateral triple; mpmath at 50 digits.
+adjoint : central finite differences (h = 1e-4..1e-6 converge as h^2);
+          translation and rotation invariance; linearity in dE/dzeta;
+          the reference's own order-8 FD routine (golden values).
+layouts : species-resolved blocks sum to the reference layout.
+"""
+import numpy as np
+import pytest
+
+from conftest import behler, golden, rel_err
+from oracle import orc, ref
+
+# every kind once: g1; g2 x3 (eta, Rs); g3 x2 (kappa); g4 x3 and g5 x1 (zeta, lambda, eta)
+ALL_KINDS = (["g1", "g2", "g3", "g4", "g5"], [1, 0, 3, 2, 2, 1, 3, 3, 1, 3],
+             [0.7, 0.3, 0.05, 1.2, 1.9, 1.0,
+              1.0, 0.003,
+              2.0, -1.0, 0.02, 5.0, 1.0, 0.004, 1.0, -1.0, 0.01,
+              3.5, 1.0, 0.002])
+
+
+def cluster(n, seed, box=7.0, dmin=1.6):
+    rng = np.random.default_rng(seed)
+    pts = []
+    while len(pts) < n:
+        c = rng.random(3) * box
+        if all(np.linalg.norm(c - p) >= dmin for p in pts):
+            pts.append(c)
+    return np.array(pts)
+
+
+def full_list(pos, rc, n_centers=None):
+    n = len(pos)
+    n_centers = n if n_centers is None else n_centers
+    nn, nl = [], []
+    for i in range(n_centers):
+        d = np.linalg.norm(pos - pos[i], axis=1)
+        js = [j for j in range(n) if j != i and d[j] < rc]
+        nn.append(len(js))
+        nl += js
+    return np.array(nn, np.int32), np.array(nl, np.int32)
+
+
+def fc(r, rc):
+    return 0.5 * (np.cos(np.pi * r / rc) + 1.0) if r < rc else 0.0
+
+
+# ---------------- forward ----------------
+
+def test_golden_c1_bit_exact(c1):
+    types, sizes, params = behler()
+    O = orc.OracleSF(["Si"], 4.5, types, sizes, params)
+    d = O.compute(64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"])
+    np.testing.assert_array_equal(d, c1["desc"])
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_all_kinds_match_reference_build(seed):
+    pos = cluster(14, seed)
+    nn, nl = full_list(pos, 4.0)
+    sp = np.zeros(len(pos), np.int32)
+    types, sizes, params = ALL_KINDS
+    R = ref.RefSF(["Si"], 4.0, types, sizes, params)
+    O = orc.OracleSF(["Si"], 4.0, types, sizes, params)
+    a = R.compute(len(pos), sp, nl, nn, pos)
+    b = O.compute(len(pos), sp, nl, nn, pos)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference build (oracle/_ref) not present")
+def test_species_dependent_cutoffs_match_reference_file():
+    t = golden("two_species_ref.npz")
+    # the file lists (Si,Si)=5.0 and (Al,Al)=4.2; (Al,Si) stays 0.0
+    rc = np.array([[4.2, 0.0], [0.0, 5.0]])
+    types, sizes, params = behler()
+    O = orc.OracleSF(["Al", "Si"], rc, ["g2"], [8, 2], params[:16])
+    d = O.compute(96, t["species"], t["neighlist"], t["numneigh"], t["coords"])
+    np.testing.assert_array_equal(d, t["desc_file"])
+
+
+@pytest.mark.parametrize("a", [2.1, 3.3, 4.4])
+def test_equilateral_closed_form(a):
+    rc = 4.5
+    pos = np.array([[0, 0, 0], [a, 0, 0], [a / 2, a * np.sqrt(3) / 2, 0]], dtype=float) + 10.0
+    nn, nl = full_list(pos, rc)
+    z, lam, eta, kappa = 4.0, -1.0, 0.01, 1.3
+    O = orc.OracleSF(["X"], rc, ["g1", "g2", "g3", "g4", "g5"], [1, 0, 1, 2, 1, 1, 1, 3, 1, 3],
+                     [0.2, 0.0, kappa, z, lam, eta, z, 1.0, eta])
+    d = O.compute(3, np.zeros(3, np.int32), nl, nn, pos)[0]
+    f = fc(a, rc)
+    expect = [2 * f, 2 * np.exp(-0.2 * a * a) * f, 2 * np.cos(kappa * a) * f,
+              2 ** (1 - z) * (1 + lam / 2) ** z * np.exp(-3 * eta * a * a) * f ** 3,
+              2 ** (1 - z) * 1.5 ** z * np.exp(-2 * eta * a * a) * f ** 2]
+    np.testing.assert_allclose(d, expect, rtol=1e-14)
+
+
+def test_cutoff_and_base_special_cases():
+    rc = 4.0
+    # neighbour beyond and exactly at the cutoff contribute nothing;
+    # collinear j-i-k with lambda = +1: base = 0 -> g4 term is 0
+    pos = np.array([[0, 0, 0], [2.0, 0, 0], [-2.0, 0, 0], [rc, 0, 0], [0, 5.0, 0]], dtype=float)
+    nn, nl = full_list(pos, 10.0, n_centers=1)
+    O = orc.OracleSF(["X"], rc, ["g2", "g4"], [1, 2, 2, 3], [0.1, 0.0, 1.0, 1.0, 0.0, 1.0, -1.0, 0.0])
+    d = O.compute(1, np.zeros(5, np.int32), nl, nn, pos)[0]
+    f = fc(2.0, rc)
+    assert d[0] == pytest.approx(2 * np.exp(-0.1 * 4.0) * f, rel=1e-15)
+    assert d[1] == 0.0  # lambda=+1, cos = -1 (j and k opposite; r_jk = 4.0 = rc anyway)
+    assert d[2] == 0.0  # r_jk == rc -> fc_jk = 0
+
+
+def test_mpmath_50_digits():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    pos = cluster(6, 11, box=4.5)
+    rc = 4.2
+    nn, nl = full_list(pos, rc)
+    types, sizes, params = ALL_KINDS
+    O = orc.OracleSF(["Si"], rc, types, sizes, params)
+    d = O.compute(len(pos), np.zeros(len(pos), np.int32), nl, nn, pos)
+
+    P = mp.mpf
+    pi = mp.pi
+
+    def f(r):
+        return (mp.cos(pi * r / rc) + 1) / 2 if r < rc else P(0)
+
+    off = np.r_[0, np.cumsum(nn)]
+    for i in range(len(pos)):
+        X = [[P(v) for v in p] for p in pos]
+        out = [P(0)] * O.width
+        nb = nl[off[i]:off[i + 1]]
+        dist = lambda a, b: mp.sqrt(sum((X[a][t] - X[b][t]) ** 2 for t in range(3)))  # noqa
+        for jj, j in enumerate(nb):
+            rij = dist(i, j)
+            out[0] += f(rij)
+            for q, (eta, rs) in enumerate([(P(0.7), P(0.3)), (P(0.05), P(1.2)), (P(1.9), P(1.0))]):
+                out[1 + q] += mp.exp(-eta * (rij - rs) ** 2) * f(rij)
+            for q, kap in enumerate([P(1.0), P(0.003)]):
+                out[4 + q] += mp.cos(kap * rij) * f(rij)
+            for k in nb[jj + 1:]:
+                rik, rjk = dist(i, k), dist(j, k)
+                cos = (rij ** 2 + rik ** 2 - rjk ** 2) / (2 * rij * rik)
+                g4 = [(P(2.0), P(-1.0), P(0.02)), (P(5.0), P(1.0), P(0.004)),
+                      (P(1.0), P(-1.0), P(0.01))]
+                for q, (z, lam, eta) in enumerate(g4):
+                    b = 1 + lam * cos
+                    if rjk < rc and b > 0:
+                        out[6 + q] += 2 ** (1 - z) * b ** z * mp.exp(-eta * (rij ** 2 + rik ** 2 + rjk ** 2)) * f(rij) * f(rik) * f(rjk)
+                z, lam, eta = P(3.5), P(1.0), P(0.002)
+                b = 1 + lam * cos
+                if b > 0:
+                    out[9] += 2 ** (1 - z) * b ** z * mp.exp(-eta * (rij ** 2 + rik ** 2)) * f(rij) * f(rik)
+        exact = np.array([float(v) for v in out])
+        np.testing.assert_allclose(d[i], exact, rtol=1e-13, atol=1e-300)
+
+
+# ---------------- adjoint ----------------
+
+def _central_fd(O, pos, nl, nn, g, h, rows):
+    n = len(pos)
+    out = np.zeros((len(rows), 3))
+    sp = np.zeros(len(pos), np.int32)
+    for a, r in enumerate(rows):
+        for d in range(3):
+            p = pos.copy(); p[r, d] += h
+            m = pos.copy(); m[r, d] -= h
+            out[a, d] = np.sum(g * (O.compute(len(nn), sp, nl, nn, p) - O.compute(len(nn), sp, nl, nn, m))) / (2 * h)
+    return out
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_adjoint_central_fd_converges(seed):
+    pos = cluster(10, seed, box=5.5)
+    rc = 4.3
+    nn, nl = full_list(pos, rc)
+    types, sizes, params = ALL_KINDS
+    O = orc.OracleSF(["Si"], rc, types, sizes, params)
+    rng = np.random.default_rng(seed)
+    g = rng.normal(size=(len(pos), O.width))
+    an = O.gradient(len(pos), np.zeros(len(pos), np.int32), nl, nn, pos, g)
+    rows = list(range(len(pos)))
+    e3 = rel_err(_central_fd(O, pos, nl, nn, g, 1e-3, rows), an)
+    e4 = rel_err(_central_fd(O, pos, nl, nn, g, 1e-4, rows), an)
+    assert e4 < 1e-7
+    assert e4 < e3 / 30  # second-order convergence (a ~100x drop expected)
+
+
+def test_adjoint_invariances_and_linearity(c1):
+    types, sizes, params = behler()
+    O = orc.OracleSF(["Si"], 4.5, types, sizes, params)
+    x = c1["coords"]
+    rng = np.random.default_rng(5)
+    g1, g2 = rng.normal(size=(2, 64, 24))
+    G1 = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], x, g1)
+    G2 = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], x, g2)
+    G12 = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], x, 2.0 * g1 - 3.0 * g2)
+    scale = np.abs(G1).max()
+    # translation: sum over all rows vanishes; rotation: sum r x dE/dr vanishes
+    assert np.abs(G1.sum(axis=0)).max() < 1e-12 * scale * len(x)
+    assert np.abs(np.cross(x, G1).sum(axis=0)).max() < 1e-11 * scale * np.abs(x).max() * len(x)
+    assert rel_err(G12, 2.0 * G1 - 3.0 * G2) < 1e-13
+
+
+def test_reference_order8_fd_golden(c1):
+    types, sizes, params = behler()
+    O = orc.OracleSF(["Si"], 4.5, types, sizes, params)
+    off = np.r_[0, np.cumsum(c1["numneigh"])]
+    for a, i in enumerate(c1["fd_atoms"]):
+        nb = c1["neighlist"][off[i]:off[i + 1]]
+        fd = O.num_gradient_single_atom(int(i), c1["species"], nb, c1["coords"], c1["dE_dzeta"][i])
+        np.testing.assert_array_equal(fd, c1["fd"][a])
+
+
+def test_order8_fd_agrees_away_from_cutoff():
+    # no pair within 4h (~0.15 A) of the cutoff: the order-8 stencil is accurate
+    pos = cluster(8, 21, box=4.0, dmin=1.5)
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2)
+    rc = 6.5
+    assert np.abs(d[d > 0] - rc).min() > 0.3
+    nn, nl = full_list(pos, rc)
+    types, sizes, params = ALL_KINDS
+    O = orc.OracleSF(["Si"], rc, types, sizes, params)
+    g = np.random.default_rng(2).normal(size=O.width)
+    off = np.r_[0, np.cumsum(nn)]
+    for i in range(len(pos)):
+        nb = nl[off[i]:off[i + 1]]
+        fd = O.num_gradient_single_atom(i, np.zeros(8, np.int32), nb, pos, g)
+        an = O.gradient_single_atom(i, np.zeros(8, np.int32), nb, pos, g)[i]
+        assert rel_err(fd, an) < 1e-8
+
+
+# ---------------- species-resolved layout ----------------
+
+def test_species_resolved_block_sums():
+    pos = cluster(16, 9, box=7.5)
+    rc = 4.4
+    nn, nl = full_list(pos, rc)
+    sp = (np.arange(len(pos)) % 3 == 0).astype(np.int32)
+    types, sizes, params = behler()
+    base = orc.OracleSF(["A", "B"], rc, types, sizes, params)
+    res = orc.OracleSF(["A", "B"], rc, types, sizes, params, species_resolved=True)
+    assert res.width == 2 * 8 + 3 * 16
+    d0 = base.compute(len(pos), sp, nl, nn, pos)
+    d1 = res.compute(len(pos), sp, nl, nn, pos)
+    np.testing.assert_allclose(d1[:, 0:8] + d1[:, 8:16], d0[:, :8], rtol=1e-14)
+    np.testing.assert_allclose(d1[:, 16:32] + d1[:, 32:48] + d1[:, 48:64], d0[:, 8:], rtol=1e-13,
+                               atol=1e-300)
+    # one species: the resolved layout is the reference layout
+    one = orc.OracleSF(["A"], rc, types, sizes, params, species_resolved=True)
+    z = np.zeros(len(pos), np.int32)
+    np.testing.assert_array_equal(one.compute(len(pos), z, nl, nn, pos),
+                                  orc.OracleSF(["A"], rc, types, sizes, params).compute(len(pos), z, nl, nn, pos))
+
+
+def test_species_resolved_adjoint_fd():
+    pos = cluster(9, 13, box=5.0)
+    rc = 4.3
+    nn, nl = full_list(pos, rc)
+    sp = (np.arange(len(pos)) % 2).astype(np.int32)
+    types, sizes, params = behler()
+    O = orc.OracleSF(["A", "B"], rc, types, sizes, params, species_resolved=True)
+    g = np.random.default_rng(3).normal(size=(len(pos), O.width))
+    an = O.gradient(len(pos), sp, nl, nn, pos, g)
+    fd = np.zeros_like(an)
+    h = 1e-5
+    for r in range(len(pos)):
+        for d in range(3):
+            p = pos.copy(); p[r, d] += h
+            m = pos.copy(); m[r, d] -= h
+            fd[r, d] = np.sum(g * (O.compute(len(pos), sp, nl, nn, p) - O.compute(len(pos), sp, nl, nn, m))) / (2 * h)
+    assert rel_err(fd, an) < 1e-7
+
+
+def test_batch_walk_equals_single_configs():
+    types, sizes, params = behler()
+    O = orc.OracleSF(["Si"], 4.0, types, sizes, params)
+    a, b = cluster(7, 1), cluster(9, 2)
+    nna, nla = full_list(a, 4.0)
+    nnb, nlb = full_list(b, 4.0)
+    x = np.concatenate([a, b])
+    nn = np.concatenate([nna, nnb]).astype(np.int32)
+    nl = np.concatenate([nla, nlb + len(a)]).astype(np.int32)
+    sp = np.zeros(len(x), np.int32)
+    db = O.compute_batch([7, 9], [0, 7], sp, nl, nn, x)
+    np.testing.assert_array_equal(db[:7], O.compute(7, sp[:7], nla, nna, a))
+    np.testing.assert_array_equal(db[7:], O.compute(9, sp[:9], nlb, nnb, b))
+    g = np.random.default_rng(0).normal(size=db.shape)
+    gb = O.gradient_batch([7, 9], [0, 7], sp, nl, nn, x, g)
+    np.testing.assert_array_equal(gb[:7], O.gradient(7, sp[:7], nla, nna, a, g[:7]))
+    np.testing.assert_array_equal(gb[7:], O.gradient(9, sp[:9], nlb, nnb, b, g[7:]))
