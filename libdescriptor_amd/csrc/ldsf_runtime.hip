@@ -60,25 +60,6 @@ struct DevBuf {
     }
 };
 
-ldsf::WaveShape wave_shape_of(const ldsf_sf *sf, bool *ok)
-{
-    ldsf::WaveShape sh{};
-    const int ns = (int)sf->species.size();
-    sh.n2 = (int)sf->k2.size();
-    sh.n3 = (int)sf->k3.size();
-    sh.nsb = sf->species_resolved ? ns : 1;
-    sh.npb = sf->species_resolved ? ns * (ns + 1) / 2 : 1;
-    bool good = true;
-    for (int k : sf->k2) good = good && k == ldsf::G2;
-    for (int k : sf->k3) good = good && k == ldsf::G4;
-    // slots must be laid out block by block: one g2 block then one g4 block
-    if (sf->kind.size() > 2) good = false;
-    if (sf->kind.size() == 2 && !(sf->kind[0] == ldsf::G2 && sf->kind[1] == ldsf::G4)) good = false;
-    if (sf->kind.size() == 1 && sf->kind[0] != ldsf::G2 && sf->kind[0] != ldsf::G4) good = false;
-    *ok = good && ldsf::wave_supported(sh, ns);
-    return sh;
-}
-
 }  // namespace
 
 ldsf_sf::~ldsf_sf()
@@ -173,7 +154,7 @@ struct ldsf_plan {
     int max_nn = 0;
     int family = 0;
     bool wave_ok = false;
-    ldsf::WaveShape shape{};
+    ldsf::WaveTab wave{};
     const int *species = nullptr;  // borrowed
     const int *nl = nullptr;       // borrowed
     int *centers = nullptr;        // owned (NULL = identity)
@@ -182,6 +163,7 @@ struct ldsf_plan {
     int *in_edges = nullptr;
     double *edge = nullptr;
     double *self = nullptr;
+    int *wave_slots = nullptr;
 
     ~ldsf_plan()
     {
@@ -189,7 +171,7 @@ struct ldsf_plan {
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device);
         for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
-                        (void *)edge, (void *)self})
+                        (void *)edge, (void *)self, (void *)wave_slots})
             if (p) (void)hipFree(p);
         (void)hipSetDevice(cur);
     }
@@ -267,7 +249,17 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     plan->offs = offs.release_ownership();
     plan->in_start = in_start.release_ownership();
     plan->in_edges = in_edges.release_ownership();
-    plan->shape = wave_shape_of(sf, &plan->wave_ok);
+    // wave kernels: parameter set covered and the largest neighbourhood fits in LDS
+    int slot_table[ldsf::kWaveSlots];
+    if (ldsf::wave_build(sf, &plan->wave, slot_table) && plan->max_nn <= 1023 &&
+        ldsf::wave_lds_bytes(plan->max_nn, true) <= ldsf::kWaveMaxLds) {
+        HIP_TRY(hipMalloc((void **)&plan->wave_slots, sizeof(slot_table)), "hipMalloc(slots)");
+        HIP_TRY(hipMemcpy(plan->wave_slots, slot_table, sizeof(slot_table), hipMemcpyHostToDevice),
+                "hipMemcpy(slots)");
+        plan->wave.rc = tab.rc;
+        plan->wave.slots = plan->wave_slots;
+        plan->wave_ok = true;
+    }
     plan->family = plan->wave_ok ? 1 : 0;
     *out = guard.release();
     return LDSF_OK;
@@ -279,8 +271,9 @@ int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, 
     if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
     hipError_t e;
     if (plan->family == 1 && plan->wave_ok)
-        e = ldsf::launch_fwd_wave(tab, plan->shape, precision, plan->n_centers, plan->centers,
-                                  plan->offs, plan->nl, plan->species, x, desc, plan->max_nn, s);
+        e = ldsf::launch_fwd_wave(plan->wave, precision, plan->n_centers, plan->centers,
+                                  plan->offs, plan->nl, plan->species, x, desc, tab.width,
+                                  plan->max_nn, s);
     else
         e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
                                      plan->nl, plan->species, x, desc, s);
@@ -299,9 +292,9 @@ int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dc
         HIP_TRY(hipMalloc((void **)&plan->self, sizeof(double) * 3 * plan->n_centers), "hipMalloc(self)");
     hipError_t e;
     if (plan->family == 1 && plan->wave_ok)
-        e = ldsf::launch_adj_wave(tab, plan->shape, precision, plan->n_centers, plan->centers,
+        e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, dE, plan->edge,
-                                  plan->self, plan->max_nn, s);
+                                  plan->self, tab.width, plan->max_nn, s);
     else
         e = ldsf::launch_adj_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
                                      plan->nl, plan->species, x, dE, plan->edge, plan->self, s);

@@ -16,22 +16,40 @@ hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, co
                               const int *offs, const int *nl, const int *species, const double *x,
                               const double *dE, double *edge, double *self, hipStream_t s);
 
-// sf_wave.hip: one wavefront per centre, neighbours staged in LDS
-struct WaveShape {
-    int n2;      // two-body rows, all g2
-    int n3;      // three-body rows, all g4
-    int nsb;     // species blocks of the two-body part (1 or n_species)
-    int npb;     // pair blocks of the three-body part (1 or n_pairs)
+// sf_wave.hip: one wavefront per centre, neighbours staged in LDS.  They
+// cover parameter sets whose two-body rows are g2 (at most 8) and whose
+// three-body rows are g4 forming a grid zeta x lambda x eta (at most 4 x 2 x 2
+// distinct values), in the reference layout or the two-species resolved one.
+struct WaveTab {
+    int n_species;
+    int nr2;                // g2 rows (<= 8)
+    int nz, nl, ne;         // distinct zeta, lambda, eta of the g4 grid
+    int nsb, npb;           // species blocks (1 or 2) and pair blocks (1 or 3)
+    int has3;               // any g4 row
+    double eta2[8], rs2[8];
+    double zeta[4], coef[4];
+    int izeta[4];
+    double lam[2];
+    double eta3[2];
+    const double *rc;       // device, n_species^2
+    const int *slots;       // device: [0,16) two-body slot of k = s*8+q,
+                            //         [16,64) three-body slot of p*16 + (z*2+l)*2+e; -1 unused
 };
-// true if the wave kernels cover this parameter set
-bool wave_supported(const WaveShape &shape, int n_species);
-hipError_t launch_fwd_wave(const SFTab &tab, const WaveShape &shape, int precision, int n_centers,
-                           const int *centers, const int *offs, const int *nl, const int *species,
-                           const double *x, double *desc, int max_nn, hipStream_t s);
-hipError_t launch_adj_wave(const SFTab &tab, const WaveShape &shape, int precision, int n_centers,
-                           const int *centers, const int *offs, const int *nl, const int *species,
-                           const double *x, const double *dE, double *edge, double *self,
-                           int max_nn, hipStream_t s);
+constexpr int kWaveSlots = 64;
+// host: true (and the table, with slots in *slot_table) if the wave kernels
+// cover this parameter set
+bool wave_build(const ldsf_sf *sf, WaveTab *wt, int *slot_table);
+// bytes of LDS a launch needs for the largest neighbour count; the wave
+// kernels run only when this fits
+size_t wave_lds_bytes(int max_nn, bool adjoint);
+constexpr size_t kWaveMaxLds = 64 * 1024;
+hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
+                           const int *offs, const int *nl, const int *species, const double *x,
+                           double *desc, int width, int max_nn, hipStream_t s);
+hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
+                           const int *offs, const int *nl, const int *species, const double *x,
+                           const double *dE, double *edge, double *self, int width, int max_nn,
+                           hipStream_t s);
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,

@@ -261,3 +261,39 @@ def test_c2_full_size_properties():
     _, G0 = O.fwd_adj_omp(n, L["species"], L["neighlist"], off, L["coords"], g, 8)
     assert rel_err(G, G0) <= TOL64
     assert np.abs(G.sum(axis=0)).max() <= 1e-11 * np.abs(G).max() * len(G) ** 0.5
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("resolved", [False, True])
+def test_wave_and_general_kernels_agree(precision, resolved):
+    """Both kernel families on the same plan: the wave kernels (used by
+    default where they apply) and the thread-per-centre kernels."""
+    import torch
+    from libdescriptor_amd import _capi
+    L = small_two_species(500, seed=11)
+    species = ("A", "B")
+    ds = sf_desc(species, 5.0, resolved=resolved)
+    t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device="cuda")  # noqa: E731
+    plan = _capi.Plan(ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
+                      t(L["numneigh"]))
+    assert plan.kernel_family == 1
+    n = L["n_contrib"]
+    x = t(L["coords"], torch.float64)
+    g = t(np.random.default_rng(5).normal(size=(n, ds.width)), torch.float64)
+    out = {}
+    for fam in (0, 1):
+        plan.kernel_family = fam
+        d = torch.zeros((n, ds.width), dtype=torch.float64, device="cuda")
+        dx = torch.zeros_like(x)
+        plan.compute(x, d, precision)
+        plan.gradient(x, g, dx, precision)
+        torch.cuda.synchronize()
+        out[fam] = (d.cpu().numpy(), dx.cpu().numpy())
+    tol = TOL64 if precision == 0 else TOL32
+    assert elem_rel_err(out[1][0], out[0][0], 1e-30) <= tol
+    assert rel_err(out[1][1], out[0][1]) <= tol
+    O = oracle_of(species, 5.0, resolved=resolved)
+    d0 = O.compute(n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    G0 = O.gradient(n, L["species"], L["neighlist"], L["numneigh"], L["coords"], g.cpu().numpy())
+    assert elem_rel_err(out[1][0], d0, 1e-30) <= tol
+    assert rel_err(out[1][1], G0) <= tol
