@@ -42,14 +42,35 @@ __device__ __forceinline__ void fcut(double r, double rc, double &fc, double &df
     dfc = -0.5 * s * (kPi / rc);
 }
 
+// fc(r) = 0.5 (cos(pi r / rc) + 1) = sin^2(y) with y = (pi / 2) (rc - r) / rc,
+// and dfc/dr = -(pi / rc) sin(y) cos(y).  y is formed in FP64; sin and cos of
+// y in [0, pi/2] come from their Taylor series in FP32 (truncation below 1e-8
+// there), so fc keeps ~1e-7 relative accuracy also where it vanishes at the
+// cutoff -- a hardware cosine's absolute error would not.
+__device__ __forceinline__ void sincos_quarter(float y, float &s, float &c)
+{
+    const float y2 = y * y;
+    float t = 1.0f - y2 * (1.0f / 156.0f);
+    t = 1.0f - y2 * (1.0f / 110.0f) * t;
+    t = 1.0f - y2 * (1.0f / 72.0f) * t;
+    t = 1.0f - y2 * (1.0f / 42.0f) * t;
+    t = 1.0f - y2 * (1.0f / 20.0f) * t;
+    t = 1.0f - y2 * (1.0f / 6.0f) * t;
+    s = y * t;
+    float u = 1.0f - y2 * (1.0f / 132.0f);
+    u = 1.0f - y2 * (1.0f / 90.0f) * u;
+    u = 1.0f - y2 * (1.0f / 56.0f) * u;
+    u = 1.0f - y2 * (1.0f / 30.0f) * u;
+    u = 1.0f - y2 * (1.0f / 12.0f) * u;
+    c = 1.0f - y2 * 0.5f * u;
+}
+
 __device__ __forceinline__ void fcut(double r, double rc, float &fc, float &dfc, Real<1>)
 {
-    // v_cos_f32 / v_sin_f32 compute cos(2 pi x), sin(2 pi x)
-    float x = (float)(0.5 * r / rc);
-    float c = __builtin_amdgcn_cosf(x);
-    float s = __builtin_amdgcn_sinf(x);
-    fc = 0.5f * (c + 1.0f);
-    dfc = -0.5f * s * (float)(kPi / rc);
+    float s, c;
+    sincos_quarter((float)(0.5 * kPi * (rc - r) / rc), s, c);
+    fc = s * s;
+    dfc = -(float)(kPi / rc) * s * c;
 }
 
 __device__ __forceinline__ double fcut_value(double r, double rc, Real<0>)
@@ -59,21 +80,33 @@ __device__ __forceinline__ double fcut_value(double r, double rc, Real<0>)
 
 __device__ __forceinline__ float fcut_value(double r, double rc, Real<1>)
 {
-    return 0.5f * (__builtin_amdgcn_cosf((float)(0.5 * r / rc)) + 1.0f);
+    float s, c;
+    sincos_quarter((float)(0.5 * kPi * (rc - r) / rc), s, c);
+    return s * s;
 }
 
-// ---- exp ----
+// ---- exp of an FP64 argument ----
 __device__ __forceinline__ double dexp(double x, Real<0>) { return exp(x); }
-__device__ __forceinline__ float dexp(float x, Real<1>) { return __expf(x); }
+
+// FP32-bulk exp: x log2(e) = n + f is split in FP64 (|f| <= 1/2), 2^f comes
+// from v_exp_f32 and 2^n from ldexp, so the relative error stays ~1e-7 for
+// any |x| (scaling the argument in FP32 would lose |x| * 6e-8).
+__device__ __forceinline__ float dexp(double x, Real<1>)
+{
+    const double t = x * 1.4426950408889634;
+    const double n = rint(t);
+    return ldexpf(__builtin_amdgcn_exp2f((float)(t - n)), (int)n);
+}
 
 // ---- cos / sin for g3 ----
 __device__ __forceinline__ void dsincos(double x, double &s, double &c, Real<0>) { sincos(x, &s, &c); }
-__device__ __forceinline__ void dsincos(float x, float &s, float &c, Real<1>)
+__device__ __forceinline__ void dsincos(double x, float &s, float &c, Real<1>) { sincosf((float)x, &s, &c); }
+
+// FP32 reciprocal refined by one Newton step (~0.5 ulp)
+__device__ __forceinline__ float rcp_refined(float d)
 {
-    constexpr float inv2pi = 0.15915494309189535f;
-    float t = x * inv2pi;
-    s = __builtin_amdgcn_sinf(t);
-    c = __builtin_amdgcn_cosf(t);
+    float r = __builtin_amdgcn_rcpf(d);
+    return r * (2.0f - d * r);
 }
 
 // ---- powers: base^zeta for base > 0 ----

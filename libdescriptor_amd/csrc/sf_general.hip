@@ -50,10 +50,10 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
             const int kd = tab.k2[q];
             if (kd == G2) {
                 const double dr = rij - tab.b2[q];
-                v = dexp((T)(-tab.a2[q] * dr * dr), R) * fij;
+                v = dexp(-tab.a2[q] * dr * dr, R) * fij;
             } else if (kd == G3) {
                 T s, c;
-                dsincos((T)(tab.a2[q] * rij), s, c, R);
+                dsincos(tab.a2[q] * rij, s, c, R);
                 v = c * fij;
             } else {
                 v = fij;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
                 if (base <= (T)0) continue;
                 const T cost = dpow(base, tab.zeta3[q], tab.izeta3[q], R);
                 const double esum = kd == G4 ? rijsq + riksq + rjksq : rijsq + riksq;
-                const T eterm = dexp((T)(-tab.eta3[q] * esum), R);
+                const T eterm = dexp(-tab.eta3[q] * esum, R);
                 T v = (T)tab.coef3[q] * cost * eterm * fij * fik;
                 if (kd == G4) v *= fjk;
                 out[tab.slot3[q] + pidx * tab.str3[q]] += (double)v;
@@ -153,11 +153,11 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
             T d;
             if (kd == G2) {
                 const double dr = rij - tab.b2[q];
-                const T ex = dexp((T)(-tab.a2[q] * dr * dr), R);
+                const T ex = dexp(-tab.a2[q] * dr * dr, R);
                 d = ex * (T)(-2.0 * tab.a2[q] * dr) * fij + ex * dfij;
             } else if (kd == G3) {
                 T s, c;
-                dsincos((T)(tab.a2[q] * rij), s, c, R);
+                dsincos(tab.a2[q] * rij, s, c, R);
                 d = -(T)tab.a2[q] * s * fij + c * dfij;
             } else {
                 d = dfij;
@@ -214,14 +214,14 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
                 const double eta = tab.eta3[q];
                 const T w = (T)g[tab.slot3[q] + pidx * tab.str3[q]] * (T)tab.coef3[q];
                 if (kd == G4) {
-                    const T ex = dexp((T)(-eta * (rijsq + riksq + rjksq)), R);
+                    const T ex = dexp(-eta * (rijsq + riksq + rjksq), R);
                     const T f3 = fij * fik * fjk;
                     const T we = w * ex;
                     aij += we * (dcost * dc_ij * f3 + cost * (T)(-2 * eta * rij) * f3 + cost * dfij * fik * fjk);
                     aik += we * (dcost * dc_ik * f3 + cost * (T)(-2 * eta * rik) * f3 + cost * fij * dfik * fjk);
                     ajk += we * (dcost * dc_jk * f3 + cost * (T)(-2 * eta * rjk) * f3 + cost * fij * fik * dfjk);
                 } else {
-                    const T ex = dexp((T)(-eta * (rijsq + riksq)), R);
+                    const T ex = dexp(-eta * (rijsq + riksq), R);
                     const T f2 = fij * fik;
                     const T we = w * ex;
                     aij += we * (dcost * dc_ij * f2 + cost * (T)(-2 * eta * rij) * f2 + cost * dfij * fik);

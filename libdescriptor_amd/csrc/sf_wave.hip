@@ -162,7 +162,7 @@ __device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const double
             if (e < wt.ne) t.E[e] = (T)exp(-wt.eta3[e] * S);
     } else {
         const double r2a = s_r2[a], r2b = s_r2[b];
-        const float inv = __builtin_amdgcn_rcpf((float)(2 * t.ra * t.rb));
+        const float inv = rcp_refined((float)(2 * t.ra * t.rb));
 #pragma unroll
         for (int l = 0; l < 2; ++l) {
             if (l < wt.nl) {
@@ -178,7 +178,7 @@ __device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const double
         const double S = r2a + r2b + t.rjk2;
 #pragma unroll
         for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = __expf((float)(-wt.eta3[e] * S));
+            if (e < wt.ne) t.E[e] = dexp(-wt.eta3[e] * S, R);
     }
     (void)R;
 }
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                 for (int q = 0; q < 8; ++q) {
                     if (q < wt.nr2) {
                         const double dr = r - wt.rs2[q];
-                        const double v = (double)(dexp((T)(-wt.eta2[q] * dr * dr), R) * fc);
+                        const double v = (double)(dexp(-wt.eta2[q] * dr * dr, R) * fc);
 #pragma unroll
                         for (int s = 0; s < NSB; ++s)
                             if (NSB == 1 || s == sj) acc2[s * 8 + q] += v;
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                     if (q < wt.nr2) {
                         const int slot = wt.slots[(NSB == 1 ? 0 : sj) * 8 + q];
                         const double dr = r - wt.rs2[q];
-                        const T ex = dexp((T)(-wt.eta2[q] * dr * dr), R);
+                        const T ex = dexp(-wt.eta2[q] * dr * dr, R);
                         const T dphi = ex * (T)(-2.0 * wt.eta2[q] * dr) * fc + ex * dfc;
                         s += g[slot] * (double)dphi;
                     }
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                             dcik = (T)((riksq - rijsq + rjksq) / (2 * q.ra * riksq));
                             dcjk = (T)(-q.rjk / (q.ra * q.rb));
                         } else {
-                            const float inv = __builtin_amdgcn_rcpf((float)(2 * q.ra * q.rb));
+                            const float inv = rcp_refined((float)(2 * q.ra * q.rb));
                             dcij = (float)(rijsq - riksq + rjksq) * inv * (float)(1.0 / q.ra);
                             dcik = (float)(riksq - rijsq + rjksq) * inv * (float)(1.0 / q.rb);
                             dcjk = -(float)q.rjk * inv * 2.0f;
