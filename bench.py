@@ -149,10 +149,15 @@ def main():
     err = {}
     for b, (lo, hi) in blocks.items():
         a, r = d32[:, lo:hi], d64[:, lo:hi]
+        scale = np.abs(r).max()
+        diff = np.abs(a - r)
+        sig = np.abs(r) >= 1e-6 * scale  # entries that carry signal in this block
+        rel = diff[sig] / np.abs(r)[sig]
         nz = np.abs(r) > 0
-        rel = np.abs(a - r)[nz] / np.abs(r)[nz]
         err[b] = {"max_rel": float(rel.max()) if rel.size else 0.0,
-                  "rms_rel": float(np.sqrt((rel ** 2).mean())) if rel.size else 0.0}
+                  "rms_rel": float(np.sqrt((rel ** 2).mean())) if rel.size else 0.0,
+                  "max_rel_normwise": float(diff.max() / scale) if scale > 0 else 0.0,
+                  "max_rel_all_nonzero": float((diff[nz] / np.abs(r)[nz]).max()) if nz.any() else 0.0}
     gscale = np.abs(g64).max()
     err["dE_dr"] = {"max_rel_normwise": float(np.abs(g32 - g64).max() / gscale),
                     "rms_rel_normwise": float(np.sqrt(((g32 - g64) ** 2).mean()) / gscale)}

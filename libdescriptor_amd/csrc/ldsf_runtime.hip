@@ -163,7 +163,7 @@ struct ldsf_plan {
     int *in_edges = nullptr;
     double *edge = nullptr;
     double *self = nullptr;
-    int *wave_slots = nullptr;
+    ldsf::WaveParams *wave_par = nullptr;
 
     ~ldsf_plan()
     {
@@ -171,7 +171,7 @@ struct ldsf_plan {
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device);
         for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
-                        (void *)edge, (void *)self, (void *)wave_slots})
+                        (void *)edge, (void *)self, (void *)wave_par})
             if (p) (void)hipFree(p);
         (void)hipSetDevice(cur);
     }
@@ -250,14 +250,14 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     plan->in_start = in_start.release_ownership();
     plan->in_edges = in_edges.release_ownership();
     // wave kernels: parameter set covered and the largest neighbourhood fits in LDS
-    int slot_table[ldsf::kWaveSlots];
-    if (ldsf::wave_build(sf, &plan->wave, slot_table) && plan->max_nn <= 1023 &&
+    ldsf::WaveParams wpar;
+    if (ldsf::wave_build(sf, &plan->wave, &wpar) && plan->max_nn <= 1023 &&
         ldsf::wave_lds_bytes(plan->max_nn, true) <= ldsf::kWaveMaxLds) {
-        HIP_TRY(hipMalloc((void **)&plan->wave_slots, sizeof(slot_table)), "hipMalloc(slots)");
-        HIP_TRY(hipMemcpy(plan->wave_slots, slot_table, sizeof(slot_table), hipMemcpyHostToDevice),
-                "hipMemcpy(slots)");
+        HIP_TRY(hipMalloc((void **)&plan->wave_par, sizeof(wpar)), "hipMalloc(wave parameters)");
+        HIP_TRY(hipMemcpy(plan->wave_par, &wpar, sizeof(wpar), hipMemcpyHostToDevice),
+                "hipMemcpy(wave parameters)");
         plan->wave.rc = tab.rc;
-        plan->wave.slots = plan->wave_slots;
+        plan->wave.par = plan->wave_par;
         plan->wave_ok = true;
     }
     plan->family = plan->wave_ok ? 1 : 0;

@@ -20,25 +20,32 @@ hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, co
 // cover parameter sets whose two-body rows are g2 (at most 8) and whose
 // three-body rows are g4 forming a grid zeta x lambda x eta (at most 4 x 2 x 2
 // distinct values), in the reference layout or the two-species resolved one.
+// Row parameters of the wave kernels; one copy in device memory per plan,
+// staged into LDS by every workgroup (uniform values read from LDS instead of
+// occupying scalar registers across the loops).
+struct WaveParams {
+    double eta2[8], rs2[8];      // g2 rows
+    double zeta[4], coef[4];     // distinct g4 zeta and 2^(1 - zeta)
+    double lam[2], eta3[2];      // distinct g4 lambda and eta
+    int izeta[4];                // integer zeta or -1
+    int lamcode[2];              // +1 / -1 for lambda = +1 / -1, else 0
+    int slots[64];               // [0,16): g2 slot of k = s*8+q; [16,64): g4 slot of
+                                 // p*16 + (z*2+l)*2+e; -1 unused
+};
 struct WaveTab {
     int n_species;
     int nr2;                // g2 rows (<= 8)
     int nz, nl, ne;         // distinct zeta, lambda, eta of the g4 grid
     int nsb, npb;           // species blocks (1 or 2) and pair blocks (1 or 3)
     int has3;               // any g4 row
-    double eta2[8], rs2[8];
-    double zeta[4], coef[4];
-    int izeta[4];
-    double lam[2];
-    double eta3[2];
     const double *rc;       // device, n_species^2
-    const int *slots;       // device: [0,16) two-body slot of k = s*8+q,
-                            //         [16,64) three-body slot of p*16 + (z*2+l)*2+e; -1 unused
+    const WaveParams *par;  // device
 };
 constexpr int kWaveSlots = 64;
-// host: true (and the table, with slots in *slot_table) if the wave kernels
-// cover this parameter set
-bool wave_build(const ldsf_sf *sf, WaveTab *wt, int *slot_table);
+constexpr int kMaxWaveSpecies = 8;
+// host: true (and the table and parameters) if the wave kernels cover this
+// parameter set
+bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *par);
 // bytes of LDS a launch needs for the largest neighbour count; the wave
 // kernels run only when this fits
 size_t wave_lds_bytes(int max_nn, bool adjoint);

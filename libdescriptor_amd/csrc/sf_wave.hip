@@ -48,22 +48,40 @@ __device__ __forceinline__ int prefix_count(unsigned long long m, int lane)
 
 // Sum K per-lane values over the wave.  Afterwards v[0] of lane l holds the
 // total of value (l >> (6 - log2 K)) & (K - 1).  The butterfly order is fixed.
+// Halving step with H live values, partner lane l ^ M (all compile-time).
+template <int K, int H, int M>
+struct Halve {
+    static __device__ __forceinline__ void run(double (&v)[K], int lane)
+    {
+        const bool hi = (lane & M) != 0;
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            const double send = hi ? v[q] : v[q + H];
+            const double keep = hi ? v[q + H] : v[q];
+            v[q] = keep + __shfl_xor(send, M);
+        }
+        Halve<K, H / 2, M / 2>::run(v, lane);
+    }
+};
+
+template <int K, int M>
+struct Halve<K, 0, M> {
+    static __device__ __forceinline__ void run(double (&v)[K], int)
+    {
+        v[0] += __shfl_xor(v[0], M);
+        Halve<K, 0, M / 2>::run(v, 0);
+    }
+};
+
+template <int K>
+struct Halve<K, 0, 0> {
+    static __device__ __forceinline__ void run(double (&)[K], int) {}
+};
+
 template <int K>
 __device__ __forceinline__ void reduce_scatter(double (&v)[K], int lane)
 {
-    int m = 32;
-#pragma unroll
-    for (int h = K / 2; h >= 1; h /= 2) {
-        const bool hi = (lane & m) != 0;
-#pragma unroll
-        for (int q = 0; q < h; ++q) {
-            const double send = hi ? v[q] : v[q + h];
-            const double keep = hi ? v[q + h] : v[q];
-            v[q] = keep + __shfl_xor(send, m);
-        }
-        m >>= 1;
-    }
-    for (; m >= 1; m >>= 1) v[0] += __shfl_xor(v[0], m);
+    Halve<K, K / 2, 32>::run(v, lane);
 }
 
 template <int K>
@@ -133,7 +151,8 @@ struct Trip {
 };
 
 template <int P>
-__device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const double *s_dx,
+__device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const WaveParams &wp,
+                                                 const double *s_dx,
                                                  const double *s_dy, const double *s_dz,
                                                  const double *s_r, const double *s_r2, int a,
                                                  int b, double rcjk, Trip<P> &t)
@@ -156,56 +175,59 @@ __device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const double
         const double S = rijsq + riksq + rjksq;
 #pragma unroll
         for (int l = 0; l < 2; ++l)
-            if (l < wt.nl) t.B[l] = (T)(1 + wt.lam[l] * cosv);
+            if (l < wt.nl) t.B[l] = (T)(1 + wp.lam[l] * cosv);
 #pragma unroll
         for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = (T)exp(-wt.eta3[e] * S);
+            if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * S);
     } else {
         const double r2a = s_r2[a], r2b = s_r2[b];
         const float inv = rcp_refined((float)(2 * t.ra * t.rb));
 #pragma unroll
         for (int l = 0; l < 2; ++l) {
             if (l < wt.nl) {
-                const double lam = wt.lam[l];
-                if (lam == 1.0)
+                const int code = __builtin_amdgcn_readfirstlane(wp.lamcode[l]);
+                if (code == 1)
                     t.B[l] = (float)((t.ra + t.rb) * (t.ra + t.rb) - t.rjk2) * inv;
-                else if (lam == -1.0)
+                else if (code == -1)
                     t.B[l] = (float)(t.rjk2 - (t.ra - t.rb) * (t.ra - t.rb)) * inv;
                 else
-                    t.B[l] = 1.0f + (float)lam * ((float)(r2a + r2b - t.rjk2) * inv);
+                    t.B[l] = 1.0f + (float)wp.lam[l] * ((float)(r2a + r2b - t.rjk2) * inv);
             }
         }
         const double S = r2a + r2b + t.rjk2;
 #pragma unroll
         for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = dexp(-wt.eta3[e] * S, R);
+            if (e < wt.ne) t.E[e] = dexp(-wp.eta3[e] * S, R);
     }
     (void)R;
 }
 
 template <typename T>
-__device__ __forceinline__ T power_of(T b, int z, const WaveTab &wt)
+__device__ __forceinline__ T power_of(T b, int z, const WaveParams &wp)
 {
     if (!(b > (T)0)) return (T)0;
-    const int iz = wt.izeta[z];
+    const int iz = __builtin_amdgcn_readfirstlane(wp.izeta[z]);
     if (iz >= 0) return ipow(b, iz);
-    if (sizeof(T) == 8) return (T)pow((double)b, wt.zeta[z]);
-    return (T)__builtin_amdgcn_exp2f((float)wt.zeta[z] * __builtin_amdgcn_logf((float)b));
+    if (sizeof(T) == 8) return (T)pow((double)b, wp.zeta[z]);
+    return (T)__builtin_amdgcn_exp2f((float)wp.zeta[z] * __builtin_amdgcn_logf((float)b));
 }
 
 template <typename T>
-__device__ __forceinline__ T power_m1_of(T b, int z, const WaveTab &wt)
+__device__ __forceinline__ T power_m1_of(T b, int z, const WaveParams &wp)
 {
     if (!(b > (T)0)) return (T)0;
-    const int iz = wt.izeta[z];
+    const int iz = __builtin_amdgcn_readfirstlane(wp.izeta[z]);
     if (iz >= 1) return ipow(b, iz - 1);
-    if (sizeof(T) == 8) return (T)pow((double)b, wt.zeta[z] - 1.0);
-    return (T)__builtin_amdgcn_exp2f((float)(wt.zeta[z] - 1.0) * __builtin_amdgcn_logf((float)b));
+    if (sizeof(T) == 8) return (T)pow((double)b, wp.zeta[z] - 1.0);
+    return (T)__builtin_amdgcn_exp2f((float)(wp.zeta[z] - 1.0) * __builtin_amdgcn_logf((float)b));
 }
 
 // LDS carve-up shared by both kernels (all arrays sized for maxn neighbours)
 struct Lds {
-    double *dx, *dy, *dz, *r, *r2, *fc, *dfc, *fx, *fy, *fz, *rcij;
+    WaveParams *par;  // staged row parameters
+    double *w, *wv;   // adjoint row weights of the current pair block (16 each)
+    double *rc;       // cutoff matrix, n_species^2 <= kMaxWaveSpecies^2
+    double *dx, *dy, *dz, *r, *r2, *fc, *dfc, *fx, *fy, *fz;
     int *sp, *orig, *l0, *l1;
 };
 
@@ -213,13 +235,17 @@ __device__ __forceinline__ Lds carve(double *base, int maxn, int tmax, bool adjo
 {
     Lds L;
     double *p = base;
+    L.par = reinterpret_cast<WaveParams *>(p);
+    p += (sizeof(WaveParams) + 7) / 8;
+    L.rc = p; p += kMaxWaveSpecies * kMaxWaveSpecies;
+    L.w = p; p += 16;
+    L.wv = p; p += 16;
     L.dx = p; p += maxn;
     L.dy = p; p += maxn;
     L.dz = p; p += maxn;
     L.r = p; p += maxn;
     L.r2 = p; p += maxn;
     L.fc = p; p += maxn;
-    L.rcij = p; p += maxn;
     if (adjoint) {
         L.dfc = p; p += maxn;
         L.fx = p; p += maxn;
@@ -254,7 +280,7 @@ __device__ __forceinline__ int enumerate_triplets(const WaveTab &wt, const Lds &
             const double e0 = L.dx[b] - L.dx[a], e1 = L.dy[b] - L.dy[a], e2 = L.dz[b] - L.dz[a];
             const double rjk2 = e0 * e0 + e1 * e1 + e2 * e2;
             const int sa = L.sp[a], sb = L.sp[b];
-            const double rcjk = wt.rc[sa * ns + sb];
+            const double rcjk = L.rc[sa * ns + sb];
             // the reference's test is sqrt(r_jk^2) < rc_jk; the squared form differs
             // only within an ulp of rc_jk, where fc_jk vanishes quadratically
             act = (P == 0) ? (sqrt(rjk2) < rcjk) : (rjk2 < rcjk * rcjk);
@@ -309,6 +335,14 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
     const int lane = threadIdx.x;
     const int ns = wt.n_species;
     const CentreRange cr = centre_range(n_centers);
+    for (int k = lane; k < ns * ns; k += kW) L.rc[k] = wt.rc[k];
+    {
+        const int *src = reinterpret_cast<const int *>(wt.par);
+        int *dst = reinterpret_cast<int *>(L.par);
+        for (int k = lane; k < (int)(sizeof(WaveParams) / 4); k += kW) dst[k] = src[k];
+    }
+    __syncthreads();
+    const WaveParams &wp = *L.par;
     for (int c = cr.first; c < cr.hi; c += cr.step) {
         const int i = centers ? centers[c] : c;
         const int beg = offs[c], N = offs[c + 1] - beg;
@@ -334,7 +368,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                 d2 = x[3 * (size_t)j + 2] - xi2;
                 r2 = d0 * d0 + d1 * d1 + d2 * d2;
                 r = sqrt(r2);
-                rcij = wt.rc[si * ns + sj];
+                rcij = L.rc[si * ns + sj];
                 valid = r < rcij;  // r > rc is skipped; r == rc adds exactly 0
             }
             const unsigned long long m = __ballot(valid);
@@ -351,8 +385,8 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     if (q < wt.nr2) {
-                        const double dr = r - wt.rs2[q];
-                        const double v = (double)(dexp(-wt.eta2[q] * dr * dr, R) * fc);
+                        const double dr = r - wp.rs2[q];
+                        const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
 #pragma unroll
                         for (int s = 0; s < NSB; ++s)
                             if (NSB == 1 || s == sj) acc2[s * 8 + q] += v;
@@ -361,6 +395,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
             }
             M += __popcll(m);
         }
+        reduce_add<NSB * 8>(acc2, lane, wp.slots, row);
         __syncthreads();
 
         // ---- triplets ----
@@ -378,9 +413,9 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                     if (t < nb) {
                         const int code = lst[t];
                         const int a = code & 1023, b = (code >> 10) & 1023;
-                        const double rcjk = wt.rc[L.sp[a] * ns + L.sp[b]];
+                        const double rcjk = L.rc[L.sp[a] * ns + L.sp[b]];
                         Trip<P> g;
-                        triplet_geometry<P>(wt, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, g);
+                        triplet_geometry<P>(wt, wp, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, g);
                         const T fjk = fcut_value(g.rjk, rcjk, R);
                         const T fc3 = s_fc[a] * s_fc[b] * fjk;
                         T te[2];
@@ -392,7 +427,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
 #pragma unroll
                                 for (int l = 0; l < 2; ++l) {
                                     if (l < wt.nl) {
-                                        const T pz = (T)wt.coef[z] * power_of<T>(g.B[l], z, wt);
+                                        const T pz = (T)wp.coef[z] * power_of<T>(g.B[l], z, wp);
 #pragma unroll
                                         for (int e = 0; e < 2; ++e)
                                             if (e < wt.ne) acc3[(z * 2 + l) * 2 + e] += (double)(pz * te[e]);
@@ -402,11 +437,10 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                         }
                     }
                 }
-                reduce_add<16>(acc3, lane, wt.slots + 16 + p * 16, row);
+                reduce_add<16>(acc3, lane, wp.slots + 16 + p * 16, row);
                 if (NPB > 1) __syncthreads();
             }
         }
-        reduce_add<NSB * 8>(acc2, lane, wt.slots, row);
         __syncthreads();
     }
 }
@@ -432,6 +466,14 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     const int lane = threadIdx.x;
     const int ns = wt.n_species;
     const CentreRange cr = centre_range(n_centers);
+    for (int k = lane; k < ns * ns; k += kW) L.rc[k] = wt.rc[k];
+    {
+        const int *src = reinterpret_cast<const int *>(wt.par);
+        int *dst = reinterpret_cast<int *>(L.par);
+        for (int k = lane; k < (int)(sizeof(WaveParams) / 4); k += kW) dst[k] = src[k];
+    }
+    __syncthreads();
+    const WaveParams &wp = *L.par;
     for (int c = cr.first; c < cr.hi; c += cr.step) {
         const int i = centers ? centers[c] : c;
         const int beg = offs[c], N = offs[c + 1] - beg;
@@ -454,7 +496,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 d2 = x[3 * (size_t)j + 2] - xi2;
                 r2 = d0 * d0 + d1 * d1 + d2 * d2;
                 r = sqrt(r2);
-                rcij = wt.rc[si * ns + sj];
+                rcij = L.rc[si * ns + sj];
                 valid = r < rcij;
                 if (!valid) {
                     double *ej = edge + 3 * (size_t)(beg + jj);
@@ -472,10 +514,10 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     if (q < wt.nr2) {
-                        const int slot = wt.slots[(NSB == 1 ? 0 : sj) * 8 + q];
-                        const double dr = r - wt.rs2[q];
-                        const T ex = dexp(-wt.eta2[q] * dr * dr, R);
-                        const T dphi = ex * (T)(-2.0 * wt.eta2[q] * dr) * fc + ex * dfc;
+                        const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
+                        const double dr = r - wp.rs2[q];
+                        const T ex = dexp(-wp.eta2[q] * dr * dr, R);
+                        const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
                         s += g[slot] * (double)dphi;
                     }
                 }
@@ -504,27 +546,23 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
             for (int p = 0; p < NPB; ++p) {
                 int nb = 0;
                 const int *lst = block_list<NPB>(L, cnt, p, lane, nb);
-                // row weights of this block (uniform): dE/dzeta * 2^(1-zeta)
-                T w[4][2][2], wv[4][2][2];
-#pragma unroll
-                for (int z = 0; z < 4; ++z)
-#pragma unroll
-                    for (int l = 0; l < 2; ++l)
-#pragma unroll
-                        for (int e = 0; e < 2; ++e) {
-                            const int slot = wt.slots[16 + p * 16 + (z * 2 + l) * 2 + e];
-                            const double ww = slot >= 0 ? g[slot] * wt.coef[z] : 0.0;
-                            w[z][l][e] = (T)ww;
-                            wv[z][l][e] = (T)(ww * wt.zeta[z] * wt.lam[l]);
-                        }
+                // row weights of this block: dE/dzeta * 2^(1-zeta), and times zeta lambda
+                if (lane < 16) {
+                    const int z = lane >> 2, l = (lane >> 1) & 1;
+                    const int slot = wp.slots[16 + p * 16 + lane];
+                    const double ww = slot >= 0 ? g[slot] * wp.coef[z] : 0.0;
+                    L.w[lane] = ww;
+                    L.wv[lane] = ww * wp.zeta[z] * wp.lam[l];
+                }
+                __syncthreads();
                 for (int t0 = 0; t0 < nb; t0 += kW) {
                     const int t = t0 + lane;
                     if (t < nb) {
                         const int code = lst[t];
                         const int a = code & 1023, b = (code >> 10) & 1023;
-                        const double rcjk = wt.rc[L.sp[a] * ns + L.sp[b]];
+                        const double rcjk = L.rc[L.sp[a] * ns + L.sp[b]];
                         Trip<P> q;
-                        triplet_geometry<P>(wt, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, q);
+                        triplet_geometry<P>(wt, wp, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, q);
                         T fjk, dfjk;
                         fcut(q.rjk, rcjk, fjk, dfjk, R);
                         const T fa = s_fc[a], fb = s_fc[b], dfa = s_dfc[a], dfb = s_dfc[b];
@@ -535,12 +573,12 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
 #pragma unroll
                                 for (int l = 0; l < 2; ++l) {
                                     if (l < wt.nl) {
-                                        const T pm1 = power_m1_of<T>(q.B[l], z, wt);
+                                        const T pm1 = power_m1_of<T>(q.B[l], z, wp);
                                         const T pw = (q.B[l] > (T)0) ? pm1 * q.B[l] : (T)0;
 #pragma unroll
                                         for (int e = 0; e < 2; ++e) {
-                                            U[e] += w[z][l][e] * pw;
-                                            V[e] += wv[z][l][e] * pm1;
+                                            U[e] += (T)L.w[(z * 2 + l) * 2 + e] * pw;
+                                            V[e] += (T)L.wv[(z * 2 + l) * 2 + e] * pm1;
                                         }
                                     }
                                 }
@@ -553,7 +591,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                 SA += V[e] * q.E[e];
                                 const T ue = U[e] * q.E[e];
                                 SB += ue;
-                                SC += (T)wt.eta3[e] * ue;
+                                SC += (T)wp.eta3[e] * ue;
                             }
                         }
                         // partial derivatives of cos(theta) w.r.t. r_ij, r_ik, r_jk
@@ -623,15 +661,20 @@ size_t wave_lds_bytes(int max_nn, bool adjoint)
 {
     const size_t maxn = std::max(max_nn, 1);
     const size_t tmax = std::max<size_t>(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * maxn * (adjoint ? 11 : 7) + sizeof(int) * (2 * maxn + 2 * tmax);
+    return sizeof(double) * ((sizeof(WaveParams) + 7) / 8 + kMaxWaveSpecies * kMaxWaveSpecies + 32 +
+                             maxn * (adjoint ? 10 : 6)) +
+           sizeof(int) * (2 * maxn + 2 * tmax);
 }
 
-bool wave_build(const ldsf_sf *sf, WaveTab *wt, int *slots)
+bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
 {
     std::memset(wt, 0, sizeof(*wt));
+    std::memset(wp, 0, sizeof(*wp));
+    int *slots = wp->slots;
     for (int k = 0; k < kWaveSlots; ++k) slots[k] = -1;
     const int ns = (int)sf->species.size();
     wt->n_species = ns;
+    if (ns > kMaxWaveSpecies) return false;
     if (sf->species_resolved && ns > 2) return false;
     wt->nsb = (sf->species_resolved && ns == 2) ? 2 : 1;
     wt->npb = (sf->species_resolved && ns == 2) ? 3 : 1;
@@ -640,8 +683,8 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, int *slots)
     if (n2 > 8) return false;
     for (int q = 0; q < n2; ++q) {
         if (sf->k2[q] != G2) return false;
-        wt->eta2[q] = sf->a2[q];
-        wt->rs2[q] = sf->b2[q];
+        wp->eta2[q] = sf->a2[q];
+        wp->rs2[q] = sf->b2[q];
         for (int s = 0; s < wt->nsb; ++s) slots[s * 8 + q] = sf->slot2[q] + s * sf->str2[q];
     }
     wt->nr2 = n2;
@@ -670,14 +713,17 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, int *slots)
     wt->nl = (int)ls.size();
     wt->ne = (int)es.size();
     for (int z = 0; z < wt->nz; ++z) {
-        wt->zeta[z] = zs[z];
-        wt->coef[z] = std::pow(2, 1 - zs[z]);
+        wp->zeta[z] = zs[z];
+        wp->coef[z] = std::pow(2, 1 - zs[z]);
         int iz = -1;
         if (zs[z] >= 0 && zs[z] <= kMaxIntZeta && zs[z] == std::floor(zs[z])) iz = (int)zs[z];
-        wt->izeta[z] = iz;
+        wp->izeta[z] = iz;
     }
-    for (int l = 0; l < wt->nl; ++l) wt->lam[l] = ls[l];
-    for (int e = 0; e < wt->ne; ++e) wt->eta3[e] = es[e];
+    for (int l = 0; l < wt->nl; ++l) {
+        wp->lam[l] = ls[l];
+        wp->lamcode[l] = ls[l] == 1.0 ? 1 : (ls[l] == -1.0 ? -1 : 0);
+    }
+    for (int e = 0; e < wt->ne; ++e) wp->eta3[e] = es[e];
     for (auto &kv : seen) {
         const int q = kv.second;
         for (int p = 0; p < wt->npb; ++p) slots[16 + p * 16 + kv.first] = sf->slot3[q] + p * sf->str3[q];
