@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--family", type=int, default=-1, help="force kernel family (0 general, 1 wave)")
+    ap.add_argument("--only", choices=["fp64", "fp32_bulk"], default=None,
+                    help="time one form only and print its timings (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path "
@@ -112,7 +114,10 @@ def main():
 
     res = {}
     outs = {}
-    for name, prec in (("fp64", _capi.FP64), ("fp32_bulk", _capi.FP32_BULK)):
+    forms = (("fp64", _capi.FP64), ("fp32_bulk", _capi.FP32_BULK))
+    if args.only:
+        forms = tuple(f for f in forms if f[0] == args.only)
+    for name, prec in forms:
         for _ in range(args.warmup):
             step(prec)
         torch.cuda.synchronize()
@@ -140,6 +145,10 @@ def main():
         outs[name] = (desc.cpu().numpy(), dx.cpu().numpy())
         res[name] = dict(ms_per_step=ms, fwd_ms=fwd_ms, adj_ms=adj_ms)
         log(f"{name}: {ms:.3f} ms/step (forward {fwd_ms:.3f} ms, adjoint+gather {adj_ms:.3f} ms)")
+
+    if args.only:
+        print(json.dumps({"only": args.only, **res[args.only]}), flush=True)
+        return
 
     # FP32-bulk error against FP64, per block
     d64, g64 = outs["fp64"]

@@ -161,8 +161,8 @@ struct ldsf_plan {
     int *offs = nullptr;
     int *in_start = nullptr;
     int *in_edges = nullptr;
+    int *inv_pos = nullptr;
     double *edge = nullptr;
-    double *self = nullptr;
     ldsf::WaveParams *wave_par = nullptr;
 
     ~ldsf_plan()
@@ -171,7 +171,7 @@ struct ldsf_plan {
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device);
         for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
-                        (void *)edge, (void *)self, (void *)wave_par})
+                        (void *)inv_pos, (void *)edge, (void *)wave_par})
             if (p) (void)hipFree(p);
         (void)hipSetDevice(cur);
     }
@@ -245,10 +245,14 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, in_start.p,
                                            count.p, in_edges.p, s), "fill");
     HIP_TRY(ldsf::launch_sort_buckets(n_total, in_start.p, in_edges.p, s), "sort");
+    DevBuf<int> inv_pos;
+    HIP_TRY(inv_pos.alloc(plan->n_edges + n_centers), "hipMalloc(inv_pos)");
+    HIP_TRY(ldsf::launch_inverse(plan->n_edges + n_centers, in_edges.p, inv_pos.p, s), "inverse");
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     plan->offs = offs.release_ownership();
     plan->in_start = in_start.release_ownership();
     plan->in_edges = in_edges.release_ownership();
+    plan->inv_pos = inv_pos.release_ownership();
     // wave kernels: parameter set covered and the largest neighbourhood fits in LDS
     ldsf::WaveParams wpar;
     if (ldsf::wave_build(sf, &plan->wave, &wpar) && plan->max_nn <= 1023 &&
@@ -286,21 +290,21 @@ int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dc
 {
     ldsf::SFTab tab;
     if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
-    if (!plan->edge && plan->n_edges > 0)
-        HIP_TRY(hipMalloc((void **)&plan->edge, sizeof(double) * 3 * plan->n_edges), "hipMalloc(edge)");
-    if (!plan->self && plan->n_centers > 0)
-        HIP_TRY(hipMalloc((void **)&plan->self, sizeof(double) * 3 * plan->n_centers), "hipMalloc(self)");
+    const long n_rec = plan->n_edges + plan->n_centers;
+    if (!plan->edge && n_rec > 0)
+        HIP_TRY(hipMalloc((void **)&plan->edge, sizeof(double) * 3 * n_rec), "hipMalloc(edge)");
     hipError_t e;
     if (plan->family == 1 && plan->wave_ok)
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
-                                  plan->offs, plan->nl, plan->species, x, dE, plan->edge,
-                                  plan->self, tab.width, plan->max_nn, s);
+                                  plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
+                                  plan->edge, (int)plan->n_edges, tab.width, plan->max_nn, s);
     else
         e = ldsf::launch_adj_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
-                                     plan->nl, plan->species, x, dE, plan->edge, plan->self, s);
+                                     plan->nl, plan->species, x, dE, plan->inv_pos, plan->edge,
+                                     (int)plan->n_edges, s);
     HIP_TRY(e, "adjoint kernel launch");
-    HIP_TRY(ldsf::launch_gather(plan->n_total, plan->in_start, plan->in_edges, plan->n_edges,
-                                plan->edge, plan->self, dcoords, s), "gather launch");
+    HIP_TRY(ldsf::launch_gather(plan->n_total, plan->in_start, plan->edge, dcoords, s),
+            "gather launch");
     return LDSF_OK;
 }
 

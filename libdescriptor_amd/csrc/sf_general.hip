@@ -117,8 +117,8 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
                                                      const int *__restrict__ species,
                                                      const double *__restrict__ x,
                                                      const double *__restrict__ dE,
-                                                     double *__restrict__ edge,
-                                                     double *__restrict__ self)
+                                                     const int *__restrict__ inv_pos,
+                                                     double *__restrict__ edge, int n_edges)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -129,9 +129,10 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
     const double *g = dE + (size_t)t * tab.width;
     const int beg = offs[t], end = offs[t + 1];
     for (int e = beg; e < end; ++e) {
-        edge[3 * (size_t)e] = 0.0;
-        edge[3 * (size_t)e + 1] = 0.0;
-        edge[3 * (size_t)e + 2] = 0.0;
+        double *v = edge + 3 * (size_t)inv_pos[e];
+        v[0] = 0.0;
+        v[1] = 0.0;
+        v[2] = 0.0;
     }
     const int si = species[i];
     const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
         }
         const double ir = 1.0 / rij;
         double v0 = sp * dj0 * ir, v1 = sp * dj1 * ir, v2 = sp * dj2 * ir;
-        double *ej = edge + 3 * (size_t)jj;
+        double *ej = edge + 3 * (size_t)inv_pos[jj];
         ej[0] += v0; ej[1] += v1; ej[2] += v2;
         s0 -= v0; s1 -= v1; s2 -= v2;
         if (tab.n3 == 0) continue;
@@ -235,14 +236,15 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
             const double vik0 = a2 * dk0, vik1 = a2 * dk1, vik2 = a2 * dk2;
             const double vjk0 = a3 * e0, vjk1 = a3 * e1, vjk2 = a3 * e2;
             ej[0] += vij0 - vjk0; ej[1] += vij1 - vjk1; ej[2] += vij2 - vjk2;
-            double *ek = edge + 3 * (size_t)kk;
+            double *ek = edge + 3 * (size_t)inv_pos[kk];
             ek[0] += vik0 + vjk0; ek[1] += vik1 + vjk1; ek[2] += vik2 + vjk2;
             s0 -= vij0 + vik0; s1 -= vij1 + vik1; s2 -= vij2 + vik2;
         }
     }
-    self[3 * (size_t)t] = s0;
-    self[3 * (size_t)t + 1] = s1;
-    self[3 * (size_t)t + 2] = s2;
+    double *vs = edge + 3 * (size_t)inv_pos[n_edges + t];
+    vs[0] = s0;
+    vs[1] = s1;
+    vs[2] = s2;
 }
 
 }  // namespace
@@ -264,16 +266,17 @@ hipError_t launch_fwd_general(const SFTab &tab, int precision, int n_centers, co
 
 hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, const int *centers,
                               const int *offs, const int *nl, const int *species, const double *x,
-                              const double *dE, double *edge, double *self, hipStream_t s)
+                              const double *dE, const int *inv_pos, double *edge, int n_edges,
+                              hipStream_t s)
 {
     if (n_centers <= 0) return hipSuccess;
     dim3 block(256), grid((n_centers + 255) / 256);
     if (precision == 0)
         hipLaunchKernelGGL(k_adj_general<0>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
-                           species, x, dE, edge, self);
+                           species, x, dE, inv_pos, edge, n_edges);
     else
         hipLaunchKernelGGL(k_adj_general<1>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
-                           species, x, dE, edge, self);
+                           species, x, dE, inv_pos, edge, n_edges);
     return hipGetLastError();
 }
 

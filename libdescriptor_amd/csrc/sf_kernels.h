@@ -14,7 +14,8 @@ hipError_t launch_fwd_general(const SFTab &tab, int precision, int n_centers, co
                               double *desc, hipStream_t s);
 hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, const int *centers,
                               const int *offs, const int *nl, const int *species, const double *x,
-                              const double *dE, double *edge, double *self, hipStream_t s);
+                              const double *dE, const int *inv_pos, double *edge, int n_edges,
+                              hipStream_t s);
 
 // sf_wave.hip: one wavefront per centre, neighbours staged in LDS.  They
 // cover parameter sets whose two-body rows are g2 (at most 8) and whose
@@ -29,6 +30,7 @@ struct WaveParams {
     double lam[2], eta3[2];      // distinct g4 lambda and eta
     int izeta[4];                // integer zeta or -1
     int lamcode[2];              // +1 / -1 for lambda = +1 / -1, else 0
+    int nsq;                     // squarings needed for the largest integer zeta
     int slots[64];               // [0,16): g2 slot of k = s*8+q; [16,64): g4 slot of
                                  // p*16 + (z*2+l)*2+e; -1 unused
 };
@@ -53,10 +55,12 @@ constexpr size_t kWaveMaxLds = 64 * 1024;
 hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            double *desc, int width, int max_nn, hipStream_t s);
+// The adjoint kernels write the vector of edge e (and of centre c, as edge
+// n_edges + c) at record inv_pos[e] of `edge`, the order the gather reads.
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
-                           const double *dE, double *edge, double *self, int width, int max_nn,
-                           hipStream_t s);
+                           const double *dE, const int *inv_pos, double *edge, int n_edges,
+                           int width, int max_nn, hipStream_t s);
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,
@@ -66,8 +70,9 @@ hipError_t launch_edge_targets_count(long n_edges, int n_centers, const int *nl,
 hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, const int *centers,
                                     const int *in_start, int *cursor, int *in_edges, hipStream_t s);
 hipError_t launch_sort_buckets(int n_total, const int *in_start, int *in_edges, hipStream_t s);
-hipError_t launch_gather(int n_total, const int *in_start, const int *in_edges, long n_edges,
-                         const double *edge, const double *self, double *dcoords, hipStream_t s);
+hipError_t launch_inverse(long n, const int *in_edges, int *inv_pos, hipStream_t s);
+hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
+                         hipStream_t s);
 hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s);
 hipError_t launch_scan_int(int n, const int *in, int *out_exclusive_plus_total, void *tmp,
                            size_t *tmp_bytes, hipStream_t s);

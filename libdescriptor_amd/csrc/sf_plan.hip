@@ -10,6 +10,8 @@
 // on every run and in every batch.  Centre self-terms are edges too (ids
 // n_edges + t), so one gather writes d_coordinates, padding rows included
 // (the layout the reference's gradient returns, python_bindings.cpp:235,252).
+// The adjoint kernels store each vector at its position in that transposed
+// order (inv_pos), so the gather reads every row's records contiguously.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -89,22 +91,26 @@ __global__ void k_sort(int n_total, const int *__restrict__ in_start, int *in_ed
     }
 }
 
+__global__ void k_inverse(long n, const int *__restrict__ in_edges, int *__restrict__ inv_pos)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) inv_pos[in_edges[p]] = (int)p;
+}
+
+// records of row a are contiguous: [in_start[a], in_start[a+1]), in ascending
+// edge id, written there by the adjoint kernels
 __global__ void k_gather(int n_total, const int *__restrict__ in_start,
-                         const int *__restrict__ in_edges, long n_edges,
-                         const double *__restrict__ edge, const double *__restrict__ self,
-                         double *__restrict__ dcoords)
+                         const double *__restrict__ edge, double *__restrict__ dcoords)
 {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_total) return;
     int b = in_start[a], e = in_start[a + 1];
     if (b == e) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int p = b; p < e; ++p) {
-        long id = in_edges[p];
-        const double *v = id < n_edges ? edge + 3 * id : self + 3 * (id - n_edges);
-        s0 += v[0];
-        s1 += v[1];
-        s2 += v[2];
+    for (long p = b; p < e; ++p) {
+        s0 += edge[3 * p];
+        s1 += edge[3 * p + 1];
+        s2 += edge[3 * p + 2];
     }
     dcoords[3 * (long)a] += s0;
     dcoords[3 * (long)a + 1] += s1;
@@ -160,12 +166,19 @@ hipError_t launch_sort_buckets(int n_total, const int *in_start, int *in_edges, 
     return hipGetLastError();
 }
 
-hipError_t launch_gather(int n_total, const int *in_start, const int *in_edges, long n_edges,
-                         const double *edge, const double *self, double *dcoords, hipStream_t s)
+hipError_t launch_inverse(long n, const int *in_edges, int *inv_pos, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_inverse, grid_for(n), dim3(kBlock), 0, s, n, in_edges, inv_pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
+                         hipStream_t s)
 {
     if (n_total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start,
-                       in_edges, n_edges, edge, self, dcoords);
+    hipLaunchKernelGGL(k_gather, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start, edge,
+                       dcoords);
     return hipGetLastError();
 }
 

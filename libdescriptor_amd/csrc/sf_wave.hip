@@ -202,24 +202,43 @@ __device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const WavePa
     (void)R;
 }
 
+// B^(2^k), k = 0..nsq (nsq uniform): every integer zeta <= 63 of the grid is a
+// product of these, so one squaring chain serves all rows of a triplet
 template <typename T>
-__device__ __forceinline__ T power_of(T b, int z, const WaveParams &wp)
+__device__ __forceinline__ void squarings(T b, int nsq, T (&sq)[6])
 {
-    if (!(b > (T)0)) return (T)0;
-    const int iz = __builtin_amdgcn_readfirstlane(wp.izeta[z]);
-    if (iz >= 0) return ipow(b, iz);
-    if (sizeof(T) == 8) return (T)pow((double)b, wp.zeta[z]);
-    return (T)__builtin_amdgcn_exp2f((float)wp.zeta[z] * __builtin_amdgcn_logf((float)b));
+    sq[0] = b;
+#pragma unroll
+    for (int k = 1; k < 6; ++k) sq[k] = (k <= nsq) ? sq[k - 1] * sq[k - 1] : (T)0;
 }
 
+// product of sq[k] over the set bits of `bits` (uniform)
 template <typename T>
-__device__ __forceinline__ T power_m1_of(T b, int z, const WaveParams &wp)
+__device__ __forceinline__ T pow_bits(const T (&sq)[6], int bits)
 {
-    if (!(b > (T)0)) return (T)0;
+    T r = (T)1;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if (bits & (1 << k)) r *= sq[k];
+    return r;
+}
+
+// b^zeta_z (with zeta_z - 1 when minus_one) for b > 0; 0 for b <= 0 as the
+// reference's guard (src/SymmetryFunctions.cpp:121)
+template <typename T>
+__device__ __forceinline__ T power_z(T b, const T (&sq)[6], int z, const WaveParams &wp,
+                                     bool minus_one)
+{
     const int iz = __builtin_amdgcn_readfirstlane(wp.izeta[z]);
-    if (iz >= 1) return ipow(b, iz - 1);
-    if (sizeof(T) == 8) return (T)pow((double)b, wp.zeta[z] - 1.0);
-    return (T)__builtin_amdgcn_exp2f((float)(wp.zeta[z] - 1.0) * __builtin_amdgcn_logf((float)b));
+    T v;
+    if (iz >= (minus_one ? 1 : 0)) {
+        v = pow_bits(sq, minus_one ? iz - 1 : iz);
+    } else {
+        const double zz = wp.zeta[z] - (minus_one ? 1.0 : 0.0);
+        if (sizeof(T) == 8) v = (T)pow((double)b, zz);
+        else v = (T)__builtin_amdgcn_exp2f((float)zz * __builtin_amdgcn_logf((float)b));
+    }
+    return b > (T)0 ? v : (T)0;
 }
 
 // LDS carve-up shared by both kernels (all arrays sized for maxn neighbours)
@@ -343,6 +362,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
     }
     __syncthreads();
     const WaveParams &wp = *L.par;
+    const int wp_nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     for (int c = cr.first; c < cr.hi; c += cr.step) {
         const int i = centers ? centers[c] : c;
         const int beg = offs[c], N = offs[c + 1] - beg;
@@ -418,19 +438,25 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                         triplet_geometry<P>(wt, wp, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, g);
                         const T fjk = fcut_value(g.rjk, rcjk, R);
                         const T fc3 = s_fc[a] * s_fc[b] * fjk;
-                        T te[2];
+                        // row value = 2^(1-zeta) B^zeta * exp(-eta S) fc3, accumulated in
+                        // FP64 as one fma of the two converted factors
+                        double te[2];
 #pragma unroll
-                        for (int e = 0; e < 2; ++e) te[e] = g.E[e] * fc3;
+                        for (int e = 0; e < 2; ++e) te[e] = (double)(g.E[e] * fc3);
 #pragma unroll
-                        for (int z = 0; z < 4; ++z) {
-                            if (z < wt.nz) {
+                        for (int l = 0; l < 2; ++l) {
+                            if (l < wt.nl) {
+                                T sq[6];
+                                squarings(g.B[l], wp_nsq, sq);
 #pragma unroll
-                                for (int l = 0; l < 2; ++l) {
-                                    if (l < wt.nl) {
-                                        const T pz = (T)wp.coef[z] * power_of<T>(g.B[l], z, wp);
+                                for (int z = 0; z < 4; ++z) {
+                                    if (z < wt.nz) {
+                                        const double pz =
+                                            (double)((T)wp.coef[z] * power_z<T>(g.B[l], sq, z, wp, false));
 #pragma unroll
                                         for (int e = 0; e < 2; ++e)
-                                            if (e < wt.ne) acc3[(z * 2 + l) * 2 + e] += (double)(pz * te[e]);
+                                            if (e < wt.ne)
+                                                acc3[(z * 2 + l) * 2 + e] = fma(pz, te[e], acc3[(z * 2 + l) * 2 + e]);
                                     }
                                 }
                             }
@@ -453,9 +479,9 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ species,
                                                  const double *__restrict__ x,
                                                  const double *__restrict__ dE,
-                                                 double *__restrict__ edge,
-                                                 double *__restrict__ self, int width, int maxn,
-                                                 int tmax)
+                                                 const int *__restrict__ inv_pos,
+                                                 double *__restrict__ edge, int n_edges, int width,
+                                                 int maxn, int tmax)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -474,6 +500,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     }
     __syncthreads();
     const WaveParams &wp = *L.par;
+    const int wp_nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     for (int c = cr.first; c < cr.hi; c += cr.step) {
         const int i = centers ? centers[c] : c;
         const int beg = offs[c], N = offs[c + 1] - beg;
@@ -499,7 +526,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 rcij = L.rc[si * ns + sj];
                 valid = r < rcij;
                 if (!valid) {
-                    double *ej = edge + 3 * (size_t)(beg + jj);
+                    double *ej = edge + 3 * (size_t)inv_pos[beg + jj];
                     ej[0] = 0.0;
                     ej[1] = 0.0;
                     ej[2] = 0.0;
@@ -568,13 +595,15 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         const T fa = s_fc[a], fb = s_fc[b], dfa = s_dfc[a], dfb = s_dfc[b];
                         T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
-                        for (int z = 0; z < 4; ++z) {
-                            if (z < wt.nz) {
+                        for (int l = 0; l < 2; ++l) {
+                            if (l < wt.nl) {
+                                T sq[6];
+                                squarings(q.B[l], wp_nsq, sq);
 #pragma unroll
-                                for (int l = 0; l < 2; ++l) {
-                                    if (l < wt.nl) {
-                                        const T pm1 = power_m1_of<T>(q.B[l], z, wp);
-                                        const T pw = (q.B[l] > (T)0) ? pm1 * q.B[l] : (T)0;
+                                for (int z = 0; z < 4; ++z) {
+                                    if (z < wt.nz) {
+                                        const T pm1 = power_z<T>(q.B[l], sq, z, wp, true);
+                                        const T pw = pm1 * q.B[l];
 #pragma unroll
                                         for (int e = 0; e < 2; ++e) {
                                             U[e] += (T)L.w[(z * 2 + l) * 2 + e] * pw;
@@ -632,7 +661,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
             const int a = a0 + lane;
             if (a < M) {
                 const double f0 = L.fx[a], f1 = L.fy[a], f2 = L.fz[a];
-                double *ej = edge + 3 * (size_t)(beg + L.orig[a]);
+                double *ej = edge + 3 * (size_t)inv_pos[beg + L.orig[a]];
                 ej[0] = f0;
                 ej[1] = f1;
                 ej[2] = f2;
@@ -643,7 +672,8 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         }
         reduce_scatter<4>(sv, lane);
         // lane l holds component (l >> 4) & 3
-        if ((lane & 15) == 0 && (lane >> 4) < 3) self[3 * (size_t)c + (lane >> 4)] = sv[0];
+        if ((lane & 15) == 0 && (lane >> 4) < 3)
+            edge[3 * (size_t)inv_pos[n_edges + c] + (lane >> 4)] = sv[0];
         __syncthreads();
     }
 }
@@ -716,8 +746,10 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
         wp->zeta[z] = zs[z];
         wp->coef[z] = std::pow(2, 1 - zs[z]);
         int iz = -1;
-        if (zs[z] >= 0 && zs[z] <= kMaxIntZeta && zs[z] == std::floor(zs[z])) iz = (int)zs[z];
+        if (zs[z] >= 0 && zs[z] <= 63 && zs[z] == std::floor(zs[z])) iz = (int)zs[z];
         wp->izeta[z] = iz;
+        for (int k = 5; k >= 1; --k)
+            if (iz >= (1 << k)) { wp->nsq = std::max(wp->nsq, k); break; }
     }
     for (int l = 0; l < wt->nl; ++l) {
         wp->lam[l] = ls[l];
@@ -754,8 +786,8 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
 
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
-                           const double *dE, double *edge, double *self, int width, int max_nn,
-                           hipStream_t s)
+                           const double *dE, const int *inv_pos, double *edge, int n_edges,
+                           int width, int max_nn, hipStream_t s)
 {
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
@@ -764,7 +796,7 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     const dim3 grid(grid_for(n_centers)), block(kW);
 #define LDSF_ADJ(P, NSB, NPB)                                                                     \
     hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers,    \
-                       offs, nl, species, x, dE, edge, self, width, maxn, tmax)
+                       offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, tmax)
     if (wt.nsb == 1) {
         if (precision == 0) LDSF_ADJ(0, 1, 1); else LDSF_ADJ(1, 1, 1);
     } else {
