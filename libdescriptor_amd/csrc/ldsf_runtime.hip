@@ -155,6 +155,7 @@ struct ldsf_plan {
     int family = 0;
     bool wave_ok = false;
     ldsf::WaveTab wave{};
+    int wave_group = 0;
     const int *species = nullptr;  // borrowed
     const int *nl = nullptr;       // borrowed
     int *centers = nullptr;        // owned (NULL = identity)
@@ -255,8 +256,10 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     plan->inv_pos = inv_pos.release_ownership();
     // wave kernels: parameter set covered and the largest neighbourhood fits in LDS
     ldsf::WaveParams wpar;
-    if (ldsf::wave_build(sf, &plan->wave, &wpar) && plan->max_nn <= 1023 &&
-        ldsf::wave_lds_bytes(plan->max_nn, true) <= ldsf::kWaveMaxLds) {
+    int group = 0;
+    if (ldsf::wave_build(sf, &plan->wave, &wpar) &&
+        (group = ldsf::wave_group_size(plan->max_nn, plan->wave.npb)) > 0) {
+        plan->wave_group = group;
         HIP_TRY(hipMalloc((void **)&plan->wave_par, sizeof(wpar)), "hipMalloc(wave parameters)");
         HIP_TRY(hipMemcpy(plan->wave_par, &wpar, sizeof(wpar), hipMemcpyHostToDevice),
                 "hipMemcpy(wave parameters)");
@@ -277,7 +280,7 @@ int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, 
     if (plan->family == 1 && plan->wave_ok)
         e = ldsf::launch_fwd_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, desc, tab.width,
-                                  plan->max_nn, s);
+                                  plan->max_nn, plan->wave_group, s);
     else
         e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
                                      plan->nl, plan->species, x, desc, s);
@@ -297,7 +300,8 @@ int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dc
     if (plan->family == 1 && plan->wave_ok)
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
-                                  plan->edge, (int)plan->n_edges, tab.width, plan->max_nn, s);
+                                  plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,
+                                  plan->wave_group, s);
     else
         e = ldsf::launch_adj_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
                                      plan->nl, plan->species, x, dE, plan->inv_pos, plan->edge,

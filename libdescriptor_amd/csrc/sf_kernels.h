@@ -48,19 +48,21 @@ constexpr int kMaxWaveSpecies = 8;
 // host: true (and the table and parameters) if the wave kernels cover this
 // parameter set
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *par);
-// bytes of LDS a launch needs for the largest neighbour count; the wave
-// kernels run only when this fits
-size_t wave_lds_bytes(int max_nn, bool adjoint);
+// Lanes per centre (16, 32 or 64) for the largest neighbour count and number
+// of species-pair blocks; 0 if the wave kernels cannot run (more than 64
+// neighbours, or too much LDS).
+int wave_group_size(int max_nn, int npb);
+size_t wave_lds_bytes(int max_nn, bool adjoint, int npb, int group);
 constexpr size_t kWaveMaxLds = 64 * 1024;
 hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
-                           double *desc, int width, int max_nn, hipStream_t s);
+                           double *desc, int width, int max_nn, int group, hipStream_t s);
 // The adjoint kernels write the vector of edge e (and of centre c, as edge
 // n_edges + c) at record inv_pos[e] of `edge`, the order the gather reads.
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
-                           int width, int max_nn, hipStream_t s);
+                           int width, int max_nn, int group, hipStream_t s);
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,

@@ -1,33 +1,37 @@
-// Wave-per-centre symmetry-function kernels for gfx950.
+// Lane-group symmetry-function kernels for gfx950.
 //
-// One 64-lane wavefront owns one centre atom at a time (persistent grid,
-// XCD-contiguous centre ranges).  Per centre:
-//   1. pair phase: lanes over the centre's neighbour-list entries compute the
-//      FP64 displacement and distance; the entries inside the cutoff are
-//      compacted, in list order, into LDS (ballot + popcount) with their
-//      cutoff value; each lane adds its g2 rows into FP64 accumulators.
-//   2. triplet enumeration: lanes over all pairs (a < b) of compacted
-//      neighbours, walked along diagonals b - a = 1, 2, ... so that the lanes
-//      of one instruction touch distinct neighbours; pairs with r_jk < rc_jk
-//      are compacted into an LDS list (tagged with the species-pair block).
-//   3. triplet phase: lanes over the active list only (full lane occupancy in
-//      the expensive part).  cos(theta), fc_jk, one exp per distinct g4 eta
-//      and one power per distinct (zeta, lambda) are shared by all rows of a
-//      triplet; integer zeta uses exponentiation by squaring.
-//   4. the per-lane FP64 accumulators are summed across the wave with a
-//      fixed-order butterfly (reduce-scatter) and added into the caller's
-//      descriptor row (the reference's += semantics).
-// The adjoint kernel runs the same phases, forms dE/d r_ij, dE/d r_ik and
-// dE/d r_jk per triplet from three row sums, and adds the resulting vectors
-// into per-neighbour FP64 LDS accumulators (ds_add_f64; one wave owns the
-// centre, so the order is fixed); it then writes one vector per neighbour-list
-// edge and the centre's own vector for the gather in sf_plan.hip.
-// Nothing is atomic in global memory, and results do not depend on which
-// other centres share the launch.
+// A wavefront is split into 64/G groups of G lanes (G = 16, 32 or 64); each
+// group owns one centre atom, so one wave works on 64/G centres at a time and
+// the cross-lane steps below (ballot compaction, loop bookkeeping, the
+// reduction of the accumulators) are paid once per wave for all of them.
+// Persistent grid; tiles of 64/G consecutive centres; each XCD takes one
+// contiguous slice of tiles.  Per tile:
+//   1. pair phase: the group's lanes walk its centre's neighbour-list entries,
+//      form the FP64 displacement and distance, and compact the entries inside
+//      the cutoff (list order kept) into the group's LDS arrays; each lane adds
+//      its g2 rows into FP64 accumulators.
+//   2. triplet enumeration: the lanes walk all pairs (a < b) of compacted
+//      neighbours along diagonals b - a = 1, 2, ... (so one instruction touches
+//      distinct neighbours) and compact the pairs with r_jk < rc_jk into an LDS
+//      list tagged with the species-pair block.
+//   3. triplet phase: the lanes walk the active list only.  cos(theta),
+//      fc_jk, one exp per distinct g4 eta and one squaring chain per distinct
+//      lambda serve all rows of a triplet; each row adds one FP64 fma.
+//   4. the accumulators are summed over the group's lanes with a fixed-order
+//      butterfly (reduce-scatter) and added into the caller's descriptor row
+//      (the reference's += semantics).
+// The adjoint kernel forms dE/d r_ij, dE/d r_ik and dE/d r_jk of a triplet
+// from three row sums and adds the resulting vectors into per-neighbour FP64
+// LDS accumulators (ds_add_f64; a group owns its centre, so the order is
+// fixed); it then writes one record per neighbour-list edge and one for the
+// centre, at the positions the atomic-free gather in sf_plan.hip reads.
+// Nothing is atomic in global memory, and a centre's results do not depend
+// on which other centres share the launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -40,27 +44,58 @@ namespace ldsf {
 namespace {
 
 constexpr int kW = 64;
+constexpr int kTab = kMaxWaveSpecies * kMaxWaveSpecies;
 
-__device__ __forceinline__ int prefix_count(unsigned long long m, int lane)
+template <int G>
+struct Lg;
+template <> struct Lg<16> { static constexpr int v = 4; };
+template <> struct Lg<32> { static constexpr int v = 5; };
+template <> struct Lg<64> { static constexpr int v = 6; };
+
+template <int K>
+struct Log2;
+template <> struct Log2<1> { static constexpr int v = 0; };
+template <> struct Log2<2> { static constexpr int v = 1; };
+template <> struct Log2<4> { static constexpr int v = 2; };
+template <> struct Log2<8> { static constexpr int v = 3; };
+template <> struct Log2<16> { static constexpr int v = 4; };
+
+// bits of group g in a wave ballot
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long m, int g)
 {
-    return __popcll(m & ((1ull << lane) - 1ull));
+    if (G == 64) return m;
+    return (m >> (g * G)) & ((1ull << G) - 1ull);
 }
 
-// Sum K per-lane values over the wave.  Afterwards v[0] of lane l holds the
-// total of value (l >> (6 - log2 K)) & (K - 1).  The butterfly order is fixed.
-// Halving step with H live values, partner lane l ^ M (all compile-time).
+__device__ __forceinline__ int below(unsigned long long m, int s)
+{
+    return __popcll(m & ((1ull << s) - 1ull));
+}
+
+// maximum of a per-group value over the groups of the wave (uniform result)
+template <int G>
+__device__ __forceinline__ int tile_max(int v)
+{
+#pragma unroll
+    for (int m = G; m < kW; m <<= 1) v = max(v, __shfl_xor(v, m));
+    return v;
+}
+
+// Recursive-halving reduce-scatter of K values over the G lanes of a group:
+// step with H live values and partner lane ^ M, all compile-time.
 template <int K, int H, int M>
 struct Halve {
-    static __device__ __forceinline__ void run(double (&v)[K], int lane)
+    static __device__ __forceinline__ void run(double (&v)[K], int s)
     {
-        const bool hi = (lane & M) != 0;
+        const bool hi = (s & M) != 0;
 #pragma unroll
         for (int q = 0; q < H; ++q) {
             const double send = hi ? v[q] : v[q + H];
             const double keep = hi ? v[q + H] : v[q];
             v[q] = keep + __shfl_xor(send, M);
         }
-        Halve<K, H / 2, M / 2>::run(v, lane);
+        Halve<K, H / 2, M / 2>::run(v, s);
     }
 };
 
@@ -78,29 +113,22 @@ struct Halve<K, 0, 0> {
     static __device__ __forceinline__ void run(double (&)[K], int) {}
 };
 
-template <int K>
-__device__ __forceinline__ void reduce_scatter(double (&v)[K], int lane)
+// Afterwards v[0] of sub-lane s holds the group total of value
+// (s >> (log2 G - log2 K)) & (K - 1); the butterfly order is fixed.
+template <int K, int G>
+__device__ __forceinline__ void group_reduce(double (&v)[K], int s)
 {
-    Halve<K, K / 2, 32>::run(v, lane);
+    static_assert(K <= G, "more values than lanes");
+    Halve<K, K / 2, G / 2>::run(v, s);
 }
 
-template <int K>
-struct Log2;
-template <> struct Log2<1> { static constexpr int v = 0; };
-template <> struct Log2<2> { static constexpr int v = 1; };
-template <> struct Log2<4> { static constexpr int v = 2; };
-template <> struct Log2<8> { static constexpr int v = 3; };
-template <> struct Log2<16> { static constexpr int v = 4; };
-
-// reduce and add into row[slot(idx)] (slot < 0: nothing)
-template <int K>
-__device__ __forceinline__ void reduce_add(double (&v)[K], int lane, const int *__restrict__ slots,
-                                           double *__restrict__ row)
+template <int K, int G>
+__device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *row)
 {
-    reduce_scatter<K>(v, lane);
-    constexpr int sh = 6 - Log2<K>::v;
-    if ((lane & ((1 << sh) - 1)) == 0) {
-        const int slot = slots[(lane >> sh) & (K - 1)];
+    group_reduce<K, G>(v, s);
+    constexpr int sh = Lg<G>::v - Log2<K>::v;
+    if (row && (s & ((1 << sh) - 1)) == 0) {
+        const int slot = slots[(s >> sh) & (K - 1)];
         if (slot >= 0) row[slot] += v[0];
     }
 }
@@ -113,93 +141,29 @@ __device__ __forceinline__ void diag_norm(int &d, int &a, int M)
     }
 }
 
-// centre range of this block: blocks are dealt to the 8 XCDs round-robin, so
-// block b works on XCD b % 8; each XCD takes one contiguous slice of centres
-struct CentreRange {
-    int lo, hi, step, first;
+// tiles of this block: blocks are dealt to the 8 XCDs round-robin (block b
+// on XCD b % 8); each XCD takes one contiguous slice of tiles
+struct TileRange {
+    int hi, step, first;
 };
 
-__device__ __forceinline__ CentreRange centre_range(int n)
+__device__ __forceinline__ TileRange tile_range(int n)
 {
     const int G = gridDim.x, b = blockIdx.x;
-    CentreRange r;
+    TileRange r;
     if (G >= 8 && (G & 7) == 0) {
         const int xcd = b & 7, local = b >> 3, per = G >> 3;
         const int chunk = (n + 7) >> 3;
-        r.lo = min(n, xcd * chunk);
-        r.hi = min(n, r.lo + chunk);
-        r.first = r.lo + local;
+        const int lo = min(n, xcd * chunk);
+        r.hi = min(n, lo + chunk);
+        r.first = lo + local;
         r.step = per;
     } else {
-        r.lo = 0;
         r.hi = n;
         r.first = b;
         r.step = G;
     }
     return r;
-}
-
-// shared geometry of one triplet
-template <int P>
-struct Trip {
-    using T = typename Real<P>::T;
-    double ra, rb, rjk, rjk2;  // r_ij, r_ik, r_jk, r_jk^2
-    double e0, e1, e2;         // x_k - x_j
-    T fa, fb, fjk;             // cutoff values
-    T B[2];                    // 1 + lambda cos(theta)
-    T E[2];                    // exp(-eta (r_ij^2 + r_ik^2 + r_jk^2)) per distinct eta
-};
-
-template <int P>
-__device__ __forceinline__ void triplet_geometry(const WaveTab &wt, const WaveParams &wp,
-                                                 const double *s_dx,
-                                                 const double *s_dy, const double *s_dz,
-                                                 const double *s_r, const double *s_r2, int a,
-                                                 int b, double rcjk, Trip<P> &t)
-{
-    using T = typename Real<P>::T;
-    const Real<P> R;
-    t.B[0] = t.B[1] = (T)0;
-    t.E[0] = t.E[1] = (T)0;
-    t.ra = s_r[a];
-    t.rb = s_r[b];
-    t.e0 = s_dx[b] - s_dx[a];
-    t.e1 = s_dy[b] - s_dy[a];
-    t.e2 = s_dz[b] - s_dz[a];
-    t.rjk2 = t.e0 * t.e0 + t.e1 * t.e1 + t.e2 * t.e2;
-    t.rjk = sqrt(t.rjk2);
-    if (P == 0) {
-        // sym_g4's own arithmetic (src/SymmetryFunctions.cpp:112-122)
-        const double rijsq = t.ra * t.ra, riksq = t.rb * t.rb, rjksq = t.rjk * t.rjk;
-        const double cosv = (rijsq + riksq - rjksq) / (2 * t.ra * t.rb);
-        const double S = rijsq + riksq + rjksq;
-#pragma unroll
-        for (int l = 0; l < 2; ++l)
-            if (l < wt.nl) t.B[l] = (T)(1 + wp.lam[l] * cosv);
-#pragma unroll
-        for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * S);
-    } else {
-        const double r2a = s_r2[a], r2b = s_r2[b];
-        const float inv = rcp_refined((float)(2 * t.ra * t.rb));
-#pragma unroll
-        for (int l = 0; l < 2; ++l) {
-            if (l < wt.nl) {
-                const int code = __builtin_amdgcn_readfirstlane(wp.lamcode[l]);
-                if (code == 1)
-                    t.B[l] = (float)((t.ra + t.rb) * (t.ra + t.rb) - t.rjk2) * inv;
-                else if (code == -1)
-                    t.B[l] = (float)(t.rjk2 - (t.ra - t.rb) * (t.ra - t.rb)) * inv;
-                else
-                    t.B[l] = 1.0f + (float)wp.lam[l] * ((float)(r2a + r2b - t.rjk2) * inv);
-            }
-        }
-        const double S = r2a + r2b + t.rjk2;
-#pragma unroll
-        for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = dexp(-wp.eta3[e] * S, R);
-    }
-    (void)R;
 }
 
 // B^(2^k), k = 0..nsq (nsq uniform): every integer zeta <= 63 of the grid is a
@@ -212,7 +176,6 @@ __device__ __forceinline__ void squarings(T b, int nsq, T (&sq)[6])
     for (int k = 1; k < 6; ++k) sq[k] = (k <= nsq) ? sq[k - 1] * sq[k - 1] : (T)0;
 }
 
-// product of sq[k] over the set bits of `bits` (uniform)
 template <typename T>
 __device__ __forceinline__ T pow_bits(const T (&sq)[6], int bits)
 {
@@ -223,7 +186,7 @@ __device__ __forceinline__ T pow_bits(const T (&sq)[6], int bits)
     return r;
 }
 
-// b^zeta_z (with zeta_z - 1 when minus_one) for b > 0; 0 for b <= 0 as the
+// b^zeta_z (b^(zeta_z - 1) when minus_one) for b > 0, and 0 for b <= 0 as the
 // reference's guard (src/SymmetryFunctions.cpp:121)
 template <typename T>
 __device__ __forceinline__ T power_z(T b, const T (&sq)[6], int z, const WaveParams &wp,
@@ -241,57 +204,125 @@ __device__ __forceinline__ T power_z(T b, const T (&sq)[6], int z, const WavePar
     return b > (T)0 ? v : (T)0;
 }
 
-// LDS carve-up shared by both kernels (all arrays sized for maxn neighbours)
-struct Lds {
-    WaveParams *par;  // staged row parameters
-    double *w, *wv;   // adjoint row weights of the current pair block (16 each)
-    double *rc;       // cutoff matrix, n_species^2 <= kMaxWaveSpecies^2
-    double *dx, *dy, *dz, *r, *r2, *fc, *dfc, *fx, *fy, *fz;
-    int *sp, *orig, *l0, *l1;
+// ---- LDS ----
+
+// per-wave tables: staged parameters and species-pair cutoff data
+struct Shared {
+    WaveParams *par;
+    double *rc, *rcinv, *rc2, *rcs;  // rc, 1/rc, rc^2, (pi/2)/rc per species pair
 };
 
-__device__ __forceinline__ Lds carve(double *base, int maxn, int tmax, bool adjoint)
+// one group's arrays, all sized for maxn neighbours (lists for tmax pairs)
+struct Grp {
+    double *dx, *dy, *dz, *r2, *r, *rinv, *fc;
+    double *dfc, *fx, *fy, *fz, *w, *wv;  // adjoint only
+    int *sp, *orig;
+    unsigned short *l0, *l1;
+};
+
+constexpr size_t shared_doubles() { return (sizeof(WaveParams) + 7) / 8 + 4 * kTab; }
+
+__host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint, int npb)
 {
-    Lds L;
-    double *p = base;
-    L.par = reinterpret_cast<WaveParams *>(p);
-    p += (sizeof(WaveParams) + 7) / 8;
-    L.rc = p; p += kMaxWaveSpecies * kMaxWaveSpecies;
-    L.w = p; p += 16;
-    L.wv = p; p += 16;
+    const size_t nd = adjoint ? 11 : 7;
+    size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 : 0)) + sizeof(int) * 2 * maxn +
+               sizeof(unsigned short) * tmax * (npb > 1 ? 2 : 1);
+    return (b + 15) & ~size_t(15);
+}
+
+__device__ __forceinline__ Shared carve_shared(double *base)
+{
+    Shared S;
+    S.par = reinterpret_cast<WaveParams *>(base);
+    double *p = base + (sizeof(WaveParams) + 7) / 8;
+    S.rc = p; p += kTab;
+    S.rcinv = p; p += kTab;
+    S.rc2 = p; p += kTab;
+    S.rcs = p;
+    return S;
+}
+
+__device__ __forceinline__ Grp carve_group(double *base, int g, int maxn, int tmax, bool adjoint,
+                                           int npb)
+{
+    char *q = reinterpret_cast<char *>(base + shared_doubles()) +
+              (size_t)g * group_bytes(maxn, tmax, adjoint, npb);
+    double *p = reinterpret_cast<double *>(q);
+    Grp L;
     L.dx = p; p += maxn;
     L.dy = p; p += maxn;
     L.dz = p; p += maxn;
-    L.r = p; p += maxn;
     L.r2 = p; p += maxn;
+    L.r = p; p += maxn;
+    L.rinv = p; p += maxn;
     L.fc = p; p += maxn;
     if (adjoint) {
         L.dfc = p; p += maxn;
         L.fx = p; p += maxn;
         L.fy = p; p += maxn;
         L.fz = p; p += maxn;
+        L.w = p; p += 16;
+        L.wv = p; p += 16;
     } else {
-        L.dfc = L.fx = L.fy = L.fz = nullptr;
+        L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = nullptr;
     }
-    int *q = reinterpret_cast<int *>(p);
-    L.sp = q; q += maxn;
-    L.orig = q; q += maxn;
-    L.l0 = q; q += tmax;
-    L.l1 = q;
+    int *ip = reinterpret_cast<int *>(p);
+    L.sp = ip; ip += maxn;
+    L.orig = ip; ip += maxn;
+    L.l0 = reinterpret_cast<unsigned short *>(ip);
+    L.l1 = L.l0 + tmax;
     return L;
 }
 
-// Phase 2: active pairs of compacted neighbours into L.l0, diagonal order.
-// Returns the count (uniform).
-template <int P, int NPB>
-__device__ __forceinline__ int enumerate_triplets(const WaveTab &wt, const Lds &L, int M, int lane)
+// staging shared by both kernels; ends with a barrier
+__device__ __forceinline__ void stage_shared(const Shared &S, const WaveTab &wt, int lane)
 {
     const int ns = wt.n_species;
+    for (int k = lane; k < ns * ns; k += kW) {
+        const double rc = wt.rc[k];
+        S.rc[k] = rc;
+        S.rcinv[k] = 1.0 / rc;
+        S.rc2[k] = rc * rc;
+        S.rcs[k] = 0.5 * kPi / rc;
+    }
+    const int *src = reinterpret_cast<const int *>(wt.par);
+    int *dst = reinterpret_cast<int *>(S.par);
+    for (int k = lane; k < (int)(sizeof(WaveParams) / 4); k += kW) dst[k] = src[k];
+    __syncthreads();
+}
+
+// cutoff value and derivative of a neighbour at distance r < rc; in the
+// FP32-bulk form, y = (pi/2)(rc - r)/rc is formed from the exact FP64 rc - r
+__device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, double rcs, double &fc,
+                                         double &dfc, Real<0>)
+{
+    double s, c;
+    sincos(kPi * r * rcinv, &s, &c);
+    fc = 0.5 * (c + 1.0);
+    dfc = -0.5 * s * (kPi * rcinv);
+}
+
+__device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, double rcs, float &fc,
+                                         float &dfc, Real<1>)
+{
+    float s, c;
+    sincos_quarter((float)((rc - r) * rcs), s, c);
+    fc = s * s;
+    dfc = -(float)(2.0 * rcs) * s * c;
+}
+
+// Phase 2: compact the active pairs of the group's neighbours into L.l0 in
+// diagonal order; returns the group's count (codes: a | b << 6 | block << 12).
+template <int P, int NPB, int G>
+__device__ __forceinline__ int enumerate_pairs(const Shared &S, const Grp &L, int ns, int M, int g,
+                                               int s)
+{
     const int T = M * (M - 1) / 2;
+    const int Tmax = tile_max<G>(T);
     int cnt = 0;
-    int d = 1, a = lane;
+    int d = 1, a = s;
     diag_norm(d, a, M);
-    for (int t0 = 0; t0 < T; t0 += kW) {
+    for (int t0 = 0; t0 < Tmax; t0 += G) {
         bool act = false;
         int code = 0;
         if (d < M) {
@@ -299,36 +330,37 @@ __device__ __forceinline__ int enumerate_triplets(const WaveTab &wt, const Lds &
             const double e0 = L.dx[b] - L.dx[a], e1 = L.dy[b] - L.dy[a], e2 = L.dz[b] - L.dz[a];
             const double rjk2 = e0 * e0 + e1 * e1 + e2 * e2;
             const int sa = L.sp[a], sb = L.sp[b];
-            const double rcjk = L.rc[sa * ns + sb];
-            // the reference's test is sqrt(r_jk^2) < rc_jk; the squared form differs
+            // the reference tests sqrt(r_jk^2) < rc_jk; the squared test differs
             // only within an ulp of rc_jk, where fc_jk vanishes quadratically
-            act = (P == 0) ? (sqrt(rjk2) < rcjk) : (rjk2 < rcjk * rcjk);
-            code = a | (b << 10) | ((NPB > 1 ? pair_index(ns, sa, sb) : 0) << 20);
+            act = (P == 0) ? (sqrt(rjk2) < S.rc[sa * ns + sb]) : (rjk2 < S.rc2[sa * ns + sb]);
+            code = a | (b << 6) | ((NPB > 1 ? pair_index(ns, sa, sb) : 0) << 12);
         }
-        const unsigned long long m = __ballot(act);
-        if (act) L.l0[cnt + prefix_count(m, lane)] = code;
+        const unsigned long long m = group_bits<G>(__ballot(act), g);
+        if (act) L.l0[cnt + below(m, s)] = (unsigned short)code;
         cnt += __popcll(m);
-        a += kW;
+        a += G;
         diag_norm(d, a, M);
     }
     return cnt;
 }
 
-// the active triplets of pair block p: L.l0 itself, or a compacted copy in L.l1
-template <int NPB>
-__device__ __forceinline__ const int *block_list(const Lds &L, int cnt, int p, int lane, int &nb)
+// the active pairs of block p: L.l0 itself, or a compacted copy in L.l1
+template <int NPB, int G>
+__device__ __forceinline__ const unsigned short *block_list(const Grp &L, int cnt, int p, int g,
+                                                            int s, int &nb)
 {
     if (NPB == 1) {
         nb = cnt;
         return L.l0;
     }
+    const int cmax = tile_max<G>(cnt);
     int k = 0;
-    for (int t0 = 0; t0 < cnt; t0 += kW) {
-        const int t = t0 + lane;
+    for (int t0 = 0; t0 < cmax; t0 += G) {
+        const int t = t0 + s;
         const int code = t < cnt ? L.l0[t] : 0;
-        const bool mine = t < cnt && (code >> 20) == p;
-        const unsigned long long m = __ballot(mine);
-        if (mine) L.l1[k + prefix_count(m, lane)] = code;
+        const bool mine = t < cnt && (code >> 12) == p;
+        const unsigned long long m = group_bits<G>(__ballot(mine), g);
+        if (mine) L.l1[k + below(m, s)] = (unsigned short)code;
         k += __popcll(m);
     }
     __syncthreads();
@@ -336,7 +368,99 @@ __device__ __forceinline__ const int *block_list(const Lds &L, int cnt, int p, i
     return L.l1;
 }
 
-template <int P, int NSB, int NPB>
+// shared quantities of one active triplet (a, b) of a group
+template <int P>
+struct Trip {
+    using T = typename Real<P>::T;
+    double ra, rb, ria, rib, rjk, rjkinv;
+    double e0, e1, e2;  // x_k - x_j
+    T fa, fb, fjk, dfjk;
+    T B[2], E[2];
+    T inv2;           // 1 / (2 r_ij r_ik)
+    double nij, nik;  // r_ij^2 - r_ik^2 + r_jk^2 and r_ik^2 - r_ij^2 + r_jk^2
+};
+
+template <int P, bool ADJ>
+__device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, const WaveTab &wt,
+                                        const Grp &L, int ns, int a, int b, Trip<P> &t)
+{
+    using T = typename Real<P>::T;
+    const Real<P> R;
+    const int k = L.sp[a] * ns + L.sp[b];
+    const double rc = S.rc[k], rcinv = S.rcinv[k];
+    t.ra = L.r[a];
+    t.rb = L.r[b];
+    t.ria = L.rinv[a];
+    t.rib = L.rinv[b];
+    t.fa = (T)L.fc[a];
+    t.fb = (T)L.fc[b];
+    t.e0 = L.dx[b] - L.dx[a];
+    t.e1 = L.dy[b] - L.dy[a];
+    t.e2 = L.dz[b] - L.dz[a];
+    const double rjk2 = t.e0 * t.e0 + t.e1 * t.e1 + t.e2 * t.e2;
+    t.B[0] = t.B[1] = (T)0;
+    t.E[0] = t.E[1] = (T)0;
+    if (P == 0) {
+        // sym_g4's own arithmetic (src/SymmetryFunctions.cpp:112-125)
+        t.rjk = sqrt(rjk2);
+        const double rijsq = t.ra * t.ra, riksq = t.rb * t.rb, rjksq = t.rjk * t.rjk;
+        const double h = 0.5 * t.ria * t.rib;
+        const double cosv = (rijsq + riksq - rjksq) * h;
+        const double S2 = rijsq + riksq + rjksq;
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+            if (l < wt.nl) t.B[l] = (T)(1 + wp.lam[l] * cosv);
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+            if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * S2);
+        double sn, cs;
+        sincos(kPi * t.rjk * rcinv, &sn, &cs);
+        t.fjk = (T)(0.5 * (cs + 1.0));
+        t.dfjk = (T)(-0.5 * sn * (kPi * rcinv));
+        t.inv2 = (T)h;
+        if (ADJ) {
+            t.rjkinv = 1.0 / t.rjk;
+            t.nij = rijsq - riksq + rjksq;
+            t.nik = riksq - rijsq + rjksq;
+        }
+    } else {
+        const double r2a = L.r2[a], r2b = L.r2[b];
+        const float rjkf = __builtin_sqrtf((float)rjk2);
+        t.rjk = rjkf;
+        const double h = 0.5 * t.ria * t.rib;
+        const float inv = (float)h;
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+            if (l < wt.nl) {
+                const int code = __builtin_amdgcn_readfirstlane(wp.lamcode[l]);
+                if (code == 1)  // 1 + cos = ((r_ij + r_ik)^2 - r_jk^2) / (2 r_ij r_ik)
+                    t.B[l] = (float)((t.ra + t.rb) * (t.ra + t.rb) - rjk2) * inv;
+                else if (code == -1)  // 1 - cos = (r_jk^2 - (r_ij - r_ik)^2) / (2 r_ij r_ik)
+                    t.B[l] = (float)(rjk2 - (t.ra - t.rb) * (t.ra - t.rb)) * inv;
+                else
+                    t.B[l] = 1.0f + (float)wp.lam[l] * ((float)(r2a + r2b - rjk2) * inv);
+            }
+        }
+        const double S2 = r2a + r2b + rjk2;
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+            if (e < wt.ne) t.E[e] = dexp(-wp.eta3[e] * S2, R);
+        // rc - r_jk = (rc^2 - r_jk^2) / (rc + r_jk) keeps its relative accuracy near rc
+        const float u = (float)(S.rc2[k] - rjk2) / ((float)rc + rjkf);
+        float sn, cs;
+        sincos_quarter(u * (float)S.rcs[k], sn, cs);
+        t.fjk = sn * sn;
+        t.dfjk = -(float)(2.0 * S.rcs[k]) * sn * cs;
+        t.inv2 = inv;
+        if (ADJ) {
+            t.rjkinv = (double)rcp_refined(rjkf);
+            t.nij = r2a - r2b + rjk2;
+            t.nik = r2b - r2a + rjk2;
+        }
+    }
+}
+
+template <int P, int NSB, int NPB, int G>
 __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
@@ -348,38 +472,44 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
 {
     using T = typename Real<P>::T;
     const Real<P> R;
+    constexpr int C = kW / G;
     extern __shared__ double lds_base[];
-    const Lds L = carve(lds_base, maxn, tmax, false);
-    T *s_fc = reinterpret_cast<T *>(L.fc);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, g = lane / G, s = lane % G;
+    const Shared S = carve_shared(lds_base);
+    const Grp L = carve_group(lds_base, g, maxn, tmax, false, NPB);
+    stage_shared(S, wt, lane);
+    const WaveParams &wp = *S.par;
+    const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     const int ns = wt.n_species;
-    const CentreRange cr = centre_range(n_centers);
-    for (int k = lane; k < ns * ns; k += kW) L.rc[k] = wt.rc[k];
-    {
-        const int *src = reinterpret_cast<const int *>(wt.par);
-        int *dst = reinterpret_cast<int *>(L.par);
-        for (int k = lane; k < (int)(sizeof(WaveParams) / 4); k += kW) dst[k] = src[k];
-    }
-    __syncthreads();
-    const WaveParams &wp = *L.par;
-    const int wp_nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
-    for (int c = cr.first; c < cr.hi; c += cr.step) {
-        const int i = centers ? centers[c] : c;
-        const int beg = offs[c], N = offs[c + 1] - beg;
-        const int si = species[i];
-        const double xi0 = x[3 * (size_t)i], xi1 = x[3 * (size_t)i + 1], xi2 = x[3 * (size_t)i + 2];
-        double *row = desc + (size_t)c * width;
+    const int n_tiles = (n_centers + C - 1) / C;
+    const TileRange tr = tile_range(n_tiles);
+    for (int tile = tr.first; tile < tr.hi; tile += tr.step) {
+        const int c = tile * C + g;
+        int N = 0, beg = 0, si = 0;
+        double xi0 = 0, xi1 = 0, xi2 = 0;
+        double *row = nullptr;
+        if (c < n_centers) {
+            const int i = centers ? centers[c] : c;
+            beg = offs[c];
+            N = offs[c + 1] - beg;
+            si = species[i];
+            xi0 = x[3 * (size_t)i];
+            xi1 = x[3 * (size_t)i + 1];
+            xi2 = x[3 * (size_t)i + 2];
+            row = desc + (size_t)c * width;
+        }
+        const int Nmax = tile_max<G>(N);
 
         // ---- pairs ----
         double acc2[NSB * 8];
 #pragma unroll
         for (int k = 0; k < NSB * 8; ++k) acc2[k] = 0.0;
         int M = 0;
-        for (int j0 = 0; j0 < N; j0 += kW) {
-            const int jj = j0 + lane;
+        for (int j0 = 0; j0 < Nmax; j0 += G) {
+            const int jj = j0 + s;
             bool valid = false;
             int sj = 0;
-            double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0, rcij = 0;
+            double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
             if (jj < N) {
                 const int j = nl[beg + jj];
                 sj = species[j];
@@ -388,82 +518,81 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                 d2 = x[3 * (size_t)j + 2] - xi2;
                 r2 = d0 * d0 + d1 * d1 + d2 * d2;
                 r = sqrt(r2);
-                rcij = L.rc[si * ns + sj];
-                valid = r < rcij;  // r > rc is skipped; r == rc adds exactly 0
+                valid = r < S.rc[si * ns + sj];  // r > rc is skipped; r == rc adds exactly 0
             }
-            const unsigned long long m = __ballot(valid);
+            const unsigned long long m = group_bits<G>(__ballot(valid), g);
             if (valid) {
-                const int pos = M + prefix_count(m, lane);
+                const int pos = M + below(m, s);
+                const int k = si * ns + sj;
+                T fc, dfc;
+                cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
                 L.dx[pos] = d0;
                 L.dy[pos] = d1;
                 L.dz[pos] = d2;
-                L.r[pos] = r;
                 L.r2[pos] = r2;
+                L.r[pos] = r;
+                L.rinv[pos] = 1.0 / r;
+                L.fc[pos] = (double)fc;
                 L.sp[pos] = sj;
-                const T fc = fcut_value(r, rcij, R);
-                s_fc[pos] = fc;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     if (q < wt.nr2) {
                         const double dr = r - wp.rs2[q];
                         const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
 #pragma unroll
-                        for (int s = 0; s < NSB; ++s)
-                            if (NSB == 1 || s == sj) acc2[s * 8 + q] += v;
+                        for (int sb = 0; sb < NSB; ++sb)
+                            if (NSB == 1 || sb == sj) acc2[sb * 8 + q] += v;
                     }
                 }
             }
             M += __popcll(m);
         }
-        reduce_add<NSB * 8>(acc2, lane, wp.slots, row);
+        reduce_add<NSB * 8, G>(acc2, s, wp.slots, row);
         __syncthreads();
 
         // ---- triplets ----
-        if (wt.has3 && M > 1) {
-            const int cnt = enumerate_triplets<P, NPB>(wt, L, M, lane);
+        if (wt.has3) {
+            const int cnt = enumerate_pairs<P, NPB, G>(S, L, ns, M, g, s);
             __syncthreads();
             for (int p = 0; p < NPB; ++p) {
                 int nb = 0;
-                const int *lst = block_list<NPB>(L, cnt, p, lane, nb);
+                const unsigned short *lst = block_list<NPB, G>(L, cnt, p, g, s, nb);
+                const int nbmax = tile_max<G>(nb);
                 double acc3[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
-                for (int t0 = 0; t0 < nb; t0 += kW) {
-                    const int t = t0 + lane;
+                for (int t0 = 0; t0 < nbmax; t0 += G) {
+                    const int t = t0 + s;
                     if (t < nb) {
                         const int code = lst[t];
-                        const int a = code & 1023, b = (code >> 10) & 1023;
-                        const double rcjk = L.rc[L.sp[a] * ns + L.sp[b]];
-                        Trip<P> g;
-                        triplet_geometry<P>(wt, wp, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, g);
-                        const T fjk = fcut_value(g.rjk, rcjk, R);
-                        const T fc3 = s_fc[a] * s_fc[b] * fjk;
-                        // row value = 2^(1-zeta) B^zeta * exp(-eta S) fc3, accumulated in
-                        // FP64 as one fma of the two converted factors
+                        Trip<P> q;
+                        triplet<P, false>(S, wp, wt, L, ns, code & 63, (code >> 6) & 63, q);
+                        const T fc3 = q.fa * q.fb * q.fjk;
                         double te[2];
 #pragma unroll
-                        for (int e = 0; e < 2; ++e) te[e] = (double)(g.E[e] * fc3);
+                        for (int e = 0; e < 2; ++e) te[e] = (double)(q.E[e] * fc3);
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
                             if (l < wt.nl) {
                                 T sq[6];
-                                squarings(g.B[l], wp_nsq, sq);
+                                squarings(q.B[l], nsq, sq);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
                                     if (z < wt.nz) {
                                         const double pz =
-                                            (double)((T)wp.coef[z] * power_z<T>(g.B[l], sq, z, wp, false));
+                                            (double)((T)wp.coef[z] * power_z<T>(q.B[l], sq, z, wp, false));
 #pragma unroll
                                         for (int e = 0; e < 2; ++e)
                                             if (e < wt.ne)
-                                                acc3[(z * 2 + l) * 2 + e] = fma(pz, te[e], acc3[(z * 2 + l) * 2 + e]);
+                                                acc3[(z * 2 + l) * 2 + e] =
+                                                    fma(pz, te[e], acc3[(z * 2 + l) * 2 + e]);
                                     }
                                 }
                             }
                         }
                     }
                 }
-                reduce_add<16>(acc3, lane, wp.slots + 16 + p * 16, row);
+                reduce_add<16, G>(acc3, s, wp.slots + 16 + p * 16, row);
                 if (NPB > 1) __syncthreads();
             }
         }
@@ -471,7 +600,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
     }
 }
 
-template <int P, int NSB, int NPB>
+template <int P, int NSB, int NPB, int G>
 __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
@@ -485,36 +614,42 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
 {
     using T = typename Real<P>::T;
     const Real<P> R;
+    constexpr int C = kW / G;
     extern __shared__ double lds_base[];
-    const Lds L = carve(lds_base, maxn, tmax, true);
-    T *s_fc = reinterpret_cast<T *>(L.fc);
-    T *s_dfc = reinterpret_cast<T *>(L.dfc);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, g = lane / G, s = lane % G;
+    const Shared S = carve_shared(lds_base);
+    const Grp L = carve_group(lds_base, g, maxn, tmax, true, NPB);
+    stage_shared(S, wt, lane);
+    const WaveParams &wp = *S.par;
+    const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     const int ns = wt.n_species;
-    const CentreRange cr = centre_range(n_centers);
-    for (int k = lane; k < ns * ns; k += kW) L.rc[k] = wt.rc[k];
-    {
-        const int *src = reinterpret_cast<const int *>(wt.par);
-        int *dst = reinterpret_cast<int *>(L.par);
-        for (int k = lane; k < (int)(sizeof(WaveParams) / 4); k += kW) dst[k] = src[k];
-    }
-    __syncthreads();
-    const WaveParams &wp = *L.par;
-    const int wp_nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
-    for (int c = cr.first; c < cr.hi; c += cr.step) {
-        const int i = centers ? centers[c] : c;
-        const int beg = offs[c], N = offs[c + 1] - beg;
-        const int si = species[i];
-        const double xi0 = x[3 * (size_t)i], xi1 = x[3 * (size_t)i + 1], xi2 = x[3 * (size_t)i + 2];
-        const double *g = dE + (size_t)c * width;
+    const int n_tiles = (n_centers + C - 1) / C;
+    const TileRange tr = tile_range(n_tiles);
+    for (int tile = tr.first; tile < tr.hi; tile += tr.step) {
+        const int c = tile * C + g;
+        const bool live = c < n_centers;
+        int N = 0, beg = 0, si = 0;
+        double xi0 = 0, xi1 = 0, xi2 = 0;
+        const double *gv = dE;
+        if (live) {
+            const int i = centers ? centers[c] : c;
+            beg = offs[c];
+            N = offs[c + 1] - beg;
+            si = species[i];
+            xi0 = x[3 * (size_t)i];
+            xi1 = x[3 * (size_t)i + 1];
+            xi2 = x[3 * (size_t)i + 2];
+            gv = dE + (size_t)c * width;
+        }
+        const int Nmax = tile_max<G>(N);
 
-        // ---- pairs: radial part of dE/d r_j, written as the accumulator's start ----
+        // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
         int M = 0;
-        for (int j0 = 0; j0 < N; j0 += kW) {
-            const int jj = j0 + lane;
+        for (int j0 = 0; j0 < Nmax; j0 += G) {
+            const int jj = j0 + s;
             bool valid = false;
             int sj = 0;
-            double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0, rcij = 0;
+            double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
             if (jj < N) {
                 const int j = nl[beg + jj];
                 sj = species[j];
@@ -523,8 +658,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 d2 = x[3 * (size_t)j + 2] - xi2;
                 r2 = d0 * d0 + d1 * d1 + d2 * d2;
                 r = sqrt(r2);
-                rcij = L.rc[si * ns + sj];
-                valid = r < rcij;
+                valid = r < S.rc[si * ns + sj];
                 if (!valid) {
                     double *ej = edge + 3 * (size_t)inv_pos[beg + jj];
                     ej[0] = 0.0;
@@ -532,12 +666,13 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                     ej[2] = 0.0;
                 }
             }
-            const unsigned long long m = __ballot(valid);
+            const unsigned long long m = group_bits<G>(__ballot(valid), g);
             if (valid) {
-                const int pos = M + prefix_count(m, lane);
+                const int pos = M + below(m, s);
+                const int k = si * ns + sj;
                 T fc, dfc;
-                fcut(r, rcij, fc, dfc, R);
-                double s = 0.0;
+                cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
+                double sum = 0.0;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     if (q < wt.nr2) {
@@ -545,60 +680,59 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         const double dr = r - wp.rs2[q];
                         const T ex = dexp(-wp.eta2[q] * dr * dr, R);
                         const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
-                        s += g[slot] * (double)dphi;
+                        sum += gv[slot] * (double)dphi;
                     }
                 }
                 const double ir = 1.0 / r;
                 L.dx[pos] = d0;
                 L.dy[pos] = d1;
                 L.dz[pos] = d2;
-                L.r[pos] = r;
                 L.r2[pos] = r2;
+                L.r[pos] = r;
+                L.rinv[pos] = ir;
+                L.fc[pos] = (double)fc;
+                L.dfc[pos] = (double)dfc;
                 L.sp[pos] = sj;
                 L.orig[pos] = jj;
-                s_fc[pos] = fc;
-                s_dfc[pos] = dfc;
-                L.fx[pos] = s * d0 * ir;
-                L.fy[pos] = s * d1 * ir;
-                L.fz[pos] = s * d2 * ir;
+                L.fx[pos] = sum * d0 * ir;
+                L.fy[pos] = sum * d1 * ir;
+                L.fz[pos] = sum * d2 * ir;
             }
             M += __popcll(m);
         }
         __syncthreads();
 
         // ---- triplets ----
-        if (wt.has3 && M > 1) {
-            const int cnt = enumerate_triplets<P, NPB>(wt, L, M, lane);
+        if (wt.has3) {
+            const int cnt = enumerate_pairs<P, NPB, G>(S, L, ns, M, g, s);
             __syncthreads();
             for (int p = 0; p < NPB; ++p) {
                 int nb = 0;
-                const int *lst = block_list<NPB>(L, cnt, p, lane, nb);
-                // row weights of this block: dE/dzeta * 2^(1-zeta), and times zeta lambda
-                if (lane < 16) {
-                    const int z = lane >> 2, l = (lane >> 1) & 1;
-                    const int slot = wp.slots[16 + p * 16 + lane];
-                    const double ww = slot >= 0 ? g[slot] * wp.coef[z] : 0.0;
-                    L.w[lane] = ww;
-                    L.wv[lane] = ww * wp.zeta[z] * wp.lam[l];
+                const unsigned short *lst = block_list<NPB, G>(L, cnt, p, g, s, nb);
+                // row weights of this block: dE/dzeta 2^(1-zeta), and times zeta lambda
+                if (s < 16) {
+                    const int z = s >> 2, l = (s >> 1) & 1;
+                    const int slot = wp.slots[16 + p * 16 + s];
+                    const double ww = (live && slot >= 0) ? gv[slot] * wp.coef[z] : 0.0;
+                    L.w[s] = ww;
+                    L.wv[s] = ww * wp.zeta[z] * wp.lam[l];
                 }
                 __syncthreads();
-                for (int t0 = 0; t0 < nb; t0 += kW) {
-                    const int t = t0 + lane;
+                const int nbmax = tile_max<G>(nb);
+                for (int t0 = 0; t0 < nbmax; t0 += G) {
+                    const int t = t0 + s;
                     if (t < nb) {
                         const int code = lst[t];
-                        const int a = code & 1023, b = (code >> 10) & 1023;
-                        const double rcjk = L.rc[L.sp[a] * ns + L.sp[b]];
+                        const int a = code & 63, b = (code >> 6) & 63;
                         Trip<P> q;
-                        triplet_geometry<P>(wt, wp, L.dx, L.dy, L.dz, L.r, L.r2, a, b, rcjk, q);
-                        T fjk, dfjk;
-                        fcut(q.rjk, rcjk, fjk, dfjk, R);
-                        const T fa = s_fc[a], fb = s_fc[b], dfa = s_dfc[a], dfb = s_dfc[b];
+                        triplet<P, true>(S, wp, wt, L, ns, a, b, q);
+                        const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
                         T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
                             if (l < wt.nl) {
                                 T sq[6];
-                                squarings(q.B[l], wp_nsq, sq);
+                                squarings(q.B[l], nsq, sq);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
                                     if (z < wt.nz) {
@@ -623,25 +757,16 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                 SC += (T)wp.eta3[e] * ue;
                             }
                         }
-                        // partial derivatives of cos(theta) w.r.t. r_ij, r_ik, r_jk
-                        const double rijsq = q.ra * q.ra, riksq = q.rb * q.rb, rjksq = q.rjk * q.rjk;
-                        T dcij, dcik, dcjk;
-                        if (P == 0) {
-                            dcij = (T)((rijsq - riksq + rjksq) / (2 * rijsq * q.rb));
-                            dcik = (T)((riksq - rijsq + rjksq) / (2 * q.ra * riksq));
-                            dcjk = (T)(-q.rjk / (q.ra * q.rb));
-                        } else {
-                            const float inv = rcp_refined((float)(2 * q.ra * q.rb));
-                            dcij = (float)(rijsq - riksq + rjksq) * inv * (float)(1.0 / q.ra);
-                            dcik = (float)(riksq - rijsq + rjksq) * inv * (float)(1.0 / q.rb);
-                            dcjk = -(float)q.rjk * inv * 2.0f;
-                        }
-                        const T fc3 = fa * fb * fjk;
-                        const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + fb * fjk * dfa * SB;
-                        const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + fa * fjk * dfb * SB;
-                        const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + fa * fb * dfjk * SB;
-                        const double a1 = (double)Aij / q.ra, a2 = (double)Aik / q.rb,
-                                     a3 = (double)Ajk / q.rjk;
+                        // d cos(theta) / d r_ij, d r_ik, d r_jk
+                        const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
+                        const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
+                        const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
+                        const T fc3 = q.fa * q.fb * q.fjk;
+                        const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
+                        const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
+                        const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
+                        const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
+                                     a3 = (double)Ajk * q.rjkinv;
                         atomicAdd(&L.fx[a], a1 * L.dx[a] - a3 * q.e0);
                         atomicAdd(&L.fy[a], a1 * L.dy[a] - a3 * q.e1);
                         atomicAdd(&L.fz[a], a1 * L.dz[a] - a3 * q.e2);
@@ -650,15 +775,15 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         atomicAdd(&L.fz[b], a2 * L.dz[b] + a3 * q.e2);
                     }
                 }
-                if (NPB > 1) __syncthreads();
+                __syncthreads();
             }
-            __syncthreads();
         }
 
-        // ---- write edge vectors and the centre's own vector ----
+        // ---- edge records and the centre's own record ----
+        const int Mmax = tile_max<G>(M);
         double sv[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int a0 = 0; a0 < M; a0 += kW) {
-            const int a = a0 + lane;
+        for (int a0 = 0; a0 < Mmax; a0 += G) {
+            const int a = a0 + s;
             if (a < M) {
                 const double f0 = L.fx[a], f1 = L.fy[a], f2 = L.fz[a];
                 double *ej = edge + 3 * (size_t)inv_pos[beg + L.orig[a]];
@@ -670,30 +795,49 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 sv[2] -= f2;
             }
         }
-        reduce_scatter<4>(sv, lane);
-        // lane l holds component (l >> 4) & 3
-        if ((lane & 15) == 0 && (lane >> 4) < 3)
-            edge[3 * (size_t)inv_pos[n_edges + c] + (lane >> 4)] = sv[0];
+        group_reduce<4, G>(sv, s);
+        constexpr int sh = Lg<G>::v - 2;
+        if (live && (s & ((1 << sh) - 1)) == 0 && (s >> sh) < 3)
+            edge[3 * (size_t)inv_pos[n_edges + c] + (s >> sh)] = sv[0];
         __syncthreads();
     }
 }
 
-int grid_for(int n_centers)
+int grid_for(int n_tiles)
 {
-    int g = std::min(n_centers, 16384);
+    int g = std::min(n_tiles, 16384);
     if (g >= 8) g = (g + 7) & ~7;
     return std::max(g, 1);
 }
 
+size_t lds_for(int max_nn, bool adjoint, int npb, int G)
+{
+    const int maxn = std::max(max_nn, 1);
+    const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
+    return sizeof(double) * shared_doubles() + (size_t)(kW / G) * group_bytes(maxn, tmax, adjoint, npb);
+}
+
 }  // namespace
 
-size_t wave_lds_bytes(int max_nn, bool adjoint)
+int wave_group_size(int max_nn, int npb)
 {
-    const size_t maxn = std::max(max_nn, 1);
-    const size_t tmax = std::max<size_t>(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * ((sizeof(WaveParams) + 7) / 8 + kMaxWaveSpecies * kMaxWaveSpecies + 32 +
-                             maxn * (adjoint ? 10 : 6)) +
-           sizeof(int) * (2 * maxn + 2 * tmax);
+    if (max_nn > 64) return 0;
+    // LDSF_WAVE_GROUP=16|32|64 fixes the group size (tuning), when its LDS fits
+    if (const char *env = std::getenv("LDSF_WAVE_GROUP")) {
+        const int G = std::atoi(env);
+        if ((G == 16 || G == 32 || G == 64) && lds_for(max_nn, true, npb, G) <= kWaveMaxLds)
+            return G;
+    }
+    // A whole wave per centre: the LDS per centre is the same for every group
+    // size, so smaller groups only mean fewer resident waves (measured on the
+    // 262,144-atom two-species system: 3.14 / 3.61 / 5.27 ms per fwd+adjoint
+    // step for G = 64 / 32 / 16).
+    return lds_for(max_nn, true, npb, 64) <= kWaveMaxLds ? 64 : 0;
+}
+
+size_t wave_lds_bytes(int max_nn, bool adjoint, int npb, int group)
+{
+    return lds_for(max_nn, adjoint, npb, group);
 }
 
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
@@ -765,21 +909,28 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
 
 hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
-                           double *desc, int width, int max_nn, hipStream_t s)
+                           double *desc, int width, int max_nn, int group, hipStream_t s)
 {
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    const size_t lds = wave_lds_bytes(max_nn, false);
-    const dim3 grid(grid_for(n_centers)), block(kW);
-#define LDSF_FWD(P, NSB, NPB)                                                                     \
-    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers,    \
+    const size_t lds = lds_for(max_nn, false, wt.npb, group);
+    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group))), block(kW);
+#define LDSF_FWD(P, NSB, NPB, G)                                                                  \
+    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, G>), grid, block, lds, s, wt, n_centers, centers, \
                        offs, nl, species, x, desc, width, maxn, tmax)
+#define LDSF_FWD_G(P, NSB, NPB)                                                                   \
+    do {                                                                                          \
+        if (group == 16) LDSF_FWD(P, NSB, NPB, 16);                                               \
+        else if (group == 32) LDSF_FWD(P, NSB, NPB, 32);                                          \
+        else LDSF_FWD(P, NSB, NPB, 64);                                                           \
+    } while (0)
     if (wt.nsb == 1) {
-        if (precision == 0) LDSF_FWD(0, 1, 1); else LDSF_FWD(1, 1, 1);
+        if (precision == 0) LDSF_FWD_G(0, 1, 1); else LDSF_FWD_G(1, 1, 1);
     } else {
-        if (precision == 0) LDSF_FWD(0, 2, 3); else LDSF_FWD(1, 2, 3);
+        if (precision == 0) LDSF_FWD_G(0, 2, 3); else LDSF_FWD_G(1, 2, 3);
     }
+#undef LDSF_FWD_G
 #undef LDSF_FWD
     return hipGetLastError();
 }
@@ -787,21 +938,28 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
-                           int width, int max_nn, hipStream_t s)
+                           int width, int max_nn, int group, hipStream_t s)
 {
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    const size_t lds = wave_lds_bytes(max_nn, true);
-    const dim3 grid(grid_for(n_centers)), block(kW);
-#define LDSF_ADJ(P, NSB, NPB)                                                                     \
-    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers,    \
+    const size_t lds = lds_for(max_nn, true, wt.npb, group);
+    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group))), block(kW);
+#define LDSF_ADJ(P, NSB, NPB, G)                                                                  \
+    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, G>), grid, block, lds, s, wt, n_centers, centers, \
                        offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, tmax)
+#define LDSF_ADJ_G(P, NSB, NPB)                                                                   \
+    do {                                                                                          \
+        if (group == 16) LDSF_ADJ(P, NSB, NPB, 16);                                               \
+        else if (group == 32) LDSF_ADJ(P, NSB, NPB, 32);                                          \
+        else LDSF_ADJ(P, NSB, NPB, 64);                                                           \
+    } while (0)
     if (wt.nsb == 1) {
-        if (precision == 0) LDSF_ADJ(0, 1, 1); else LDSF_ADJ(1, 1, 1);
+        if (precision == 0) LDSF_ADJ_G(0, 1, 1); else LDSF_ADJ_G(1, 1, 1);
     } else {
-        if (precision == 0) LDSF_ADJ(0, 2, 3); else LDSF_ADJ(1, 2, 3);
+        if (precision == 0) LDSF_ADJ_G(0, 2, 3); else LDSF_ADJ_G(1, 2, 3);
     }
+#undef LDSF_ADJ_G
 #undef LDSF_ADJ
     return hipGetLastError();
 }

@@ -140,9 +140,21 @@ def config_c2(seed=0):
     return layout(pos, cell, pbc, np.zeros(len(pos), np.int32), 4.5, symbols=["Si"])
 
 
-def config_c3(n=262144, seed=0):
+def config_c3(n=262144, seed=0, cache_dir=None):
+    """262,144-atom two-species system; with ``cache_dir`` (or $LDSF_WORKLOAD_CACHE)
+    the layout is stored there as .npz and reused."""
+    import os
+    cache_dir = cache_dir or os.environ.get("LDSF_WORKLOAD_CACHE")
+    path = os.path.join(cache_dir, f"c3_{n}_{seed}.npz") if cache_dir else None
+    if path and os.path.exists(path):
+        z = np.load(path, allow_pickle=False)
+        return {k: (int(z[k]) if k == "n_contrib" else z[k]) for k in z.files}
     pos, cell, pbc, sp = random_system(n, 0.05, 1.8, seed=seed)
-    return layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+    L = layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+    if path:
+        os.makedirs(cache_dir, exist_ok=True)
+        np.savez(path, **L)
+    return L
 
 
 def config_c4(n_configs=512, seed=0, nmin=8, nmax=4096):
