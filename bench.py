@@ -71,12 +71,18 @@ def main():
     ap.add_argument("--family", type=int, default=-1, help="force kernel family (0 general, 1 wave)")
     ap.add_argument("--only", choices=["fp64", "fp32_bulk"], default=None,
                     help="time one form only and print its timings (for profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path "
               "shards by configuration or by spatial slab, which this build does not run",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import torch
     import libdescriptor_amd as ld
@@ -126,10 +132,11 @@ def main():
             step(prec)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) * 1e3 / args.steps
-        # per-kernel split with events on the stream the kernels run on
+        # per-kernel split with HIP events on the stream the kernels run on:
+        # forward kernel | adjoint kernel | gather kernel
         s = torch.cuda.current_stream()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-        fwd_ms = adj_ms = 0.0
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        fwd_ms = adj_ms = gat_ms = 0.0
         reps = max(3, min(args.steps, 10))
         for _ in range(reps):
             desc.zero_()
@@ -137,14 +144,18 @@ def main():
             ev[0].record(s)
             plan.compute(x, desc, prec)
             ev[1].record(s)
-            plan.gradient(x, g, dx, prec)
+            plan.adjoint_edges(x, g, prec)
             ev[2].record(s)
+            plan.gather(dx)
+            ev[3].record(s)
             torch.cuda.synchronize()
             fwd_ms += ev[0].elapsed_time(ev[1]) / reps
             adj_ms += ev[1].elapsed_time(ev[2]) / reps
+            gat_ms += ev[2].elapsed_time(ev[3]) / reps
         outs[name] = (desc.cpu().numpy(), dx.cpu().numpy())
-        res[name] = dict(ms_per_step=ms, fwd_ms=fwd_ms, adj_ms=adj_ms)
-        log(f"{name}: {ms:.3f} ms/step (forward {fwd_ms:.3f} ms, adjoint+gather {adj_ms:.3f} ms)")
+        res[name] = dict(ms_per_step=ms, fwd_ms=fwd_ms, adj_ms=adj_ms, gather_ms=gat_ms)
+        log(f"{name}: {ms:.3f} ms/step (forward {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms, "
+            f"gather {gat_ms:.3f} ms)")
 
     if args.only:
         print(json.dumps({"only": args.only, **res[args.only]}), flush=True)
@@ -182,6 +193,9 @@ def main():
     dom_flop = adj_flop if dom == "adjoint" else fwd_flop
     dom_ms = max(r32["adj_ms"], r32["fwd_ms"])
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
+    dom_kernel = ("k_adj_wave" if dom == "adjoint" else "k_fwd_wave") \
+        if plan.kernel_family == 1 else ("k_adj_general" if dom == "adjoint" else "k_fwd_general")
+    traffic = measured_traffic(dom_kernel, n) if args.n == 262144 else None
 
     cpu = None
     if not args.no_cpu:
@@ -198,25 +212,56 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-bulk (f64 geometry + accumulation)",
+        "dtype": "f32 (f64 geometry and accumulation)",
         "data": "synthetic",
         "config": {"workload": "two-species random periodic system, species-resolved SF "
                                "(2x8 G2 + 3x16 G4 = 64), forward + adjoint",
                    "n_atoms": n, "n_rows_with_padding": n_total, "cutoff_A": 5.0,
                    "density_per_A3": 0.05, "width": W, "seed": 0},
         "fp64": {"value": value64, "ms_per_step": res["fp64"]["ms_per_step"],
-                 "forward_ms": res["fp64"]["fwd_ms"], "adjoint_gather_ms": res["fp64"]["adj_ms"]},
-        "fp32_bulk": {"forward_ms": r32["fwd_ms"], "adjoint_gather_ms": r32["adj_ms"]},
+                 "forward_ms": res["fp64"]["fwd_ms"], "adjoint_ms": res["fp64"]["adj_ms"],
+                 "gather_ms": res["fp64"]["gather_ms"]},
+        "fp32_bulk": {"forward_ms": r32["fwd_ms"], "adjoint_ms": r32["adj_ms"],
+                      "gather_ms": r32["gather_ms"]},
         "speedup_fp32_bulk_vs_fp64": value / value64,
         "error_fp32_bulk_vs_fp64": err,
         "counts": {"valid_neighbours": Nv, "triplet_tests": tests, "active_triplets": active,
                    "fwd_flop_per_step": fwd_flop, "adj_flop_per_step": adj_flop},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK["fp32"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None},
+        "roofline": {"bound": "valu", "kernel": dom_kernel, "achieved": achieved,
+                     "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
+                     "traffic": traffic,
+                     "hbm_GBps": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None},
         "cpu_baseline": cpu,
         "kernel_family": plan.kernel_family,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` on this workload, from the PMC summary
+    kept beside the kernel traces (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration recorded in the same file); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_c3_{n}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("hbm_bytes_per_launch"):
+                return float(ln.split()[1])
+    return None
+
+
+def copy_calibration(mib):
+    """A device-to-device copy of `mib` MiB, 5 times, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 2 * n * 8, "launches": 5}), flush=True)
 
 
 def cpu_baseline(L, types, sizes, params, g):
@@ -231,10 +276,17 @@ def cpu_baseline(L, types, sizes, params, g):
     threads = max(1, min(cores, 16))
     off = np.r_[0, np.cumsum(L["numneigh"])].astype(np.int32)
     n = L["n_contrib"]
-    # calibrate on 512 centres, then size the sample for ~10 s
-    t = time.perf_counter()
-    O.fwd_adj_omp(512, L["species"], L["neighlist"], off, L["coords"], g, threads)
-    per = (time.perf_counter() - t) / 512
+    # calibrate on growing prefixes until one takes >= 1 s, then size the
+    # sample for ~10 s
+    k = 512
+    while True:
+        t = time.perf_counter()
+        O.fwd_adj_omp(k, L["species"], L["neighlist"], off, L["coords"], g, threads)
+        dt = time.perf_counter() - t
+        if dt >= 1.0 or k >= n:
+            break
+        k = min(n, k * 4)
+    per = dt / k
     k = int(min(n, max(512, 10.0 / max(per, 1e-9))))
     t = time.perf_counter()
     O.fwd_adj_omp(k, L["species"], L["neighlist"], off, L["coords"], g, threads)

@@ -171,6 +171,16 @@ LDSF_API int ldsf_plan_gradient(ldsf_plan *plan, const double *d_coordinates,
                                 const double *d_dE_dzeta, double *d_dcoords, int precision,
                                 void *stream);
 
+/* The two halves of ldsf_plan_gradient.  ldsf_plan_adjoint_edges runs the
+ * adjoint kernel and keeps one dE/dr vector per neighbour-list edge and per
+ * centre in the plan's workspace; ldsf_plan_gather then adds, for every
+ * coordinate row, its vectors (in ascending edge order, so the sum is the same
+ * on every run) into d_dcoords[n_total][3].  Calls on one plan must not
+ * overlap. */
+LDSF_API int ldsf_plan_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
+                                     const double *d_dE_dzeta, int precision, void *stream);
+LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream);
+
 /* Which kernel family a plan uses: 0 = general (any parameter set), 1 = wave-
  * per-centre LDS kernels.  Setting 0 forces the general kernels. */
 LDSF_API int ldsf_plan_kernel_family(const ldsf_plan *plan);

@@ -42,6 +42,8 @@ _PROTOS = {
     "ldsf_plan_max_neighbors": (_i, [_vp]),
     "ldsf_plan_compute": (_i, [_vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_gradient": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_adjoint_edges": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_gather": (_i, [_vp, _vp, _vp]),
     "ldsf_plan_kernel_family": (_i, [_vp]),
     "ldsf_plan_set_kernel_family": (_i, [_vp, _i]),
     "ldsf_nbl_create_paddings": (_i, [_i, _d, _vp, _vp, _vp, _vp, _vp]),
@@ -155,6 +157,15 @@ class Plan:
     def gradient(self, coords, dE_dzeta, dcoords, precision=FP64, stream=None):
         check(lib().ldsf_plan_gradient(self.handle, _ptr(coords), _ptr(dE_dzeta), _ptr(dcoords),
                                        int(precision), _stream_ptr(stream)))
+
+    def adjoint_edges(self, coords, dE_dzeta, precision=FP64, stream=None):
+        """First half of gradient(): the adjoint kernel alone."""
+        check(lib().ldsf_plan_adjoint_edges(self.handle, _ptr(coords), _ptr(dE_dzeta),
+                                            int(precision), _stream_ptr(stream)))
+
+    def gather(self, dcoords, stream=None):
+        """Second half of gradient(): dcoords += the per-row sums of the records."""
+        check(lib().ldsf_plan_gather(self.handle, _ptr(dcoords), _stream_ptr(stream)))
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -156,6 +156,7 @@ struct ldsf_plan {
     bool wave_ok = false;
     ldsf::WaveTab wave{};
     int wave_group = 0;
+    bool edges_ready = false;  // plan->edge holds the last adjoint's records
     const int *species = nullptr;  // borrowed
     const int *nl = nullptr;       // borrowed
     int *centers = nullptr;        // owned (NULL = identity)
@@ -288,8 +289,9 @@ int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, 
     return LDSF_OK;
 }
 
-int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dcoords,
-                  int precision, hipStream_t s)
+// adjoint kernel only: one dE/dr record per edge and per centre in plan->edge
+int plan_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, int precision,
+                       hipStream_t s)
 {
     ldsf::SFTab tab;
     if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
@@ -307,9 +309,24 @@ int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dc
                                      plan->nl, plan->species, x, dE, plan->inv_pos, plan->edge,
                                      (int)plan->n_edges, s);
     HIP_TRY(e, "adjoint kernel launch");
+    plan->edges_ready = true;
+    return LDSF_OK;
+}
+
+int plan_gather(ldsf_plan *plan, double *dcoords, hipStream_t s)
+{
+    if (!plan->edges_ready && plan->n_edges + plan->n_centers > 0)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gather: no adjoint records (run the adjoint first)");
     HIP_TRY(ldsf::launch_gather(plan->n_total, plan->in_start, plan->edge, dcoords, s),
             "gather launch");
     return LDSF_OK;
+}
+
+int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dcoords,
+                  int precision, hipStream_t s)
+{
+    if (int rc = plan_adjoint_edges(plan, x, dE, precision, s)) return rc;
+    return plan_gather(plan, dcoords, s);
 }
 
 // ---- host-pointer entry points: stage, run a plan, copy back ----
@@ -501,6 +518,23 @@ LDSF_API int ldsf_plan_gradient(ldsf_plan *plan, const double *d_coordinates,
     if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
         return ldsf::fail(LDSF_ERR_ARG, "plan_gradient: unknown precision");
     return plan_gradient(plan, d_coordinates, d_dE_dzeta, d_dcoords, precision, (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_plan_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
+                                     const double *d_dE_dzeta, int precision, void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_adjoint_edges: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_adjoint_edges: unknown precision");
+    return plan_adjoint_edges(plan, d_coordinates, d_dE_dzeta, precision, (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream)
+{
+    if (!plan || (plan->n_total > 0 && !d_dcoords))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gather: bad arguments");
+    return plan_gather(plan, d_dcoords, (hipStream_t)stream);
 }
 
 LDSF_API int ldsf_compute(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
