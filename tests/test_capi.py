@@ -100,3 +100,16 @@ def test_python_api_surface():
         assert hasattr(ld, name), name
     assert int(ld.AvailableDescriptors.SymmetryFunctions) == 0
     assert int(ld.AvailableDescriptors.Bispectrum) == 1
+
+
+def test_cpp_drop_in_caller_links_against_the_library():
+    """tests/cpp/drop_in.cpp (built by build()) includes the drop-in headers
+    and resolves every C++ API symbol from libldsf.so."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "_build", "drop_in")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    out = subprocess.run(["ldd", "-r", exe], capture_output=True, text=True)
+    assert out.returncode == 0
+    assert "undefined symbol" not in out.stdout + out.stderr
+    assert "not found" not in out.stdout

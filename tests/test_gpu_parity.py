@@ -351,3 +351,33 @@ def test_lane_group_sizes_agree(group, monkeypatch):
             out.append((d.cpu().numpy(), dx.cpu().numpy()))
         assert elem_rel_err(out[1][0], out[0][0], 1e-30) <= (1e-13 if prec == 0 else 1e-6)
         assert rel_err(out[1][1], out[0][1]) <= (1e-13 if prec == 0 else 1e-6)
+
+
+def test_cpp_drop_in_caller(c1, tmp_path):
+    """A reference-style C++ program (tests/cpp/drop_in.cpp) through
+    Descriptor::compute / gradient / compute_single_atom on the GPU."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "_build", "drop_in")
+    n, nt, nnl = 64, len(c1["coords"]), len(c1["neighlist"])
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.asarray([n, nt, nnl], np.int32).tofile(f)
+        np.asarray(c1["species"], np.int32).tofile(f)
+        np.asarray(c1["numneigh"], np.int32).tofile(f)
+        np.asarray(c1["neighlist"], np.int32).tofile(f)
+        np.asarray(c1["coords"], np.float64).tofile(f)
+        np.asarray(c1["dE_dzeta"], np.float64).tofile(f)
+    print(f"running {exe} on C1", flush=True)
+    r = subprocess.run([exe, str(inp), str(outp)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = np.fromfile(outp, dtype=np.float64)
+    desc = out[: n * 24].reshape(n, 24)
+    grad = out[n * 24: n * 24 + nt * 3].reshape(nt, 3)
+    single = out[n * 24 + nt * 3:]
+    assert elem_rel_err(desc, c1["desc"], 1e-300) <= TOL64
+    assert elem_rel_err(single, c1["desc"][0], 1e-300) <= TOL64
+    O = oracle_of()
+    G0 = O.gradient(n, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"],
+                    c1["dE_dzeta"])
+    assert rel_err(grad, G0) <= TOL64
