@@ -44,6 +44,7 @@ def _rsa(n, rng, dmin, sample, periodic_L=None, batch=4096, max_tries=200):
     in order when no accepted point lies closer than dmin."""
     from scipy.spatial import cKDTree
     pts = np.zeros((0, 3))
+    batch = min(batch, max(64, 2 * n))  # small systems: few candidates per round
     tries = 0
     while len(pts) < n:
         tries += 1

@@ -46,6 +46,9 @@ def lib():
                                          _f64p, _f64p, _f64p]
         L.orc_fwd_adj_omp.argtypes = [vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p, _f64p,
                                       _f64p, _f64p, _f64p, ctypes.c_int]
+        L.orc_fwd_adj_batch_omp.argtypes = [vp, ctypes.c_int, _i32p, _i32p, ctypes.c_int, _i32p,
+                                            _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                            ctypes.c_int]
         L.orc_num_gradient_single_atom.argtypes = [vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
                                                    ctypes.c_int, _f64p, _f64p, _f64p]
         _lib = L
@@ -147,6 +150,18 @@ class OracleSF:
         lib().orc_num_gradient_single_atom(self._h, index, c.shape[0], _i32(species), nb, len(nb),
                                            c, _f64(dE_dzeta), out)
         return out
+
+    def fwd_adj_batch_omp(self, n_atoms, config_ptr, species, neighlist, numneigh, coords,
+                          dE_dzeta, n_threads):
+        """compute_batch + gradient_batch, OpenMP over configurations."""
+        n_atoms = _i32(n_atoms)
+        c = _f64(coords)
+        desc = np.zeros((int(n_atoms.sum()), self.width))
+        out = np.zeros_like(c)
+        lib().orc_fwd_adj_batch_omp(self._h, len(n_atoms), n_atoms, _i32(config_ptr), c.shape[0],
+                                    _i32(species), _i32(neighlist), _i32(numneigh), c,
+                                    _f64(dE_dzeta), desc, out, int(n_threads))
+        return desc, out
 
     def fwd_adj_omp(self, n_centers, species, neighlist, offsets, coords, dE_dzeta, n_threads):
         """Forward + adjoint of the first ``n_centers`` centres on ``n_threads``

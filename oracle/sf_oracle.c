@@ -489,6 +489,66 @@ void orc_fwd_adj_omp(const orc_sf *sf, int n_centers, int n_total, const int *sp
     free(buf);
 }
 
+/* The batch walk of src/Descriptors.cpp:138-163 / 263-292 (centre j of
+ * configuration c is row config_ptr[c] + j, its neighbours follow those of the
+ * previous centre), OpenMP over configurations, per-thread dE/dr buffers. */
+void orc_fwd_adj_batch_omp(const orc_sf *sf, int n_configs, const int *n_atoms,
+                           const int *config_ptr, int n_total, const int *species,
+                           const int *neighbor_list, const int *number_of_neighs,
+                           const double *coords, const double *dE_dzeta, double *desc,
+                           double *d_coords, int n_threads)
+{
+#ifdef _OPENMP
+    if (n_threads < 1) n_threads = omp_get_max_threads();
+#else
+    n_threads = 1;
+#endif
+    /* first centre and first neighbour entry of each configuration */
+    long *c0 = (long *)malloc(sizeof(long) * (n_configs + 1));
+    long *e0 = (long *)malloc(sizeof(long) * (n_configs + 1));
+    c0[0] = 0;
+    e0[0] = 0;
+    for (int c = 0; c < n_configs; ++c) {
+        long e = 0;
+        for (int j = 0; j < n_atoms[c]; ++j) e += number_of_neighs[c0[c] + j];
+        c0[c + 1] = c0[c] + n_atoms[c];
+        e0[c + 1] = e0[c] + e;
+    }
+    double *buf = (double *)calloc((size_t)n_threads * n_total * 3, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(n_threads)
+#endif
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double *mine = buf + (size_t)t * n_total * 3;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int c = 0; c < n_configs; ++c) {
+            const int *nl = neighbor_list + e0[c];
+            for (int j = 0; j < n_atoms[c]; ++j) {
+                const long k = c0[c] + j;
+                const int nn = number_of_neighs[k];
+                const int row = config_ptr[c] + j;
+                orc_compute_atom(sf, row, species, nl, nn, coords, desc + (size_t)k * sf->width);
+                orc_gradient_atom(sf, row, species, nl, nn, coords,
+                                  dE_dzeta + (size_t)k * sf->width, mine);
+                nl += nn;
+            }
+        }
+    }
+    for (int t = 0; t < n_threads; ++t) {
+        const double *src = buf + (size_t)t * n_total * 3;
+        for (size_t a = 0; a < (size_t)n_total * 3; ++a) d_coords[a] += src[a];
+    }
+    free(buf);
+    free(c0);
+    free(e0);
+}
+
 /* ---- finite differences ---- */
 
 /* numdiff::make_xph_representable (include/finite_difference.hpp:46-57) */

@@ -1,0 +1,149 @@
+"""Batched differentiable adjoint on the 512-configuration workload (atom
+counts log-uniform in [8, 4096], half periodic bulk at 0.04-0.06 atoms/A^3,
+half non-periodic clusters, two species, species-resolved 64-entry set),
+through torch.autograd with one plan -- one forward and one adjoint launch --
+over every centre of every configuration.
+
+Checks (tolerances written here):
+  FP64 descriptors vs the oracle's batch walk   : elementwise relative <= 1e-12
+  FP64 dE/dr vs the oracle's adjoint            : normwise per configuration <= 1e-12
+  dE/dr vs central differences, h = 1e-5 A      : |FD - g| <= 2e-6 * max(1, |g|), on rows
+                                                  with no pair distance within 4h of rc
+  FP32-bulk vs FP64                             : relative <= 1e-5 on entries >= 1e-6 of
+                                                  their block's maximum; dE/dr normwise <= 1e-5
+  a configuration alone vs inside the batch     : bitwise equal
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import behler, elem_rel_err, rel_err
+
+pytestmark = pytest.mark.gpu
+
+RC = 5.0
+
+
+@pytest.fixture(scope="module")
+def c4():
+    from libdescriptor_amd import workloads as w
+    Ls = w.config_c4()
+    return Ls, w.concat(Ls)
+
+
+def sf(precision=0):
+    import libdescriptor_amd as ld
+    types, sizes, params = behler()
+    return ld.DescriptorKind.init_symmetry_functions(["A", "B"], RC, types, sizes, params,
+                                                     species_resolved=True, precision=precision)
+
+
+def layer_for(B, ds, precision=0):
+    from libdescriptor_amd.autograd import SymmetryFunctionsLayer
+    return SymmetryFunctionsLayer(ds, B["species"], B["neighlist"], B["numneigh"],
+                                  n_atoms=B["n_atoms"], config_ptr=B["config_ptr"],
+                                  precision=precision)
+
+
+def test_c4_autograd_matches_oracle_and_central_fd(c4):
+    import torch
+    from oracle import orc
+    Ls, B = c4
+    n = int(B["n_atoms"].sum())
+    ds = sf()
+    layer = layer_for(B, ds)
+    assert layer.plan.n_centers == n
+    assert layer.plan.kernel_family == 1
+    x = torch.tensor(B["coords"], device="cuda", requires_grad=True)
+    print(f"c4: {len(Ls)} configurations, {n} centres, {len(B['coords'])} rows", flush=True)
+    z = layer(x)
+    w = torch.tensor(np.random.default_rng(42).normal(size=(n, ds.width)), device="cuda")
+    (w * z).sum().backward()
+    torch.cuda.synchronize()
+    zd = z.detach().cpu().numpy()
+    G = x.grad.cpu().numpy()
+
+    types, sizes, params = behler()
+    O = orc.OracleSF(["A", "B"], RC, types, sizes, params, species_resolved=True)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    d0, G0 = O.fwd_adj_batch_omp(B["n_atoms"], B["config_ptr"], B["species"], B["neighlist"],
+                                 B["numneigh"], B["coords"], w.cpu().numpy(), threads)
+    assert elem_rel_err(zd, d0, 1e-30) <= 1e-12
+    rows = np.r_[B["config_ptr"], len(B["coords"])]
+    worst = max(rel_err(G[rows[c]:rows[c + 1]], G0[rows[c]:rows[c + 1]], 1e-300)
+                for c in range(len(Ls)))
+    assert worst <= 1e-12
+
+    # central differences of sum(w * zeta) on sampled coordinate rows
+    h = 1e-5
+    rng = np.random.default_rng(7)
+    xs = x.detach()
+    checked = 0
+    for c in rng.permutation(len(Ls)):
+        lo, hi = int(rows[c]), int(rows[c + 1])
+        r = int(rng.integers(lo, hi))
+        d = np.linalg.norm(B["coords"][lo:hi] - B["coords"][r], axis=1)
+        if np.any(np.abs(d - RC) < 4 * h):
+            continue
+        k = int(rng.integers(3))
+        with torch.no_grad():
+            xp = xs.clone()
+            xp[r, k] += h
+            xm = xs.clone()
+            xm[r, k] -= h
+            fd = float((w * (layer(xp) - layer(xm))).sum()) / (2 * h)
+        assert abs(fd - G[r, k]) <= 2e-6 * max(1.0, abs(G[r, k])), (c, r, k, fd, G[r, k])
+        checked += 1
+        if checked == 12:
+            break
+    assert checked == 12
+
+
+def test_c4_fp32_bulk_against_fp64(c4):
+    import torch
+    _, B = c4
+    n = int(B["n_atoms"].sum())
+    out = {}
+    for prec in (0, 1):
+        layer = layer_for(B, sf(prec), prec)
+        x = torch.tensor(B["coords"], device="cuda", requires_grad=True)
+        z = layer(x)
+        w = torch.tensor(np.random.default_rng(43).normal(size=(n, 64)), device="cuda")
+        (w * z).sum().backward()
+        torch.cuda.synchronize()
+        out[prec] = (z.detach().cpu().numpy(), x.grad.cpu().numpy())
+    for lo, hi in ((0, 8), (8, 16), (16, 32), (32, 48), (48, 64)):
+        a, r = out[1][0][:, lo:hi], out[0][0][:, lo:hi]
+        sig = np.abs(r) >= 1e-6 * np.abs(r).max()
+        assert (np.abs(a - r)[sig] / np.abs(r)[sig]).max() <= 1e-5
+    assert rel_err(out[1][1], out[0][1]) <= 1e-5
+
+
+def test_c4_configuration_alone_equals_in_batch(c4):
+    """Results of a configuration do not depend on the batch it runs in."""
+    import torch
+    from libdescriptor_amd.autograd import SymmetryFunctionsLayer
+    Ls, B = c4
+    n = int(B["n_atoms"].sum())
+    rows = np.r_[B["config_ptr"], len(B["coords"])]
+    cent = np.r_[0, np.cumsum(B["n_atoms"])]
+    sizes = B["n_atoms"]
+    picks = [int(np.argmin(sizes)), int(np.argmax(sizes)), 0, 1, 255, 510]
+    for prec in (0, 1):
+        ds = sf(prec)
+        layer = layer_for(B, ds, prec)
+        wt = torch.tensor(np.random.default_rng(44).normal(size=(n, 64)), device="cuda")
+        x = torch.tensor(B["coords"], device="cuda", requires_grad=True)
+        z = layer(x)
+        (wt * z).sum().backward()
+        for c in picks:
+            L = Ls[c]
+            one = SymmetryFunctionsLayer(ds, L["species"], L["neighlist"], L["numneigh"],
+                                         precision=prec)
+            xc = torch.tensor(L["coords"], device="cuda", requires_grad=True)
+            zc = one(xc)
+            (wt[cent[c]:cent[c + 1]] * zc).sum().backward()
+            torch.cuda.synchronize()
+            assert torch.equal(zc.detach(), z.detach()[cent[c]:cent[c + 1]]), c
+            assert torch.equal(xc.grad, x.grad[rows[c]:rows[c + 1]]), c

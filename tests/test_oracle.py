@@ -292,3 +292,23 @@ def test_batch_walk_equals_single_configs():
     gb = O.gradient_batch([7, 9], [0, 7], sp, nl, nn, x, g)
     np.testing.assert_array_equal(gb[:7], O.gradient(7, sp[:7], nla, nna, a, g[:7]))
     np.testing.assert_array_equal(gb[7:], O.gradient(9, sp[:9], nlb, nnb, b, g[7:]))
+
+
+def test_batch_omp_equals_serial_batch_walk():
+    from libdescriptor_amd import workloads as w
+    Ls = [w.layout(*w.random_system(60, seed=s)[:3], w.random_system(60, seed=s)[3], 5.0,
+                   symbols=["A", "B"]) for s in (1, 2)]
+    Ls.append(w.layout(*w.cluster(40, seed=3)[:3], w.cluster(40, seed=3)[3], 5.0,
+                       symbols=["A", "B"]))
+    B = w.concat(Ls)
+    types, sizes, params = behler()
+    O = orc.OracleSF(["A", "B"], 5.0, types, sizes, params, species_resolved=True)
+    n = int(B["n_atoms"].sum())
+    g = np.random.default_rng(2).normal(size=(n, O.width))
+    args = (B["n_atoms"], B["config_ptr"], B["species"], B["neighlist"], B["numneigh"],
+            B["coords"])
+    d0 = O.compute_batch(*args)
+    G0 = O.gradient_batch(*args, g)
+    d1, G1 = O.fwd_adj_batch_omp(*args, g, 4)
+    np.testing.assert_array_equal(d1, d0)
+    assert np.abs(G1 - G0).max() <= 1e-13 * np.abs(G0).max()
