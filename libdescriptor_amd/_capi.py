@@ -108,14 +108,7 @@ class Plan:
         self._keep = (species, neighlist, numneigh, centers)
         out = ctypes.c_void_p()
         st = _stream_ptr(stream)
-        if n_atoms is not None:
-            na = np.ascontiguousarray(n_atoms, dtype=np.int32)
-            cp = np.ascontiguousarray(config_ptr, dtype=np.int32)
-            check(lib().ldsf_plan_create_batch(ctypes.byref(out), sf_handle, len(na), na.ctypes.data,
-                                               cp.ctypes.data, n_total, _ptr(species),
-                                               _ptr(neighlist), _ptr(numneigh), st))
-        else:
-            n_centers = int(numneigh.numel()) if hasattr(numneigh, "numel") else len(numneigh)
+        n_centers = int(numneigh.numel()) if hasattr(numneigh, "numel") else len(numneigh)
         # the C-ABI trusts the neighbour list to hold sum(numneigh) entries
         total = int(numneigh.sum()) if n_centers else 0
         n_nl = int(neighlist.numel()) if hasattr(neighlist, "numel") else len(neighlist)
@@ -123,7 +116,15 @@ class Plan:
         if total > n_nl or n_sp < n_total:
             raise ValueError("neighbour list shorter than sum(numneigh), or species shorter "
                              "than n_total")
-        if n_atoms is None:
+        if n_atoms is not None:
+            na = np.ascontiguousarray(n_atoms, dtype=np.int32)
+            cp = np.ascontiguousarray(config_ptr, dtype=np.int32)
+            if len(cp) != len(na) or int(na.sum()) != n_centers:
+                raise ValueError("n_atoms / config_ptr do not match numneigh")
+            check(lib().ldsf_plan_create_batch(ctypes.byref(out), sf_handle, len(na), na.ctypes.data,
+                                               cp.ctypes.data, n_total, _ptr(species),
+                                               _ptr(neighlist), _ptr(numneigh), st))
+        else:
             check(lib().ldsf_plan_create(ctypes.byref(out), sf_handle, n_centers, _ptr(centers),
                                          n_total, _ptr(species), _ptr(neighlist), _ptr(numneigh),
                                          st))

@@ -31,6 +31,46 @@ struct Real<0> { using T = double; };
 template <>
 struct Real<1> { using T = float; };
 
+// ---- the reference's FP64 expressions, evaluated without FMA contraction ----
+
+// |d|^2 as d0*d0 + d1*d1 + d2*d2 (src/SymmetryFunctions.cpp:162,218-219)
+__device__ __forceinline__ double dist2(double d0, double d1, double d2)
+{
+#pragma clang fp contract(off)
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// argument of cut_cos: MY_PI * r / rcut (src/SymmetryFunctions.cpp:68)
+__device__ __forceinline__ double ref_arg(double r, double rc)
+{
+#pragma clang fp contract(off)
+    return kPi * r / rc;
+}
+
+struct RefTrip {
+    double rijsq, riksq, rjksq, cosv, s2;
+};
+
+// squares from the magnitudes, cos_ijk by division, and the exponent's sum
+// (src/SymmetryFunctions.cpp:112-122)
+__device__ __forceinline__ RefTrip ref_trip(double rij, double rik, double rjk)
+{
+#pragma clang fp contract(off)
+    RefTrip t;
+    t.rijsq = rij * rij;
+    t.riksq = rik * rik;
+    t.rjksq = rjk * rjk;
+    t.cosv = (t.rijsq + t.riksq - t.rjksq) / (2 * rij * rik);
+    t.s2 = t.rijsq + t.riksq + t.rjksq;
+    return t;
+}
+
+__device__ __forceinline__ double ref_base(double lam, double cosv)
+{
+#pragma clang fp contract(off)
+    return 1 + lam * cosv;
+}
+
 // ---- cutoff function fc(r) = 0.5 (cos(pi r / rc) + 1) for r < rc, and dfc/dr ----
 
 __device__ __forceinline__ void fcut(double r, double rc, double &fc, double &dfc, Real<0>)

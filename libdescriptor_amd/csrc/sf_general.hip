@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
         const int sj = species[j];
         const double rcij = tab.rc[si * ns + sj];
         const double dj0 = x[3 * j] - xi0, dj1 = x[3 * j + 1] - xi1, dj2 = x[3 * j + 2] - xi2;
-        const double rij = sqrt(dj0 * dj0 + dj1 * dj1 + dj2 * dj2);
+        const double rij = sqrt(dist2(dj0, dj1, dj2));
         if (!(rij < rcij)) continue;  // r > rc is skipped; r == rc contributes exactly 0
         const T fij = fcut_value(rij, rcij, R);
         for (int q = 0; q < tab.n2; ++q) {
@@ -69,10 +69,11 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
             const double dk0 = x[3 * k] - xi0, dk1 = x[3 * k + 1] - xi1, dk2 = x[3 * k + 2] - xi2;
             const double e0 = x[3 * k] - x[3 * j], e1 = x[3 * k + 1] - x[3 * j + 1],
                          e2 = x[3 * k + 2] - x[3 * j + 2];
-            const double rik = sqrt(dk0 * dk0 + dk1 * dk1 + dk2 * dk2);
-            const double rjk = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            const double rik = sqrt(dist2(dk0, dk1, dk2));
+            const double rjk = sqrt(dist2(e0, e1, e2));
             if (!(rik < rcik)) continue;
-            const double rijsq = rij * rij, riksq = rik * rik, rjksq = rjk * rjk;
+            const RefTrip rt = ref_trip(rij, rik, rjk);
+            const double rijsq = rt.rijsq, riksq = rt.riksq, rjksq = rt.rjksq;
             const bool jk_in = rjk < rcjk;
             const T fik = fcut_value(rik, rcik, R);
             const T fjk = jk_in ? fcut_value(rjk, rcjk, R) : (T)0;
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
             const double den = 2 * rij * rik;
             T cosv, onep, onem;
             if (P == 0) {
-                cosv = (T)((rijsq + riksq - rjksq) / den);
+                cosv = (T)rt.cosv;
                 onep = (T)0;
                 onem = (T)0;
             } else {
@@ -95,11 +96,11 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
                 if (kd == G4 && !jk_in) continue;
                 const double lam = tab.lam3[q];
                 T base;
-                if (P == 0) base = (T)(1 + lam * (double)cosv);
+                if (P == 0) base = (T)ref_base(lam, (double)cosv);
                 else base = lam == 1.0 ? onep : (lam == -1.0 ? onem : (T)1 + (T)lam * cosv);
                 if (base <= (T)0) continue;
                 const T cost = dpow(base, tab.zeta3[q], tab.izeta3[q], R);
-                const double esum = kd == G4 ? rijsq + riksq + rjksq : rijsq + riksq;
+                const double esum = kd == G4 ? rt.s2 : rijsq + riksq;
                 const T eterm = dexp(-tab.eta3[q] * esum, R);
                 T v = (T)tab.coef3[q] * cost * eterm * fij * fik;
                 if (kd == G4) v *= fjk;
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
         const int sj = species[j];
         const double rcij = tab.rc[si * ns + sj];
         const double dj0 = x[3 * j] - xi0, dj1 = x[3 * j + 1] - xi1, dj2 = x[3 * j + 2] - xi2;
-        const double rij = sqrt(dj0 * dj0 + dj1 * dj1 + dj2 * dj2);
+        const double rij = sqrt(dist2(dj0, dj1, dj2));
         if (!(rij < rcij)) continue;
         T fij, dfij;
         fcut(rij, rcij, fij, dfij, R);
@@ -179,10 +180,11 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
             const double dk0 = x[3 * k] - xi0, dk1 = x[3 * k + 1] - xi1, dk2 = x[3 * k + 2] - xi2;
             const double e0 = x[3 * k] - x[3 * j], e1 = x[3 * k + 1] - x[3 * j + 1],
                          e2 = x[3 * k + 2] - x[3 * j + 2];
-            const double rik = sqrt(dk0 * dk0 + dk1 * dk1 + dk2 * dk2);
-            const double rjk = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            const double rik = sqrt(dist2(dk0, dk1, dk2));
+            const double rjk = sqrt(dist2(e0, e1, e2));
             if (!(rik < rcik)) continue;
-            const double rijsq = rij * rij, riksq = rik * rik, rjksq = rjk * rjk;
+            const RefTrip rt = ref_trip(rij, rik, rjk);
+            const double rijsq = rt.rijsq, riksq = rt.riksq, rjksq = rt.rjksq;
             const bool jk_in = rjk < rcjk;
             T fik, dfik, fjk = (T)0, dfjk = (T)0;
             fcut(rik, rcik, fik, dfik, R);
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
             const int pidx = pair_index(ns, sj, sk);
             const double den = 2 * rij * rik;
             // cos(theta) and its partial derivatives w.r.t. rij, rik, rjk
-            const T cosv = (T)((rijsq + riksq - rjksq) / den);
+            const T cosv = P == 0 ? (T)rt.cosv : (T)((rijsq + riksq - rjksq) / den);
             const T dc_ij = (T)((rijsq - riksq + rjksq) / (2 * rijsq * rik));
             const T dc_ik = (T)((riksq - rijsq + rjksq) / (2 * rij * riksq));
             const T dc_jk = (T)(-rjk / (rij * rik));
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
                 if (kd == G4 && !jk_in) continue;
                 const double lam = tab.lam3[q];
                 T base;
-                if (P == 0) base = (T)(1 + lam * (double)cosv);
+                if (P == 0) base = (T)ref_base(lam, (double)cosv);
                 else base = lam == 1.0 ? onep : (lam == -1.0 ? onem : (T)1 + (T)lam * cosv);
                 if (base <= (T)0) continue;
                 const double zeta = tab.zeta3[q];

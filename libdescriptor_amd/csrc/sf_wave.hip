@@ -214,7 +214,7 @@ struct Shared {
 
 // one group's arrays, all sized for maxn neighbours (lists for tmax pairs)
 struct Grp {
-    double *dx, *dy, *dz, *r2, *r, *rinv, *fc;
+    double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
     double *dfc, *fx, *fy, *fz, *w, *wv;  // adjoint only
     int *sp, *orig;
     unsigned short *l0, *l1;
@@ -249,9 +249,9 @@ __device__ __forceinline__ Grp carve_group(double *base, int g, int maxn, int tm
               (size_t)g * group_bytes(maxn, tmax, adjoint, npb);
     double *p = reinterpret_cast<double *>(q);
     Grp L;
-    L.dx = p; p += maxn;
-    L.dy = p; p += maxn;
-    L.dz = p; p += maxn;
+    L.px = p; p += maxn;
+    L.py = p; p += maxn;
+    L.pz = p; p += maxn;
     L.r2 = p; p += maxn;
     L.r = p; p += maxn;
     L.rinv = p; p += maxn;
@@ -297,7 +297,7 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
                                          double &dfc, Real<0>)
 {
     double s, c;
-    sincos(kPi * r * rcinv, &s, &c);
+    sincos(ref_arg(r, rc), &s, &c);
     fc = 0.5 * (c + 1.0);
     dfc = -0.5 * s * (kPi * rcinv);
 }
@@ -327,8 +327,7 @@ __device__ __forceinline__ int enumerate_pairs(const Shared &S, const Grp &L, in
         int code = 0;
         if (d < M) {
             const int b = a + d;
-            const double e0 = L.dx[b] - L.dx[a], e1 = L.dy[b] - L.dy[a], e2 = L.dz[b] - L.dz[a];
-            const double rjk2 = e0 * e0 + e1 * e1 + e2 * e2;
+            const double rjk2 = dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
             const int sa = L.sp[a], sb = L.sp[b];
             // the reference tests sqrt(r_jk^2) < rc_jk; the squared test differs
             // only within an ulp of rc_jk, where fc_jk vanishes quadratically
@@ -394,34 +393,33 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
     t.rib = L.rinv[b];
     t.fa = (T)L.fc[a];
     t.fb = (T)L.fc[b];
-    t.e0 = L.dx[b] - L.dx[a];
-    t.e1 = L.dy[b] - L.dy[a];
-    t.e2 = L.dz[b] - L.dz[a];
-    const double rjk2 = t.e0 * t.e0 + t.e1 * t.e1 + t.e2 * t.e2;
+    // x_k - x_j from the absolute coordinates, as src/SymmetryFunctions.cpp:216-217
+    t.e0 = L.px[b] - L.px[a];
+    t.e1 = L.py[b] - L.py[a];
+    t.e2 = L.pz[b] - L.pz[a];
+    const double rjk2 = dist2(t.e0, t.e1, t.e2);
     t.B[0] = t.B[1] = (T)0;
     t.E[0] = t.E[1] = (T)0;
     if (P == 0) {
-        // sym_g4's own arithmetic (src/SymmetryFunctions.cpp:112-125)
+        // sym_g4's own arithmetic and operation order (src/SymmetryFunctions.cpp:112-125)
         t.rjk = sqrt(rjk2);
-        const double rijsq = t.ra * t.ra, riksq = t.rb * t.rb, rjksq = t.rjk * t.rjk;
-        const double h = 0.5 * t.ria * t.rib;
-        const double cosv = (rijsq + riksq - rjksq) * h;
-        const double S2 = rijsq + riksq + rjksq;
+        const RefTrip rt = ref_trip(t.ra, t.rb, t.rjk);
 #pragma unroll
         for (int l = 0; l < 2; ++l)
-            if (l < wt.nl) t.B[l] = (T)(1 + wp.lam[l] * cosv);
+            if (l < wt.nl) t.B[l] = (T)ref_base(wp.lam[l], rt.cosv);
 #pragma unroll
         for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * S2);
+            if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * rt.s2);
         double sn, cs;
-        sincos(kPi * t.rjk * rcinv, &sn, &cs);
+        sincos(ref_arg(t.rjk, rc), &sn, &cs);
         t.fjk = (T)(0.5 * (cs + 1.0));
         t.dfjk = (T)(-0.5 * sn * (kPi * rcinv));
+        const double h = 0.5 * t.ria * t.rib;
         t.inv2 = (T)h;
         if (ADJ) {
             t.rjkinv = 1.0 / t.rjk;
-            t.nij = rijsq - riksq + rjksq;
-            t.nik = riksq - rijsq + rjksq;
+            t.nij = rt.rijsq - rt.riksq + rt.rjksq;
+            t.nik = rt.riksq - rt.rijsq + rt.rjksq;
         }
     } else {
         const double r2a = L.r2[a], r2b = L.r2[b];
@@ -509,14 +507,17 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
             const int jj = j0 + s;
             bool valid = false;
             int sj = 0;
-            double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
+            double p0 = 0, p1 = 0, p2 = 0, d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
             if (jj < N) {
                 const int j = nl[beg + jj];
                 sj = species[j];
-                d0 = x[3 * (size_t)j] - xi0;
-                d1 = x[3 * (size_t)j + 1] - xi1;
-                d2 = x[3 * (size_t)j + 2] - xi2;
-                r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                p0 = x[3 * (size_t)j];
+                p1 = x[3 * (size_t)j + 1];
+                p2 = x[3 * (size_t)j + 2];
+                d0 = p0 - xi0;
+                d1 = p1 - xi1;
+                d2 = p2 - xi2;
+                r2 = dist2(d0, d1, d2);
                 r = sqrt(r2);
                 valid = r < S.rc[si * ns + sj];  // r > rc is skipped; r == rc adds exactly 0
             }
@@ -526,9 +527,9 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                 const int k = si * ns + sj;
                 T fc, dfc;
                 cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
-                L.dx[pos] = d0;
-                L.dy[pos] = d1;
-                L.dz[pos] = d2;
+                L.px[pos] = p0;
+                L.py[pos] = p1;
+                L.pz[pos] = p2;
                 L.r2[pos] = r2;
                 L.r[pos] = r;
                 L.rinv[pos] = 1.0 / r;
@@ -649,14 +650,17 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
             const int jj = j0 + s;
             bool valid = false;
             int sj = 0;
-            double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
+            double p0 = 0, p1 = 0, p2 = 0, d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
             if (jj < N) {
                 const int j = nl[beg + jj];
                 sj = species[j];
-                d0 = x[3 * (size_t)j] - xi0;
-                d1 = x[3 * (size_t)j + 1] - xi1;
-                d2 = x[3 * (size_t)j + 2] - xi2;
-                r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                p0 = x[3 * (size_t)j];
+                p1 = x[3 * (size_t)j + 1];
+                p2 = x[3 * (size_t)j + 2];
+                d0 = p0 - xi0;
+                d1 = p1 - xi1;
+                d2 = p2 - xi2;
+                r2 = dist2(d0, d1, d2);
                 r = sqrt(r2);
                 valid = r < S.rc[si * ns + sj];
                 if (!valid) {
@@ -684,9 +688,9 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                     }
                 }
                 const double ir = 1.0 / r;
-                L.dx[pos] = d0;
-                L.dy[pos] = d1;
-                L.dz[pos] = d2;
+                L.px[pos] = p0;
+                L.py[pos] = p1;
+                L.pz[pos] = p2;
                 L.r2[pos] = r2;
                 L.r[pos] = r;
                 L.rinv[pos] = ir;
@@ -767,12 +771,12 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
                         const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
                                      a3 = (double)Ajk * q.rjkinv;
-                        atomicAdd(&L.fx[a], a1 * L.dx[a] - a3 * q.e0);
-                        atomicAdd(&L.fy[a], a1 * L.dy[a] - a3 * q.e1);
-                        atomicAdd(&L.fz[a], a1 * L.dz[a] - a3 * q.e2);
-                        atomicAdd(&L.fx[b], a2 * L.dx[b] + a3 * q.e0);
-                        atomicAdd(&L.fy[b], a2 * L.dy[b] + a3 * q.e1);
-                        atomicAdd(&L.fz[b], a2 * L.dz[b] + a3 * q.e2);
+                        atomicAdd(&L.fx[a], a1 * (L.px[a] - xi0) - a3 * q.e0);
+                        atomicAdd(&L.fy[a], a1 * (L.py[a] - xi1) - a3 * q.e1);
+                        atomicAdd(&L.fz[a], a1 * (L.pz[a] - xi2) - a3 * q.e2);
+                        atomicAdd(&L.fx[b], a2 * (L.px[b] - xi0) + a3 * q.e0);
+                        atomicAdd(&L.fy[b], a2 * (L.py[b] - xi1) + a3 * q.e1);
+                        atomicAdd(&L.fz[b], a2 * (L.pz[b] - xi2) + a3 * q.e2);
                     }
                 }
                 __syncthreads();

@@ -69,7 +69,11 @@ def test_c4_autograd_matches_oracle_and_central_fd(c4):
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     d0, G0 = O.fwd_adj_batch_omp(B["n_atoms"], B["config_ptr"], B["species"], B["neighlist"],
                                  B["numneigh"], B["coords"], w.cpu().numpy(), threads)
-    assert elem_rel_err(zd, d0, 1e-30) <= 1e-12
+    e = np.abs(zd - d0) / np.maximum(np.abs(d0), 1e-30)
+    worst_e = np.unravel_index(np.argmax(e), e.shape)
+    print(f"descriptor elementwise max rel {e.max():.3e} at {worst_e}: gpu {zd[worst_e]!r} "
+          f"oracle {d0[worst_e]!r}", flush=True)
+    assert e.max() <= 1e-12
     rows = np.r_[B["config_ptr"], len(B["coords"])]
     worst = max(rel_err(G[rows[c]:rows[c + 1]], G0[rows[c]:rows[c + 1]], 1e-300)
                 for c in range(len(Ls)))
