@@ -71,6 +71,17 @@ def main():
     ap.add_argument("--family", type=int, default=-1, help="force kernel family (0 general, 1 wave)")
     ap.add_argument("--only", choices=["fp64", "fp32_bulk"], default=None,
                     help="time one form only and print its timings (for profiling runs)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched (512 configurations) and serving measurements")
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--serving-lane-centers", type=int, default=49152)
+    ap.add_argument("--serving-min-centers", type=int, default=0)
+    ap.add_argument("--serving-load", type=float, default=0.8,
+                    help="arrival rate as a fraction of the offline request rate")
+    ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--serving-dump", default=None, metavar="NPZ",
+                    help="write request sizes, arrival times and latencies to NPZ")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -82,6 +93,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.serving_only:
+        print(json.dumps({"serving": bench_serving(args)}), flush=True)
         return
 
     import torch
@@ -197,6 +211,11 @@ def main():
         if plan.kernel_family == 1 else ("k_adj_general" if dom == "adjoint" else "k_fwd_general")
     traffic = measured_traffic(dom_kernel, n) if args.n == 262144 else None
 
+    batched = serving = None
+    if not args.no_extra:
+        batched = bench_batched(args, ds)
+        serving = bench_serving(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, types, sizes, params, np.asarray(g.cpu()))
@@ -232,9 +251,150 @@ def main():
                      "traffic": traffic,
                      "hbm_GBps": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
         "kernel_family": plan.kernel_family,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_batched(args, ds):
+    """512 configurations (atom counts log-uniform in [8, 4096], half periodic
+    bulk at 0.04-0.06 atoms/A^3, half clusters) as one batch plan: one forward
+    and one adjoint launch over every centre, both precision forms."""
+    import torch
+    from libdescriptor_amd import _capi, workloads as w
+    t0 = time.time()
+    B = w.concat(w.config_c4())
+    log(f"batched workload ready ({time.time() - t0:.1f} s)")
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.int32: torch.tensor(np.asarray(a), dtype=dt, device=dev)  # noqa: E731
+    n = int(B["n_atoms"].sum())
+    x = t(B["coords"], torch.float64)
+    g = t(np.random.default_rng(2).normal(size=(n, ds.width)), torch.float64)
+    plan = _capi.Plan(ds._handle, len(B["coords"]), t(B["species"]), t(B["neighlist"]),
+                      t(B["numneigh"]), n_atoms=B["n_atoms"], config_ptr=B["config_ptr"])
+    desc = torch.zeros((n, ds.width), dtype=torch.float64, device=dev)
+    dx = torch.zeros_like(x)
+    out = {"configurations": int(len(B["n_atoms"])), "atoms": n, "rows": int(len(B["coords"])),
+           "kernel_family": plan.kernel_family}
+    for name, prec in (("fp64", _capi.FP64), ("fp32_bulk", _capi.FP32_BULK)):
+        def step():
+            desc.zero_()
+            dx.zero_()
+            plan.compute(x, desc, prec)
+            plan.gradient(x, g, dx, prec)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t1) * 1e3 / args.steps
+        out[name] = {"ms_per_step": ms, "atom_envs_per_s": n / (ms * 1e-3)}
+        log(f"batched {name}: {ms:.3f} ms/step, {n / (ms * 1e-3):.3e} atom-envs/s")
+    return out
+
+
+def bench_serving(args):
+    """Poisson request stream through the serving loop (ldsf_server): each
+    request one configuration like the 262k-atom system's (random periodic,
+    0.05 atoms/A^3, two species, species-resolved 64-entry set), atom count
+    log-uniform in [16, 20000], asking descriptors + dE/dr for a random
+    dE/dzeta, FP32-bulk.  Inputs and outputs are device-resident; latency is
+    submission to completion.  Offline throughput: every request submitted at
+    once.  Online: Poisson arrivals at --serving-load (default 80 %) of the
+    offline request rate."""
+    import threading
+
+    import torch
+    import libdescriptor_amd as ld
+    from libdescriptor_amd import _capi, workloads as w
+    types, sizes, params = w.behler_si_params()
+    ds = ld.DescriptorKind.init_symmetry_functions(["A", "B"], 5.0, types, sizes, params,
+                                                   species_resolved=True, precision=1)
+    t0 = time.time()
+    pool = w.config_c5_pool()
+    log(f"serving pool ready: {len(pool)} configurations ({time.time() - t0:.1f} s)")
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.int32: torch.tensor(np.asarray(a), dtype=dt, device=dev)  # noqa: E731
+    rng = np.random.default_rng(5)
+    P = []
+    for k, L in enumerate(pool):
+        n = L["n_contrib"]
+        P.append(dict(species=t(L["species"]), numneigh=t(L["numneigh"]), nl=t(L["neighlist"]),
+                      x=t(L["coords"], torch.float64),
+                      dE=t(rng.normal(size=(n, ds.width)), torch.float64),
+                      max_nn=int(L["numneigh"].max()), n=n))
+    nreq = args.serving_requests
+    pick = rng.integers(len(P), size=nreq)
+    outs = [(torch.empty((P[k]["n"], ds.width), dtype=torch.float64, device=dev),
+             torch.empty_like(P[k]["x"])) for k in pick]
+    torch.cuda.synchronize()
+    atoms = int(sum(P[k]["n"] for k in pick))
+    lc = args.serving_lane_centers
+    srv = _capi.Server(ds._handle, _capi.FP32_BULK, hbm_budget_bytes=8 << 30,
+                       lane_centers=lc, lane_rows=max(3 * lc, 65536), lane_edges=lc * 48,
+                       max_requests=256, max_lanes=args.serving_lanes,
+                       dispatch_min_centers=args.serving_min_centers)
+    lanes = srv.num_lanes
+
+    def submit(i):
+        D = P[pick[i]]
+        return srv.submit(D["species"], D["numneigh"], D["nl"], D["x"], D["dE"], outs[i][0],
+                          outs[i][1], D["max_nn"])
+
+    # warm-up: capture the lanes' graphs for the buckets in use
+    for tk in [submit(i) for i in range(min(nreq, 64))]:
+        srv.wait(tk)
+    # offline: everything queued at once
+    t1 = time.perf_counter()
+    tickets = [submit(i) for i in range(nreq)]
+    for tk in tickets:
+        srv.wait(tk)
+    off_s = time.perf_counter() - t1
+    off_rate = atoms / off_s
+    log(f"serving offline: {nreq} requests, {atoms} atoms in {off_s * 1e3:.1f} ms "
+        f"({off_rate:.3e} atom-envs/s)")
+    # online: Poisson arrivals at a fraction of the offline request rate
+    lam = args.serving_load * nreq / off_s
+    arrivals = np.cumsum(np.random.default_rng(6).exponential(1.0 / lam, size=nreq))
+    lat = np.zeros(nreq)
+    tick = [None] * nreq
+    done = threading.Event()
+
+    def waiter():
+        for i in range(nreq):
+            while tick[i] is None:
+                time.sleep(0)
+            lat[i] = srv.wait(tick[i])
+        done.set()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t2 = time.perf_counter()
+    for i in range(nreq):
+        while time.perf_counter() - t2 < arrivals[i]:
+            pass
+        tick[i] = submit(i)
+    th.join()
+    on_s = time.perf_counter() - t2
+    st = srv.stats()
+    srv.close()
+    if args.serving_dump:
+        np.savez(args.serving_dump, atoms=np.array([P[k]["n"] for k in pick]), arrival=arrivals,
+                 latency=lat)
+    p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+    log(f"serving online: {atoms / on_s:.3e} atom-envs/s, p50 {p50 * 1e3:.3f} ms, "
+        f"p99 {p99 * 1e3:.3f} ms")
+    return {"requests": nreq, "atoms": atoms, "pool_configurations": len(P),
+            "offline_atom_envs_per_s": off_rate,
+            "online_atom_envs_per_s": atoms / on_s,
+            "online_fraction_of_offline": (atoms / on_s) / off_rate,
+            "arrival_rate_per_s": lam, "p50_ms": p50 * 1e3, "p99_ms": p99 * 1e3,
+            "p99_over_p50": p99 / p50, "lanes": lanes, "lane_centers": lc,
+            "dispatch_min_centers": args.serving_min_centers, "server_stats": st}
 
 
 def measured_traffic(kernel, n):

@@ -37,6 +37,7 @@ extern "C" {
 #define LDSF_ERR_SINGULAR_CELL 6   /* padding: cell matrix not invertible */
 #define LDSF_ERR_NOMEM 7           /* device allocation failed */
 #define LDSF_ERR_KIND 8            /* descriptor kind other than SymmetryFunctions */
+#define LDSF_ERR_TIMEOUT 9         /* ldsf_server_wait: request not finished in time */
 
 /* precision forms */
 #define LDSF_FP64 0       /* plain form: FP64 throughout */
@@ -185,6 +186,64 @@ LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream);
  * per-centre LDS kernels.  Setting 0 forces the general kernels. */
 LDSF_API int ldsf_plan_kernel_family(const ldsf_plan *plan);
 LDSF_API int ldsf_plan_set_kernel_family(ldsf_plan *plan, int family);
+
+/* ------------------------------------------------------------------------
+ * Serving loop (no reference counterpart).  Requests of descriptors + dE/dr
+ * over device-resident inputs are admitted in arrival order into lanes --
+ * device buffers for one batch each, sized by the config, as many lanes as
+ * hbm_budget_bytes holds (at most max_lanes) -- and each batch runs as one
+ * replay of the lane's hipGraph on the lane's stream, concurrently with the
+ * other lanes.  Requests arriving meanwhile wait only for a free lane.
+ * Outputs are overwritten (=), not accumulated.  A request's buffers must stay
+ * valid and unmodified until ldsf_server_wait has returned for it.
+ * ------------------------------------------------------------------------ */
+typedef struct ldsf_server ldsf_server;
+
+typedef struct {
+    int precision;            /* LDSF_FP64 or LDSF_FP32_BULK */
+    size_t hbm_budget_bytes;  /* device memory the lanes may hold together */
+    int lane_centers;         /* centres per batch */
+    int lane_rows;            /* coordinate rows (padding included) per batch */
+    int lane_edges;           /* neighbour-list entries per batch */
+    int max_requests;         /* requests per batch (<= 65535) */
+    int max_lanes;            /* upper bound on concurrent batches */
+    int dispatch_min_centers; /* while a batch runs, a free lane starts only once this many
+                                 centres are queued (0: at once); an idle device always
+                                 starts at once */
+} ldsf_server_config;
+
+typedef struct {
+    int n_atoms;                   /* centres: rows 0..n_atoms-1 */
+    int n_total;                   /* coordinate rows */
+    int n_neigh;                   /* entries of neighbor_list = sum(number_of_neighs) */
+    int max_neighbors;             /* bound on every number_of_neighs[i] */
+    const int *species;            /* device [n_total] */
+    const int *number_of_neighs;   /* device [n_atoms] */
+    const int *neighbor_list;      /* device [n_neigh], rows 0..n_total-1 */
+    const double *coordinates;     /* device [n_total][3] */
+    const double *dE_dzeta;        /* device [n_atoms][width]; NULL = zero (no gradient) */
+    double *desc;                  /* device [n_atoms][width], written */
+    double *d_coordinates;         /* device [n_total][3], written; NULL = not returned */
+} ldsf_request;
+
+/* Create on the current HIP device; starts one worker and one completion
+ * thread per lane. */
+LDSF_API int ldsf_server_create(ldsf_server **out, const ldsf_sf *sf,
+                                const ldsf_server_config *config);
+LDSF_API int ldsf_server_num_lanes(const ldsf_server *server);
+/* Queue a request (thread-safe); *ticket identifies it for ldsf_server_wait. */
+LDSF_API int ldsf_server_submit(ldsf_server *server, const ldsf_request *request,
+                                long long *ticket);
+/* Block until the request has finished (timeout_s < 0: no limit); returns its
+ * status (LDSF_ERR_NEIGHBOR: an index, species code or count out of range was
+ * found on the device) and the time from submission to completion.  A ticket
+ * is waited for once. */
+LDSF_API int ldsf_server_wait(ldsf_server *server, long long ticket, double timeout_s,
+                              double *latency_s);
+LDSF_API int ldsf_server_stats(const ldsf_server *server, long long *batches,
+                               long long *requests, long long *atoms, long long *graphs);
+/* Finishes every queued request, then frees the lanes. */
+LDSF_API void ldsf_server_destroy(ldsf_server *server);
 
 /* ------------------------------------------------------------------------
  * Input-layout producers (python_bindings/neighbor/neighbor_list.h)

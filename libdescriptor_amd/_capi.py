@@ -10,6 +10,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libldsf.so")
 
 LDSF_OK = 0
+LDSF_ERR_NEIGHBOR = 5
+LDSF_ERR_TIMEOUT = 9
 FP64 = 0
 FP32_BULK = 1
 
@@ -59,7 +61,29 @@ _PROTOS = {
     "ldsf_nbl_num_particles": (_i, [_vp]),
     "ldsf_nbl_total": (_l, [_vp, _i, _i]),
     "ldsf_nbl_flatten": (_i, [_vp, _i, _i, _vp, _vp]),
+    "ldsf_server_create": (_i, [_vp, _vp, _vp]),
+    "ldsf_server_num_lanes": (_i, [_vp]),
+    "ldsf_server_submit": (_i, [_vp, _vp, _vp]),
+    "ldsf_server_wait": (_i, [_vp, ctypes.c_longlong, _d, _vp]),
+    "ldsf_server_stats": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "ldsf_server_destroy": (None, [_vp]),
 }
+
+
+class ServerConfig(ctypes.Structure):
+    """ldsf_server_config (include/ldsf.h)."""
+    _fields_ = [("precision", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_size_t),
+                ("lane_centers", ctypes.c_int), ("lane_rows", ctypes.c_int),
+                ("lane_edges", ctypes.c_int), ("max_requests", ctypes.c_int),
+                ("max_lanes", ctypes.c_int), ("dispatch_min_centers", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    """ldsf_request (include/ldsf.h)."""
+    _fields_ = [("n_atoms", ctypes.c_int), ("n_total", ctypes.c_int), ("n_neigh", ctypes.c_int),
+                ("max_neighbors", ctypes.c_int), ("species", _vp), ("number_of_neighs", _vp),
+                ("neighbor_list", _vp), ("coordinates", _vp), ("dE_dzeta", _vp), ("desc", _vp),
+                ("d_coordinates", _vp)]
 
 
 def lib():
@@ -188,3 +212,68 @@ def _stream_ptr(stream):
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
+
+
+class Server:
+    """The serving loop (ldsf_server): submit() queues a request over device
+    tensors and returns a ticket; wait(ticket) blocks until it has finished
+    and returns its latency in seconds (submission to completion).  The
+    tensors of a request are kept alive until its wait."""
+
+    def __init__(self, sf_handle, precision=FP32_BULK, hbm_budget_bytes=2 << 30,
+                 lane_centers=65536, lane_rows=196608, lane_edges=65536 * 48, max_requests=256,
+                 max_lanes=4, dispatch_min_centers=0):
+        cfg = ServerConfig(int(precision), int(hbm_budget_bytes), int(lane_centers),
+                           int(lane_rows), int(lane_edges), int(max_requests), int(max_lanes),
+                           int(dispatch_min_centers))
+        out = ctypes.c_void_p()
+        check(lib().ldsf_server_create(ctypes.byref(out), sf_handle, ctypes.byref(cfg)))
+        self.handle = out.value
+        self._live = {}
+
+    @property
+    def num_lanes(self):
+        return lib().ldsf_server_num_lanes(self.handle)
+
+    def submit(self, species, numneigh, neighlist, coords, dE_dzeta, desc, dcoords,
+               max_neighbors):
+        """All tensors on the server's device: int32 species / numneigh /
+        neighlist, float64 coords (n_total, 3), dE_dzeta and desc (n, width),
+        dcoords (n_total, 3); dE_dzeta and dcoords may be None.  The server
+        runs on its own streams: work that produces the inputs (or touches the
+        outputs) on other streams must be finished before the call."""
+        n, nt, ne = int(numneigh.numel()), int(species.numel()), int(neighlist.numel())
+        for t, dt in ((species, "int32"), (numneigh, "int32"), (neighlist, "int32"),
+                      (coords, "float64"), (desc, "float64")):
+            if str(t.dtype).split(".")[-1] != dt or not t.is_contiguous():
+                raise TypeError(f"expected a contiguous {dt} tensor")
+        if coords.numel() != 3 * nt or desc.shape[0] != n:
+            raise ValueError("request arrays do not match")
+        req = Request(n, nt, ne, int(max_neighbors), _ptr(species), _ptr(numneigh),
+                      _ptr(neighlist), _ptr(coords), _ptr(dE_dzeta), _ptr(desc), _ptr(dcoords))
+        ticket = ctypes.c_longlong()
+        check(lib().ldsf_server_submit(self.handle, ctypes.byref(req), ctypes.byref(ticket)))
+        self._live[ticket.value] = (species, numneigh, neighlist, coords, dE_dzeta, desc, dcoords)
+        return ticket.value
+
+    def wait(self, ticket, timeout=-1.0):
+        lat = ctypes.c_double()
+        rc = lib().ldsf_server_wait(self.handle, int(ticket), float(timeout), ctypes.byref(lat))
+        if rc != LDSF_ERR_TIMEOUT:  # finished (or unknown): its buffers are free again
+            self._live.pop(int(ticket), None)
+        check(rc)
+        return lat.value
+
+    def stats(self):
+        v = [ctypes.c_longlong() for _ in range(4)]
+        check(lib().ldsf_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("batches", "requests", "atoms", "graphs"), (x.value for x in v)))
+
+    def close(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.ldsf_server_destroy(h)
+        self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -71,6 +71,40 @@ __device__ __forceinline__ double ref_base(double lam, double cosv)
     return 1 + lam * cosv;
 }
 
+// ---- transposed neighbour lists ----
+
+// Sorts ids[b .. b+m) ascending with one wavefront (all 64 lanes call it with
+// the same b, m): a bitonic network across the lanes for m <= 64, an insertion
+// sort on lane 0 for longer buckets.
+__device__ inline void sort_bucket_wave(int *ids, long b, int m, int lane)
+{
+    if (m <= 1) return;
+    if (m <= 64) {
+        int v = lane < m ? ids[b + lane] : 0x7fffffff;
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int u = __shfl_xor(v, j);
+                const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                v = (lower == up) ? min(v, u) : max(v, u);
+            }
+        }
+        if (lane < m) ids[b + lane] = v;
+        return;
+    }
+    if (lane != 0) return;
+    for (long p = b + 1; p < b + m; ++p) {
+        const int v = ids[p];
+        long q = p - 1;
+        while (q >= b && ids[q] > v) {
+            ids[q + 1] = ids[q];
+            --q;
+        }
+        ids[q + 1] = v;
+    }
+}
+
 // ---- cutoff function fc(r) = 0.5 (cos(pi r / rc) + 1) for r < rc, and dfc/dr ----
 
 __device__ __forceinline__ void fcut(double r, double rc, double &fc, double &dfc, Real<0>)

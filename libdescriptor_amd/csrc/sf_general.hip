@@ -24,15 +24,21 @@ __global__ __launch_bounds__(256) void k_fwd_general(SFTab tab, int n_centers,
                                                      const int *__restrict__ nl,
                                                      const int *__restrict__ species,
                                                      const double *__restrict__ x,
-                                                     double *__restrict__ desc)
+                                                     double *__restrict__ desc, DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dc.hdr) n_centers = dc.hdr[0];
     if (t >= n_centers) return;
+    if (dc.req_flag && dc.req_flag[dc.req_of[t]]) return;
     const int i = centers ? centers[t] : t;
     const int ns = tab.n_species;
     double *out = desc + (size_t)t * tab.width;
+    if (dc.out_row) {  // written (=): start from zero
+        out = dc.out_row[t];
+        for (int k = 0; k < tab.width; ++k) out[k] = 0.0;
+    }
     const int beg = offs[t], end = offs[t + 1];
     const int si = species[i];
     const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
@@ -119,15 +125,27 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
                                                      const double *__restrict__ x,
                                                      const double *__restrict__ dE,
                                                      const int *__restrict__ inv_pos,
-                                                     double *__restrict__ edge, int n_edges)
+                                                     double *__restrict__ edge, int n_edges,
+                                                     DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dc.hdr) {
+        n_centers = dc.hdr[0];
+        n_edges = dc.hdr[2];
+    }
     if (t >= n_centers) return;
+    if (dc.req_flag && dc.req_flag[dc.req_of[t]]) {
+        double *v = edge + 3 * (size_t)inv_pos[n_edges + t];
+        v[0] = 0.0;
+        v[1] = 0.0;
+        v[2] = 0.0;
+        return;
+    }
     const int i = centers ? centers[t] : t;
     const int ns = tab.n_species;
-    const double *g = dE + (size_t)t * tab.width;
+    const double *g = dc.in_row ? dc.in_row[t] : dE + (size_t)t * tab.width;
     const int beg = offs[t], end = offs[t + 1];
     for (int e = beg; e < end; ++e) {
         double *v = edge + 3 * (size_t)inv_pos[e];
@@ -253,32 +271,32 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
 
 hipError_t launch_fwd_general(const SFTab &tab, int precision, int n_centers, const int *centers,
                               const int *offs, const int *nl, const int *species, const double *x,
-                              double *desc, hipStream_t s)
+                              double *desc, hipStream_t s, DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
     dim3 block(256), grid((n_centers + 255) / 256);
     if (precision == 0)
         hipLaunchKernelGGL(k_fwd_general<0>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
-                           species, x, desc);
+                           species, x, desc, dc);
     else
         hipLaunchKernelGGL(k_fwd_general<1>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
-                           species, x, desc);
+                           species, x, desc, dc);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, const int *centers,
                               const int *offs, const int *nl, const int *species, const double *x,
                               const double *dE, const int *inv_pos, double *edge, int n_edges,
-                              hipStream_t s)
+                              hipStream_t s, DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
     dim3 block(256), grid((n_centers + 255) / 256);
     if (precision == 0)
         hipLaunchKernelGGL(k_adj_general<0>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
-                           species, x, dE, inv_pos, edge, n_edges);
+                           species, x, dE, inv_pos, edge, n_edges, dc);
     else
         hipLaunchKernelGGL(k_adj_general<1>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
-                           species, x, dE, inv_pos, edge, n_edges);
+                           species, x, dE, inv_pos, edge, n_edges, dc);
     return hipGetLastError();
 }
 

@@ -8,14 +8,32 @@
 
 namespace ldsf {
 
-// sf_general.hip: one thread per centre, any parameter set
+// Sizes of a launch read on the device, for launches replayed from a graph
+// with different batches (the serving loop): hdr[0] centres, hdr[1]
+// coordinate rows, hdr[2] neighbour-list edges.  With req_flag set, centres
+// whose request (req_of[c]) is flagged are skipped: they contribute nothing
+// and their edge records are left unwritten.  A default DevCounts (all null)
+// means the sizes passed as arguments.
+struct DevCounts {
+    const int *hdr = nullptr;
+    const int *req_of = nullptr;
+    const int *req_flag = nullptr;
+    int max_blocks = 0;  // wave kernels: cap on the persistent grid (0 = 16384)
+    // per centre: its descriptor row, written with = instead of += (null: desc + c * width)
+    double *const *out_row = nullptr;
+    // per centre: its dE/dzeta row (null: dE + c * width)
+    const double *const *in_row = nullptr;
+};
+
+// sf_general.hip: one thread per centre, any parameter set.  With dc.hdr set,
+// n_centers (and n_edges) give the grid's capacity; the launch covers hdr[0].
 hipError_t launch_fwd_general(const SFTab &tab, int precision, int n_centers, const int *centers,
                               const int *offs, const int *nl, const int *species, const double *x,
-                              double *desc, hipStream_t s);
+                              double *desc, hipStream_t s, DevCounts dc = DevCounts());
 hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, const int *centers,
                               const int *offs, const int *nl, const int *species, const double *x,
                               const double *dE, const int *inv_pos, double *edge, int n_edges,
-                              hipStream_t s);
+                              hipStream_t s, DevCounts dc = DevCounts());
 
 // sf_wave.hip: one wavefront per centre, neighbours staged in LDS.  They
 // cover parameter sets whose two-body rows are g2 (at most 8) and whose
@@ -56,13 +74,15 @@ size_t wave_lds_bytes(int max_nn, bool adjoint, int npb, int group);
 constexpr size_t kWaveMaxLds = 64 * 1024;
 hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
-                           double *desc, int width, int max_nn, int group, hipStream_t s);
+                           double *desc, int width, int max_nn, int group, hipStream_t s,
+                           DevCounts dc = DevCounts());
 // The adjoint kernels write the vector of edge e (and of centre c, as edge
 // n_edges + c) at record inv_pos[e] of `edge`, the order the gather reads.
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
-                           int width, int max_nn, int group, hipStream_t s);
+                           int width, int max_nn, int group, hipStream_t s,
+                           DevCounts dc = DevCounts());
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,

@@ -15,6 +15,9 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
+#include "sf_device.h"
 #include "sf_kernels.h"
 
 namespace ldsf {
@@ -74,21 +77,13 @@ __global__ void k_fill(long n_edges, int n_centers, const int *__restrict__ nl,
     in_edges[in_start[a] + pos] = (int)e;
 }
 
-// insertion sort of each row's bucket (buckets hold a few tens of ids)
+// each row's bucket (a few tens of ids) sorted by one wavefront
 __global__ void k_sort(int n_total, const int *__restrict__ in_start, int *in_edges)
 {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_total) return;
-    int b = in_start[a], e = in_start[a + 1];
-    for (int p = b + 1; p < e; ++p) {
-        int v = in_edges[p];
-        int q = p - 1;
-        while (q >= b && in_edges[q] > v) {
-            in_edges[q + 1] = in_edges[q];
-            --q;
-        }
-        in_edges[q + 1] = v;
-    }
+    const int lane = threadIdx.x & 63;
+    const long nw = ((long)gridDim.x * blockDim.x) >> 6;
+    for (long a = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; a < n_total; a += nw)
+        sort_bucket_wave(in_edges, in_start[a], in_start[a + 1] - in_start[a], lane);
 }
 
 __global__ void k_inverse(long n, const int *__restrict__ in_edges, int *__restrict__ inv_pos)
@@ -162,7 +157,9 @@ hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, 
 hipError_t launch_sort_buckets(int n_total, const int *in_start, int *in_edges, hipStream_t s)
 {
     if (n_total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start, in_edges);
+    const long blocks = std::min<long>(((long)n_total * 64 + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_sort, dim3((unsigned)blocks), dim3(kBlock), 0, s, n_total, in_start,
+                       in_edges);
     return hipGetLastError();
 }
 

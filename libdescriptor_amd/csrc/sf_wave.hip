@@ -122,14 +122,17 @@ __device__ __forceinline__ void group_reduce(double (&v)[K], int s)
     Halve<K, K / 2, G / 2>::run(v, s);
 }
 
+// every slot of a row is reached exactly once per centre, so `assign` may
+// store the sum instead of adding it
 template <int K, int G>
-__device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *row)
+__device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *row,
+                                           bool assign)
 {
     group_reduce<K, G>(v, s);
     constexpr int sh = Lg<G>::v - Log2<K>::v;
     if (row && (s & ((1 << sh) - 1)) == 0) {
         const int slot = slots[(s >> sh) & (K - 1)];
-        if (slot >= 0) row[slot] += v[0];
+        if (slot >= 0) row[slot] = assign ? v[0] : row[slot] + v[0];
     }
 }
 
@@ -466,7 +469,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ species,
                                                  const double *__restrict__ x,
                                                  double *__restrict__ desc, int width, int maxn,
-                                                 int tmax)
+                                                 int tmax, DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -479,6 +482,8 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     const int ns = wt.n_species;
+    const bool assign = dc.out_row != nullptr;
+    if (dc.hdr) n_centers = dc.hdr[0];
     const int n_tiles = (n_centers + C - 1) / C;
     const TileRange tr = tile_range(n_tiles);
     for (int tile = tr.first; tile < tr.hi; tile += tr.step) {
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
         int N = 0, beg = 0, si = 0;
         double xi0 = 0, xi1 = 0, xi2 = 0;
         double *row = nullptr;
-        if (c < n_centers) {
+        if (c < n_centers && !(dc.req_flag && dc.req_flag[dc.req_of[c]])) {
             const int i = centers ? centers[c] : c;
             beg = offs[c];
             N = offs[c + 1] - beg;
@@ -494,7 +499,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
             xi0 = x[3 * (size_t)i];
             xi1 = x[3 * (size_t)i + 1];
             xi2 = x[3 * (size_t)i + 2];
-            row = desc + (size_t)c * width;
+            row = dc.out_row ? dc.out_row[c] : desc + (size_t)c * width;
         }
         const int Nmax = tile_max<G>(N);
 
@@ -548,7 +553,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
             }
             M += __popcll(m);
         }
-        reduce_add<NSB * 8, G>(acc2, s, wp.slots, row);
+        reduce_add<NSB * 8, G>(acc2, s, wp.slots, row, assign);
         __syncthreads();
 
         // ---- triplets ----
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                         }
                     }
                 }
-                reduce_add<16, G>(acc3, s, wp.slots + 16 + p * 16, row);
+                reduce_add<16, G>(acc3, s, wp.slots + 16 + p * 16, row, assign);
                 if (NPB > 1) __syncthreads();
             }
         }
@@ -611,7 +616,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const double *__restrict__ dE,
                                                  const int *__restrict__ inv_pos,
                                                  double *__restrict__ edge, int n_edges, int width,
-                                                 int maxn, int tmax)
+                                                 int maxn, int tmax, DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -624,6 +629,10 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     const int ns = wt.n_species;
+    if (dc.hdr) {
+        n_centers = dc.hdr[0];
+        n_edges = dc.hdr[2];
+    }
     const int n_tiles = (n_centers + C - 1) / C;
     const TileRange tr = tile_range(n_tiles);
     for (int tile = tr.first; tile < tr.hi; tile += tr.step) {
@@ -631,8 +640,11 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         const bool live = c < n_centers;
         int N = 0, beg = 0, si = 0;
         double xi0 = 0, xi1 = 0, xi2 = 0;
-        const double *gv = dE;
-        if (live) {
+        const double *gv = nullptr;  // dE/dzeta row, read only when `work`
+        bool work = false;
+        // a skipped centre (flagged request) writes only its zero self record
+        if (live && !(dc.req_flag && dc.req_flag[dc.req_of[c]])) {
+            work = true;
             const int i = centers ? centers[c] : c;
             beg = offs[c];
             N = offs[c + 1] - beg;
@@ -640,7 +652,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
             xi0 = x[3 * (size_t)i];
             xi1 = x[3 * (size_t)i + 1];
             xi2 = x[3 * (size_t)i + 2];
-            gv = dE + (size_t)c * width;
+            gv = dc.in_row ? dc.in_row[c] : dE + (size_t)c * width;
         }
         const int Nmax = tile_max<G>(N);
 
@@ -717,7 +729,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 if (s < 16) {
                     const int z = s >> 2, l = (s >> 1) & 1;
                     const int slot = wp.slots[16 + p * 16 + s];
-                    const double ww = (live && slot >= 0) ? gv[slot] * wp.coef[z] : 0.0;
+                    const double ww = (work && slot >= 0) ? gv[slot] * wp.coef[z] : 0.0;
                     L.w[s] = ww;
                     L.wv[s] = ww * wp.zeta[z] * wp.lam[l];
                 }
@@ -807,9 +819,9 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     }
 }
 
-int grid_for(int n_tiles)
+int grid_for(int n_tiles, int cap)
 {
-    int g = std::min(n_tiles, 16384);
+    int g = std::min(n_tiles, cap > 0 ? cap : 16384);
     if (g >= 8) g = (g + 7) & ~7;
     return std::max(g, 1);
 }
@@ -913,16 +925,18 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
 
 hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
-                           double *desc, int width, int max_nn, int group, hipStream_t s)
+                           double *desc, int width, int max_nn, int group, hipStream_t s,
+                           DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
     const size_t lds = lds_for(max_nn, false, wt.npb, group);
-    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group))), block(kW);
+    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group), dc.max_blocks)),
+        block(kW);
 #define LDSF_FWD(P, NSB, NPB, G)                                                                  \
     hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, G>), grid, block, lds, s, wt, n_centers, centers, \
-                       offs, nl, species, x, desc, width, maxn, tmax)
+                       offs, nl, species, x, desc, width, maxn, tmax, dc)
 #define LDSF_FWD_G(P, NSB, NPB)                                                                   \
     do {                                                                                          \
         if (group == 16) LDSF_FWD(P, NSB, NPB, 16);                                               \
@@ -942,16 +956,17 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
-                           int width, int max_nn, int group, hipStream_t s)
+                           int width, int max_nn, int group, hipStream_t s, DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
     const size_t lds = lds_for(max_nn, true, wt.npb, group);
-    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group))), block(kW);
+    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group), dc.max_blocks)),
+        block(kW);
 #define LDSF_ADJ(P, NSB, NPB, G)                                                                  \
     hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, G>), grid, block, lds, s, wt, n_centers, centers, \
-                       offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, tmax)
+                       offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, tmax, dc)
 #define LDSF_ADJ_G(P, NSB, NPB)                                                                   \
     do {                                                                                          \
         if (group == 16) LDSF_ADJ(P, NSB, NPB, 16);                                               \

@@ -172,3 +172,28 @@ def config_c4(n_configs=512, seed=0, nmin=8, nmax=4096):
             pos, cell, pbc, sp = cluster(int(n), rho, 1.8, seed=s)
         layouts.append(layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"]))
     return layouts
+
+
+def config_c5_pool(n_pool=192, seed=0, nmin=16, nmax=20000, cache_dir=None):
+    """Configurations a serving request stream draws from: atom counts
+    log-uniform in [nmin, nmax], each generated like config_c3 (random
+    periodic, 0.05 atoms/A^3, 1.8 A minimum distance, species 80/20,
+    cutoff 5.0 A).  Cached like config_c3."""
+    import os
+    cache_dir = cache_dir or os.environ.get("LDSF_WORKLOAD_CACHE")
+    path = os.path.join(cache_dir, f"c5_{n_pool}_{seed}.npz") if cache_dir else None
+    if path and os.path.exists(path):
+        z = np.load(path, allow_pickle=False)
+        return [{k: (int(z[f"{c}_{k}"]) if k == "n_contrib" else z[f"{c}_{k}"])
+                 for k in ("coords", "species", "numneigh", "neighlist", "image", "n_contrib")}
+                for c in range(n_pool)]
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_pool)).astype(int)
+    pool = []
+    for n in sizes:
+        pos, cell, pbc, sp = random_system(int(n), 0.05, 1.8, seed=int(rng.integers(1 << 30)))
+        pool.append(layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"]))
+    if path:
+        os.makedirs(cache_dir, exist_ok=True)
+        np.savez(path, **{f"{c}_{k}": v for c, L in enumerate(pool) for k, v in L.items()})
+    return pool
