@@ -74,11 +74,14 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched (512 configurations) and serving measurements")
     ap.add_argument("--serving-requests", type=int, default=2000)
-    ap.add_argument("--serving-lanes", type=int, default=4)
-    ap.add_argument("--serving-lane-centers", type=int, default=49152)
+    ap.add_argument("--serving-lanes", type=int, default=3)
+    ap.add_argument("--serving-lane-centers", type=int, default=65536)
     ap.add_argument("--serving-min-centers", type=int, default=0)
+    ap.add_argument("--serving-window-us", type=int, default=1000)
     ap.add_argument("--serving-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the offline request rate")
+    ap.add_argument("--serving-compare", type=int, default=1,
+                    help="also run the immediate-dispatch policy (0 to skip)")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--serving-dump", default=None, metavar="NPZ",
                     help="write request sizes, arrival times and latencies to NPZ")
@@ -305,7 +308,9 @@ def bench_serving(args):
     dE/dzeta, FP32-bulk.  Inputs and outputs are device-resident; latency is
     submission to completion.  Offline throughput: every request submitted at
     once.  Online: Poisson arrivals at --serving-load (default 80 %) of the
-    offline request rate."""
+    offline request rate of the same server configuration.  The headline
+    policy forms batches over a 1 ms window on 3 lanes; the immediate-dispatch
+    policy (4 lanes, no window) is measured beside it."""
     import threading
 
     import torch
@@ -333,68 +338,72 @@ def bench_serving(args):
              torch.empty_like(P[k]["x"])) for k in pick]
     torch.cuda.synchronize()
     atoms = int(sum(P[k]["n"] for k in pick))
-    lc = args.serving_lane_centers
-    srv = _capi.Server(ds._handle, _capi.FP32_BULK, hbm_budget_bytes=8 << 30,
-                       lane_centers=lc, lane_rows=max(3 * lc, 65536), lane_edges=lc * 48,
-                       max_requests=256, max_lanes=args.serving_lanes,
-                       dispatch_min_centers=args.serving_min_centers)
-    lanes = srv.num_lanes
 
-    def submit(i):
-        D = P[pick[i]]
-        return srv.submit(D["species"], D["numneigh"], D["nl"], D["x"], D["dE"], outs[i][0],
-                          outs[i][1], D["max_nn"])
+    def run(lanes, lc, min_centers, window_us, load, dump=None):
+        srv = _capi.Server(ds._handle, _capi.FP32_BULK, hbm_budget_bytes=8 << 30,
+                           lane_centers=lc, lane_rows=max(3 * lc, 65536), lane_edges=lc * 48,
+                           max_requests=256, max_lanes=lanes, dispatch_min_centers=min_centers,
+                           batch_window_us=window_us)
 
-    # warm-up: capture the lanes' graphs for the buckets in use
-    for tk in [submit(i) for i in range(min(nreq, 64))]:
-        srv.wait(tk)
-    # offline: everything queued at once
-    t1 = time.perf_counter()
-    tickets = [submit(i) for i in range(nreq)]
-    for tk in tickets:
-        srv.wait(tk)
-    off_s = time.perf_counter() - t1
-    off_rate = atoms / off_s
-    log(f"serving offline: {nreq} requests, {atoms} atoms in {off_s * 1e3:.1f} ms "
-        f"({off_rate:.3e} atom-envs/s)")
-    # online: Poisson arrivals at a fraction of the offline request rate
-    lam = args.serving_load * nreq / off_s
-    arrivals = np.cumsum(np.random.default_rng(6).exponential(1.0 / lam, size=nreq))
-    lat = np.zeros(nreq)
-    tick = [None] * nreq
-    done = threading.Event()
+        def submit(i):
+            D = P[pick[i]]
+            return srv.submit(D["species"], D["numneigh"], D["nl"], D["x"], D["dE"],
+                              outs[i][0], outs[i][1], D["max_nn"])
 
-    def waiter():
+        for tk in [submit(i) for i in range(min(nreq, 64))]:  # warm-up
+            srv.wait(tk)
+        # offline: everything queued at once
+        t1 = time.perf_counter()
+        tickets = [submit(i) for i in range(nreq)]
+        for tk in tickets:
+            srv.wait(tk)
+        off_s = time.perf_counter() - t1
+        off_rate = atoms / off_s
+        # online: Poisson arrivals at `load` of the offline request rate
+        lam = load * nreq / off_s
+        arrivals = np.cumsum(np.random.default_rng(6).exponential(1.0 / lam, size=nreq))
+        lat = np.zeros(nreq)
+        tick = [None] * nreq
+
+        def waiter():
+            for i in range(nreq):
+                while tick[i] is None:
+                    time.sleep(0)
+                lat[i] = srv.wait(tick[i])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t2 = time.perf_counter()
         for i in range(nreq):
-            while tick[i] is None:
-                time.sleep(0)
-            lat[i] = srv.wait(tick[i])
-        done.set()
+            while time.perf_counter() - t2 < arrivals[i]:
+                pass
+            tick[i] = submit(i)
+        th.join()
+        on_s = time.perf_counter() - t2
+        st = srv.stats()
+        n_lanes = srv.num_lanes
+        srv.close()
+        if dump:
+            np.savez(dump, atoms=np.array([P[k]["n"] for k in pick]), arrival=arrivals,
+                     latency=lat)
+        p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+        log(f"serving lanes={n_lanes} window={window_us}us: offline {off_rate:.3e}, online "
+            f"{atoms / on_s:.3e} atom-envs/s, p50 {p50 * 1e3:.3f} ms, p99 {p99 * 1e3:.3f} ms")
+        return {"offline_atom_envs_per_s": off_rate, "online_atom_envs_per_s": atoms / on_s,
+                "online_fraction_of_offline": (atoms / on_s) / off_rate,
+                "arrival_rate_per_s": lam, "p50_ms": p50 * 1e3, "p99_ms": p99 * 1e3,
+                "p99_over_p50": p99 / p50, "lanes": n_lanes, "lane_centers": lc,
+                "dispatch_min_centers": min_centers, "batch_window_us": window_us,
+                "server_stats": st}
 
-    th = threading.Thread(target=waiter)
-    th.start()
-    t2 = time.perf_counter()
-    for i in range(nreq):
-        while time.perf_counter() - t2 < arrivals[i]:
-            pass
-        tick[i] = submit(i)
-    th.join()
-    on_s = time.perf_counter() - t2
-    st = srv.stats()
-    srv.close()
-    if args.serving_dump:
-        np.savez(args.serving_dump, atoms=np.array([P[k]["n"] for k in pick]), arrival=arrivals,
-                 latency=lat)
-    p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
-    log(f"serving online: {atoms / on_s:.3e} atom-envs/s, p50 {p50 * 1e3:.3f} ms, "
-        f"p99 {p99 * 1e3:.3f} ms")
-    return {"requests": nreq, "atoms": atoms, "pool_configurations": len(P),
-            "offline_atom_envs_per_s": off_rate,
-            "online_atom_envs_per_s": atoms / on_s,
-            "online_fraction_of_offline": (atoms / on_s) / off_rate,
-            "arrival_rate_per_s": lam, "p50_ms": p50 * 1e3, "p99_ms": p99 * 1e3,
-            "p99_over_p50": p99 / p50, "lanes": lanes, "lane_centers": lc,
-            "dispatch_min_centers": args.serving_min_centers, "server_stats": st}
+    out = {"requests": nreq, "atoms": atoms, "pool_configurations": len(P)}
+    # headline policy: batches formed over a window, several lanes in flight
+    out.update(run(args.serving_lanes, args.serving_lane_centers, args.serving_min_centers,
+                   args.serving_window_us, args.serving_load, args.serving_dump))
+    if args.serving_compare:
+        # dispatch as soon as a lane is free: lower median latency, wider tail
+        out["immediate_dispatch"] = run(4, 49152, 0, 0, args.serving_load)
+    return out
 
 
 def measured_traffic(kernel, n):

@@ -207,9 +207,10 @@ typedef struct {
     int lane_edges;           /* neighbour-list entries per batch */
     int max_requests;         /* requests per batch (<= 65535) */
     int max_lanes;            /* upper bound on concurrent batches */
-    int dispatch_min_centers; /* while a batch runs, a free lane starts only once this many
-                                 centres are queued (0: at once); an idle device always
-                                 starts at once */
+    int dispatch_min_centers; /* > 0: a free lane starts once this many centres are queued */
+    int batch_window_us;      /* > 0: ... or once the oldest queued request has waited this
+                                 long (microseconds).  With no window, a free lane starts at
+                                 once (no threshold) or when the device is idle */
 } ldsf_server_config;
 
 typedef struct {

@@ -75,7 +75,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_size_t),
                 ("lane_centers", ctypes.c_int), ("lane_rows", ctypes.c_int),
                 ("lane_edges", ctypes.c_int), ("max_requests", ctypes.c_int),
-                ("max_lanes", ctypes.c_int), ("dispatch_min_centers", ctypes.c_int)]
+                ("max_lanes", ctypes.c_int), ("dispatch_min_centers", ctypes.c_int),
+                ("batch_window_us", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -222,10 +223,10 @@ class Server:
 
     def __init__(self, sf_handle, precision=FP32_BULK, hbm_budget_bytes=2 << 30,
                  lane_centers=65536, lane_rows=196608, lane_edges=65536 * 48, max_requests=256,
-                 max_lanes=4, dispatch_min_centers=0):
+                 max_lanes=4, dispatch_min_centers=0, batch_window_us=0):
         cfg = ServerConfig(int(precision), int(hbm_budget_bytes), int(lane_centers),
                            int(lane_rows), int(lane_edges), int(max_requests), int(max_lanes),
-                           int(dispatch_min_centers))
+                           int(dispatch_min_centers), int(batch_window_us))
         out = ctypes.c_void_p()
         check(lib().ldsf_server_create(ctypes.byref(out), sf_handle, ctypes.byref(cfg)))
         self.handle = out.value

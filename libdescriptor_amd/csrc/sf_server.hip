@@ -580,20 +580,34 @@ void worker_main(ldsf_server *S)
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
         ldsf_lane *L = nullptr;
-        // A free lane takes the queue at once when the device is idle; while
-        // other batches run it waits until the queue holds dispatch_min_centers
-        // centres (or the server is stopping), so that batches grow with load.
-        S->cv_work.wait(lk, [&] {
-            if (S->queue.empty()) return S->stop;
-            int running = 0;
+        // A free lane takes the queue when (a) the queue holds
+        // dispatch_min_centers (> 0) centres, (b) its oldest request has
+        // waited batch_window_us (> 0), (c) no window is set and either no
+        // threshold is set or the device is idle, or (d) the server is
+        // stopping -- so that batches grow with the load.
+        const double window = 1e-6 * S->cfg.batch_window_us;
+        for (;;) {
+            if (S->queue.empty() && S->stop) break;
             bool free_lane = false;
+            int running = 0;
             for (auto &l : S->lanes) {
                 if (l.busy) ++running;
                 else free_lane = true;
             }
-            return free_lane && (running == 0 || S->stop ||
-                                 S->queued_centers >= (long long)S->cfg.dispatch_min_centers);
-        });
+            if (!S->queue.empty() && free_lane) {
+                const double age = ldsf::now_s() - S->queue.front()->t_submit;
+                const int min_c = S->cfg.dispatch_min_centers;
+                if (S->stop || (min_c > 0 && S->queued_centers >= (long long)min_c) ||
+                    (window > 0 && age >= window) ||
+                    (window <= 0 && (running == 0 || min_c == 0)))
+                    break;
+                if (window > 0) {
+                    S->cv_work.wait_for(lk, std::chrono::duration<double>(window - age));
+                    continue;
+                }
+            }
+            S->cv_work.wait(lk);
+        }
         if (S->queue.empty() && S->stop) break;
         for (auto &l : S->lanes)
             if (!l.busy) {
@@ -700,7 +714,8 @@ LDSF_API int ldsf_server_create(ldsf_server **out, const ldsf_sf *sf,
     const ldsf_server_config &c = *config;
     if ((c.precision != LDSF_FP64 && c.precision != LDSF_FP32_BULK) || c.lane_centers < 1 ||
         c.lane_rows < c.lane_centers || c.lane_edges < 0 || c.max_requests < 1 ||
-        c.max_requests > 65535 || c.max_lanes < 1 || c.dispatch_min_centers < 0 || (long)c.lane_edges + c.lane_centers > 0x7fffffffL ||
+        c.max_requests > 65535 || c.max_lanes < 1 || c.dispatch_min_centers < 0 ||
+        c.batch_window_us < 0 || (long)c.lane_edges + c.lane_centers > 0x7fffffffL ||
         (long)c.lane_rows * 3 > 0x7fffffffL)
         return ldsf::fail(LDSF_ERR_ARG, "server_create: bad configuration");
     auto *S = new ldsf_server();
