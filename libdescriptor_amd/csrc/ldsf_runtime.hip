@@ -259,7 +259,7 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     ldsf::WaveParams wpar;
     int group = 0;
     if (ldsf::wave_build(sf, &plan->wave, &wpar) &&
-        (group = ldsf::wave_group_size(plan->max_nn, plan->wave.npb)) > 0) {
+        (group = ldsf::wave_group_size(plan->max_nn, plan->wave.n_species)) > 0) {
         plan->wave_group = group;
         HIP_TRY(hipMalloc((void **)&plan->wave_par, sizeof(wpar)), "hipMalloc(wave parameters)");
         HIP_TRY(hipMemcpy(plan->wave_par, &wpar, sizeof(wpar), hipMemcpyHostToDevice),

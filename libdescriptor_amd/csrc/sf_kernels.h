@@ -67,10 +67,10 @@ constexpr int kMaxWaveSpecies = 8;
 // parameter set
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *par);
 // Lanes per centre (16, 32 or 64) for the largest neighbour count and number
-// of species-pair blocks; 0 if the wave kernels cannot run (more than 64
-// neighbours, or too much LDS).
-int wave_group_size(int max_nn, int npb);
-size_t wave_lds_bytes(int max_nn, bool adjoint, int npb, int group);
+// of species; 0 if the wave kernels cannot run (more than 64 neighbours, or
+// too much LDS).
+int wave_group_size(int max_nn, int n_species);
+size_t wave_lds_bytes(int max_nn, bool adjoint, int n_species, int group);
 constexpr size_t kWaveMaxLds = 64 * 1024;
 hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,

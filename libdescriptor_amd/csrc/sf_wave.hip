@@ -44,7 +44,6 @@ namespace ldsf {
 namespace {
 
 constexpr int kW = 64;
-constexpr int kTab = kMaxWaveSpecies * kMaxWaveSpecies;
 
 template <int G>
 struct Lg;
@@ -82,18 +81,70 @@ __device__ __forceinline__ int tile_max(int v)
     return v;
 }
 
+// Cross-lane exchanges of the reduction butterfly, for partner bit M.
+// M = 32, 16: v_permlane32_swap / v_permlane16_swap exchange the two halves
+// of a pair of registers between lane halves (rows), so that after
+// xswap<M>(a, b) every lane's a + b is its own kept value plus the partner's
+// (lanes with bit M clear keep a, lanes with it set keep b).
+template <int M>
+__device__ __forceinline__ void xswap(double &a, double &b)
+{
+    static_assert(M == 32 || M == 16, "permlane swaps exchange 32- or 16-lane halves");
+    const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+    const unsigned al = (unsigned)ua, ah = (unsigned)(ua >> 32);
+    const unsigned bl = (unsigned)ub, bh = (unsigned)(ub >> 32);
+    unsigned l0, l1, h0, h1;
+    if constexpr (M == 32) {
+        const auto rl = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+        const auto rh = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+        l0 = rl[0]; l1 = rl[1]; h0 = rh[0]; h1 = rh[1];
+    } else {
+        const auto rl = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+        const auto rh = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+        l0 = rl[0]; l1 = rl[1]; h0 = rh[0]; h1 = rh[1];
+    }
+    a = __longlong_as_double((long long)(((unsigned long long)h0 << 32) | l0));
+    b = __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
+}
+
+// M <= 8: the partner lane's value through DPP: lane ^ 1 and lane ^ 2 (quad
+// permutations), the mirror lane in its 8-lane half row (differs in bit 2)
+// and in its 16-lane row (differs in bit 3).
+template <int M>
+__device__ __forceinline__ double dpp_partner(double v)
+{
+    constexpr int ctrl = M == 1 ? 0xB1 : M == 2 ? 0x4E : M == 4 ? 0x141 : 0x140;
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8, "DPP partner within a row");
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned)lo));
+}
+
 // Recursive-halving reduce-scatter of K values over the G lanes of a group:
-// step with H live values and partner lane ^ M, all compile-time.
+// step with H live values and partner bit M, all compile-time.  Each step
+// pairs lanes that differ in bit M and agree in the bits already reduced, so
+// every lane's partial sums stay disjoint.
 template <int K, int H, int M>
 struct Halve {
     static __device__ __forceinline__ void run(double (&v)[K], int s)
     {
-        const bool hi = (s & M) != 0;
+        if constexpr (M >= 16) {
 #pragma unroll
-        for (int q = 0; q < H; ++q) {
-            const double send = hi ? v[q] : v[q + H];
-            const double keep = hi ? v[q + H] : v[q];
-            v[q] = keep + __shfl_xor(send, M);
+            for (int q = 0; q < H; ++q) {
+                double a = v[q], b = v[q + H];
+                xswap<M>(a, b);
+                v[q] = a + b;
+            }
+        } else {
+            const bool hi = (s & M) != 0;
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                const double send = hi ? v[q] : v[q + H];
+                const double keep = hi ? v[q + H] : v[q];
+                v[q] = keep + dpp_partner<M>(send);
+            }
         }
         Halve<K, H / 2, M / 2>::run(v, s);
     }
@@ -103,7 +154,13 @@ template <int K, int M>
 struct Halve<K, 0, M> {
     static __device__ __forceinline__ void run(double (&v)[K], int)
     {
-        v[0] += __shfl_xor(v[0], M);
+        if constexpr (M >= 16) {
+            double a = v[0], b = v[0];
+            xswap<M>(a, b);
+            v[0] = a + b;
+        } else {
+            v[0] += dpp_partner<M>(v[0]);
+        }
         Halve<K, 0, M / 2>::run(v, 0);
     }
 };
@@ -136,12 +193,16 @@ __device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slo
     }
 }
 
-__device__ __forceinline__ void diag_norm(int &d, int &a, int M)
+// Pair t of the column-major upper triangle, t = b (b - 1) / 2 + a with
+// a < b: b from the square root, corrected by one where the float estimate
+// lands on the wrong side of a column boundary.
+__device__ __forceinline__ void pair_of(int t, int &a, int &b)
 {
-    while (d < M && a >= M - d) {
-        a -= M - d;
-        ++d;
-    }
+    int c = (int)((1.0f + __builtin_sqrtf((float)(8 * t + 1))) * 0.5f);
+    c -= (c * (c - 1) / 2 > t) ? 1 : 0;
+    c += ((c + 1) * c / 2 <= t) ? 1 : 0;
+    b = c;
+    a = t - c * (c - 1) / 2;
 }
 
 // tiles of this block: blocks are dealt to the 8 XCDs round-robin (block b
@@ -207,12 +268,58 @@ __device__ __forceinline__ T power_z(T b, const T (&sq)[6], int z, const WavePar
     return b > (T)0 ? v : (T)0;
 }
 
+// B^zeta_z for the nz distinct zetas of the grid (0 for B <= 0).  FP64: the
+// squaring table (exact products for integer zeta).  FP32-bulk:
+// exp2(zeta log2 B), one v_log_f32 per base and one v_exp_f32 per zeta; the
+// relative error, ~zeta |log2 B| 2^-24, is below 1e-6 wherever the term is not
+// negligible against the block (B >= 2^-2 for zeta = 16), like the zeta-fold
+// growth of B's own FP32 rounding that any evaluation carries.
+template <int P>
+__device__ __forceinline__ void powers(typename Real<P>::T b, int nsq, const WaveParams &wp,
+                                       int nz, typename Real<P>::T (&pw)[4])
+{
+    using T = typename Real<P>::T;
+    if constexpr (P == 0) {
+        T sq[6];
+        squarings(b, nsq, sq);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            if (z < nz) pw[z] = power_z<T>(b, sq, z, wp, false);
+    } else {
+        const float lb = __builtin_amdgcn_logf(b);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            if (z < nz) pw[z] = b > 0.0f ? __builtin_amdgcn_exp2f((float)wp.zeta[z] * lb) : 0.0f;
+    }
+}
+
+// B^(zeta_z - 1), the same way
+template <int P>
+__device__ __forceinline__ void powers_m1(typename Real<P>::T b, int nsq, const WaveParams &wp,
+                                          int nz, typename Real<P>::T (&pm)[4])
+{
+    using T = typename Real<P>::T;
+    if constexpr (P == 0) {
+        T sq[6];
+        squarings(b, nsq, sq);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            if (z < nz) pm[z] = power_z<T>(b, sq, z, wp, true);
+    } else {
+        const float lb = __builtin_amdgcn_logf(b);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            if (z < nz)
+                pm[z] = b > 0.0f ? __builtin_amdgcn_exp2f(((float)wp.zeta[z] - 1.0f) * lb) : 0.0f;
+    }
+}
+
 // ---- LDS ----
 
 // per-wave tables: staged parameters and species-pair cutoff data
 struct Shared {
     WaveParams *par;
-    double *rc, *rcinv, *rc2, *rcs;  // rc, 1/rc, rc^2, (pi/2)/rc per species pair
+    double *rc, *rcinv, *rc2, *rcs;  // rc, 1/rc, rc^2, (pi/2)/rc per species pair (ns^2 each)
 };
 
 // one group's arrays, all sized for maxn neighbours (lists for tmax pairs)
@@ -220,36 +327,39 @@ struct Grp {
     double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
     double *dfc, *fx, *fy, *fz, *w, *wv;  // adjoint only
     int *sp, *orig;
-    unsigned short *l0, *l1;
+    unsigned short *l0;  // active pairs, one region per species-pair block
 };
 
-constexpr size_t shared_doubles() { return (sizeof(WaveParams) + 7) / 8 + 4 * kTab; }
+__host__ __device__ constexpr size_t shared_doubles(int ns)
+{
+    return (sizeof(WaveParams) + 7) / 8 + 4 * (size_t)ns * ns;
+}
 
-__host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint, int npb)
+__host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint)
 {
     const size_t nd = adjoint ? 11 : 7;
     size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 : 0)) + sizeof(int) * 2 * maxn +
-               sizeof(unsigned short) * tmax * (npb > 1 ? 2 : 1);
+               sizeof(unsigned short) * tmax;
     return (b + 15) & ~size_t(15);
 }
 
-__device__ __forceinline__ Shared carve_shared(double *base)
+__device__ __forceinline__ Shared carve_shared(double *base, int ns)
 {
     Shared S;
     S.par = reinterpret_cast<WaveParams *>(base);
     double *p = base + (sizeof(WaveParams) + 7) / 8;
-    S.rc = p; p += kTab;
-    S.rcinv = p; p += kTab;
-    S.rc2 = p; p += kTab;
+    S.rc = p; p += ns * ns;
+    S.rcinv = p; p += ns * ns;
+    S.rc2 = p; p += ns * ns;
     S.rcs = p;
     return S;
 }
 
-__device__ __forceinline__ Grp carve_group(double *base, int g, int maxn, int tmax, bool adjoint,
-                                           int npb)
+__device__ __forceinline__ Grp carve_group(double *base, int ns, int g, int maxn, int tmax,
+                                           bool adjoint)
 {
-    char *q = reinterpret_cast<char *>(base + shared_doubles()) +
-              (size_t)g * group_bytes(maxn, tmax, adjoint, npb);
+    char *q = reinterpret_cast<char *>(base + shared_doubles(ns)) +
+              (size_t)g * group_bytes(maxn, tmax, adjoint);
     double *p = reinterpret_cast<double *>(q);
     Grp L;
     L.px = p; p += maxn;
@@ -273,7 +383,6 @@ __device__ __forceinline__ Grp carve_group(double *base, int g, int maxn, int tm
     L.sp = ip; ip += maxn;
     L.orig = ip; ip += maxn;
     L.l0 = reinterpret_cast<unsigned short *>(ip);
-    L.l1 = L.l0 + tmax;
     return L;
 }
 
@@ -314,60 +423,58 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
     dfc = -(float)(2.0 * rcs) * s * c;
 }
 
-// Phase 2: compact the active pairs of the group's neighbours into L.l0 in
-// diagonal order; returns the group's count (codes: a | b << 6 | block << 12).
+// Phase 2: the active pairs (a < b, r_jk < rc_jk) of the group's M compacted
+// neighbours, walked in column-major order, compacted into one region of
+// L.l0 per species-pair block (codes a | b << 6).  With two species and n1
+// neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
+// n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
+// n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.
 template <int P, int NPB, int G>
-__device__ __forceinline__ int enumerate_pairs(const Shared &S, const Grp &L, int ns, int M, int g,
-                                               int s)
+__device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, int ns, int M,
+                                                int n1, int g, int s, int (&cnt)[3],
+                                                int (&base)[3])
 {
+    const int n0 = M - n1;
+    base[0] = 0;
+    base[1] = n0 * (n0 - 1) / 2;
+    base[2] = base[1] + n0 * n1;
+    cnt[0] = cnt[1] = cnt[2] = 0;
     const int T = M * (M - 1) / 2;
     const int Tmax = tile_max<G>(T);
-    int cnt = 0;
-    int d = 1, a = s;
-    diag_norm(d, a, M);
     for (int t0 = 0; t0 < Tmax; t0 += G) {
+        const int t = t0 + s;
         bool act = false;
-        int code = 0;
-        if (d < M) {
-            const int b = a + d;
+        int code = 0, blk = 0;
+        if (t < T) {
+            int a, b;
+            pair_of(t, a, b);
             const double rjk2 = dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
             const int sa = L.sp[a], sb = L.sp[b];
             // the reference tests sqrt(r_jk^2) < rc_jk; the squared test differs
             // only within an ulp of rc_jk, where fc_jk vanishes quadratically
             act = (P == 0) ? (sqrt(rjk2) < S.rc[sa * ns + sb]) : (rjk2 < S.rc2[sa * ns + sb]);
-            code = a | (b << 6) | ((NPB > 1 ? pair_index(ns, sa, sb) : 0) << 12);
+            code = a | (b << 6);
+            if (NPB > 1) blk = pair_index(ns, sa, sb);
         }
-        const unsigned long long m = group_bits<G>(__ballot(act), g);
-        if (act) L.l0[cnt + below(m, s)] = (unsigned short)code;
-        cnt += __popcll(m);
-        a += G;
-        diag_norm(d, a, M);
+        if (NPB == 1) {
+            const unsigned long long m = group_bits<G>(__ballot(act), g);
+            if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
+            cnt[0] += __popcll(m);
+        } else {
+            const unsigned long long m0 = group_bits<G>(__ballot(act && blk == 0), g);
+            const unsigned long long m1 = group_bits<G>(__ballot(act && blk == 1), g);
+            const unsigned long long m2 = group_bits<G>(__ballot(act && blk == 2), g);
+            if (act) {
+                const int pos = blk == 0 ? base[0] + cnt[0] + below(m0, s)
+                              : blk == 1 ? base[1] + cnt[1] + below(m1, s)
+                                         : base[2] + cnt[2] + below(m2, s);
+                L.l0[pos] = (unsigned short)code;
+            }
+            cnt[0] += __popcll(m0);
+            cnt[1] += __popcll(m1);
+            cnt[2] += __popcll(m2);
+        }
     }
-    return cnt;
-}
-
-// the active pairs of block p: L.l0 itself, or a compacted copy in L.l1
-template <int NPB, int G>
-__device__ __forceinline__ const unsigned short *block_list(const Grp &L, int cnt, int p, int g,
-                                                            int s, int &nb)
-{
-    if (NPB == 1) {
-        nb = cnt;
-        return L.l0;
-    }
-    const int cmax = tile_max<G>(cnt);
-    int k = 0;
-    for (int t0 = 0; t0 < cmax; t0 += G) {
-        const int t = t0 + s;
-        const int code = t < cnt ? L.l0[t] : 0;
-        const bool mine = t < cnt && (code >> 12) == p;
-        const unsigned long long m = group_bits<G>(__ballot(mine), g);
-        if (mine) L.l1[k + below(m, s)] = (unsigned short)code;
-        k += __popcll(m);
-    }
-    __syncthreads();
-    nb = k;
-    return L.l1;
 }
 
 // shared quantities of one active triplet (a, b) of a group
@@ -476,12 +583,12 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
     constexpr int C = kW / G;
     extern __shared__ double lds_base[];
     const int lane = threadIdx.x, g = lane / G, s = lane % G;
-    const Shared S = carve_shared(lds_base);
-    const Grp L = carve_group(lds_base, g, maxn, tmax, false, NPB);
+    const int ns = wt.n_species;
+    const Shared S = carve_shared(lds_base, ns);
+    const Grp L = carve_group(lds_base, ns, g, maxn, tmax, false);
     stage_shared(S, wt, lane);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
-    const int ns = wt.n_species;
     const bool assign = dc.out_row != nullptr;
     if (dc.hdr) n_centers = dc.hdr[0];
     const int n_tiles = (n_centers + C - 1) / C;
@@ -507,7 +614,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
         double acc2[NSB * 8];
 #pragma unroll
         for (int k = 0; k < NSB * 8; ++k) acc2[k] = 0.0;
-        int M = 0;
+        int M = 0, n1 = 0;
         for (int j0 = 0; j0 < Nmax; j0 += G) {
             const int jj = j0 + s;
             bool valid = false;
@@ -552,17 +659,20 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                 }
             }
             M += __popcll(m);
+            if (NPB > 1) n1 += __popcll(group_bits<G>(__ballot(valid && sj == 1), g));
         }
         reduce_add<NSB * 8, G>(acc2, s, wp.slots, row, assign);
         __syncthreads();
 
         // ---- triplets ----
         if (wt.has3) {
-            const int cnt = enumerate_pairs<P, NPB, G>(S, L, ns, M, g, s);
+            int cnt[3], base[3];
+            enumerate_pairs<P, NPB, G>(S, L, ns, M, n1, g, s, cnt, base);
             __syncthreads();
+#pragma unroll
             for (int p = 0; p < NPB; ++p) {
-                int nb = 0;
-                const unsigned short *lst = block_list<NPB, G>(L, cnt, p, g, s, nb);
+                const int nb = cnt[p];
+                const unsigned short *lst = L.l0 + base[p];
                 const int nbmax = tile_max<G>(nb);
                 double acc3[16];
 #pragma unroll
@@ -580,13 +690,12 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
                             if (l < wt.nl) {
-                                T sq[6];
-                                squarings(q.B[l], nsq, sq);
+                                T pw[4];
+                                powers<P>(q.B[l], nsq, wp, wt.nz, pw);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
                                     if (z < wt.nz) {
-                                        const double pz =
-                                            (double)((T)wp.coef[z] * power_z<T>(q.B[l], sq, z, wp, false));
+                                        const double pz = (double)((T)wp.coef[z] * pw[z]);
 #pragma unroll
                                         for (int e = 0; e < 2; ++e)
                                             if (e < wt.ne)
@@ -599,7 +708,6 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                     }
                 }
                 reduce_add<16, G>(acc3, s, wp.slots + 16 + p * 16, row, assign);
-                if (NPB > 1) __syncthreads();
             }
         }
         __syncthreads();
@@ -623,12 +731,12 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     constexpr int C = kW / G;
     extern __shared__ double lds_base[];
     const int lane = threadIdx.x, g = lane / G, s = lane % G;
-    const Shared S = carve_shared(lds_base);
-    const Grp L = carve_group(lds_base, g, maxn, tmax, true, NPB);
+    const int ns = wt.n_species;
+    const Shared S = carve_shared(lds_base, ns);
+    const Grp L = carve_group(lds_base, ns, g, maxn, tmax, true);
     stage_shared(S, wt, lane);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
-    const int ns = wt.n_species;
     if (dc.hdr) {
         n_centers = dc.hdr[0];
         n_edges = dc.hdr[2];
@@ -657,7 +765,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         const int Nmax = tile_max<G>(N);
 
         // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
-        int M = 0;
+        int M = 0, n1 = 0;
         for (int j0 = 0; j0 < Nmax; j0 += G) {
             const int jj = j0 + s;
             bool valid = false;
@@ -715,16 +823,19 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 L.fz[pos] = sum * d2 * ir;
             }
             M += __popcll(m);
+            if (NPB > 1) n1 += __popcll(group_bits<G>(__ballot(valid && sj == 1), g));
         }
         __syncthreads();
 
         // ---- triplets ----
         if (wt.has3) {
-            const int cnt = enumerate_pairs<P, NPB, G>(S, L, ns, M, g, s);
+            int cnt[3], base[3];
+            enumerate_pairs<P, NPB, G>(S, L, ns, M, n1, g, s, cnt, base);
             __syncthreads();
+#pragma unroll
             for (int p = 0; p < NPB; ++p) {
-                int nb = 0;
-                const unsigned short *lst = block_list<NPB, G>(L, cnt, p, g, s, nb);
+                const int nb = cnt[p];
+                const unsigned short *lst = L.l0 + base[p];
                 // row weights of this block: dE/dzeta 2^(1-zeta), and times zeta lambda
                 if (s < 16) {
                     const int z = s >> 2, l = (s >> 1) & 1;
@@ -747,12 +858,12 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
                             if (l < wt.nl) {
-                                T sq[6];
-                                squarings(q.B[l], nsq, sq);
+                                T pm[4];
+                                powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
                                     if (z < wt.nz) {
-                                        const T pm1 = power_z<T>(q.B[l], sq, z, wp, true);
+                                        const T pm1 = pm[z];
                                         const T pw = pm1 * q.B[l];
 #pragma unroll
                                         for (int e = 0; e < 2; ++e) {
@@ -826,34 +937,34 @@ int grid_for(int n_tiles, int cap)
     return std::max(g, 1);
 }
 
-size_t lds_for(int max_nn, bool adjoint, int npb, int G)
+size_t lds_for(int max_nn, bool adjoint, int ns, int G)
 {
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * shared_doubles() + (size_t)(kW / G) * group_bytes(maxn, tmax, adjoint, npb);
+    return sizeof(double) * shared_doubles(ns) + (size_t)(kW / G) * group_bytes(maxn, tmax, adjoint);
 }
 
 }  // namespace
 
-int wave_group_size(int max_nn, int npb)
+int wave_group_size(int max_nn, int n_species)
 {
     if (max_nn > 64) return 0;
     // LDSF_WAVE_GROUP=16|32|64 fixes the group size (tuning), when its LDS fits
     if (const char *env = std::getenv("LDSF_WAVE_GROUP")) {
         const int G = std::atoi(env);
-        if ((G == 16 || G == 32 || G == 64) && lds_for(max_nn, true, npb, G) <= kWaveMaxLds)
+        if ((G == 16 || G == 32 || G == 64) && lds_for(max_nn, true, n_species, G) <= kWaveMaxLds)
             return G;
     }
     // A whole wave per centre: the LDS per centre is the same for every group
     // size, so smaller groups only mean fewer resident waves (measured on the
     // 262,144-atom two-species system: 3.14 / 3.61 / 5.27 ms per fwd+adjoint
     // step for G = 64 / 32 / 16).
-    return lds_for(max_nn, true, npb, 64) <= kWaveMaxLds ? 64 : 0;
+    return lds_for(max_nn, true, n_species, 64) <= kWaveMaxLds ? 64 : 0;
 }
 
-size_t wave_lds_bytes(int max_nn, bool adjoint, int npb, int group)
+size_t wave_lds_bytes(int max_nn, bool adjoint, int n_species, int group)
 {
-    return lds_for(max_nn, adjoint, npb, group);
+    return lds_for(max_nn, adjoint, n_species, group);
 }
 
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
@@ -931,7 +1042,7 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    const size_t lds = lds_for(max_nn, false, wt.npb, group);
+    const size_t lds = lds_for(max_nn, false, wt.n_species, group);
     const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group), dc.max_blocks)),
         block(kW);
 #define LDSF_FWD(P, NSB, NPB, G)                                                                  \
@@ -961,7 +1072,7 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (n_centers <= 0) return hipSuccess;
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    const size_t lds = lds_for(max_nn, true, wt.npb, group);
+    const size_t lds = lds_for(max_nn, true, wt.n_species, group);
     const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group), dc.max_blocks)),
         block(kW);
 #define LDSF_ADJ(P, NSB, NPB, G)                                                                  \
