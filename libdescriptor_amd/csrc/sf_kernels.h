@@ -46,6 +46,7 @@ struct WaveParams {
     double eta2[8], rs2[8];      // g2 rows
     double zeta[4], coef[4];     // distinct g4 zeta and 2^(1 - zeta)
     double lam[2], eta3[2];      // distinct g4 lambda and eta
+    double lp[2], lm[2];         // (1 + lambda) / 2 and (1 - lambda) / 2
     int izeta[4];                // integer zeta or -1
     int lamcode[2];              // +1 / -1 for lambda = +1 / -1, else 0
     int nsq;                     // squarings needed for the largest integer zeta

@@ -289,7 +289,7 @@ __device__ __forceinline__ void powers(typename Real<P>::T b, int nsq, const Wav
         const float lb = __builtin_amdgcn_logf(b);
 #pragma unroll
         for (int z = 0; z < 4; ++z)
-            if (z < nz) pw[z] = b > 0.0f ? __builtin_amdgcn_exp2f((float)wp.zeta[z] * lb) : 0.0f;
+            pw[z] = b > 0.0f ? __builtin_amdgcn_exp2f((float)wp.zeta[z] * lb) : 0.0f;
     }
 }
 
@@ -309,8 +309,7 @@ __device__ __forceinline__ void powers_m1(typename Real<P>::T b, int nsq, const 
         const float lb = __builtin_amdgcn_logf(b);
 #pragma unroll
         for (int z = 0; z < 4; ++z)
-            if (z < nz)
-                pm[z] = b > 0.0f ? __builtin_amdgcn_exp2f(((float)wp.zeta[z] - 1.0f) * lb) : 0.0f;
+            pm[z] = b > 0.0f ? __builtin_amdgcn_exp2f(((float)wp.zeta[z] - 1.0f) * lb) : 0.0f;
     }
 }
 
@@ -537,22 +536,16 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
         t.rjk = rjkf;
         const double h = 0.5 * t.ria * t.rib;
         const float inv = (float)h;
+        // 1 + lambda cos = ((1 + lambda)/2 (r_ij + r_ik)^2 - (1 - lambda)/2 (r_ij - r_ik)^2
+        //                   - lambda r_jk^2) / (2 r_ij r_ik): for lambda = +1 / -1 the
+        // cancellation-free forms ((r_ij + r_ik)^2 - r_jk^2) and (r_jk^2 - (r_ij - r_ik)^2)
+        const double sp = (t.ra + t.rb) * (t.ra + t.rb), sm = (t.ra - t.rb) * (t.ra - t.rb);
 #pragma unroll
-        for (int l = 0; l < 2; ++l) {
-            if (l < wt.nl) {
-                const int code = __builtin_amdgcn_readfirstlane(wp.lamcode[l]);
-                if (code == 1)  // 1 + cos = ((r_ij + r_ik)^2 - r_jk^2) / (2 r_ij r_ik)
-                    t.B[l] = (float)((t.ra + t.rb) * (t.ra + t.rb) - rjk2) * inv;
-                else if (code == -1)  // 1 - cos = (r_jk^2 - (r_ij - r_ik)^2) / (2 r_ij r_ik)
-                    t.B[l] = (float)(rjk2 - (t.ra - t.rb) * (t.ra - t.rb)) * inv;
-                else
-                    t.B[l] = 1.0f + (float)wp.lam[l] * ((float)(r2a + r2b - rjk2) * inv);
-            }
-        }
+        for (int l = 0; l < 2; ++l)
+            t.B[l] = (float)(wp.lp[l] * sp - wp.lm[l] * sm - wp.lam[l] * rjk2) * inv;
         const double S2 = r2a + r2b + rjk2;
 #pragma unroll
-        for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = dexp(-wp.eta3[e] * S2, R);
+        for (int e = 0; e < 2; ++e) t.E[e] = dexp(-wp.eta3[e] * S2, R);
         // rc - r_jk = (rc^2 - r_jk^2) / (rc + r_jk) keeps its relative accuracy near rc
         const float u = (float)(S.rc2[k] - rjk2) / ((float)rc + rjkf);
         float sn, cs;
@@ -649,12 +642,15 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                 L.sp[pos] = sj;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    if (q < wt.nr2) {
+                    if (P == 1 || q < wt.nr2) {  // rows past nr2: eta 0, no slot
                         const double dr = r - wp.rs2[q];
                         const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
-#pragma unroll
-                        for (int sb = 0; sb < NSB; ++sb)
-                            if (NSB == 1 || sb == sj) acc2[sb * 8 + q] += v;
+                        if (NSB == 1) {
+                            acc2[q] += v;
+                        } else {  // the neighbour's species block gets v, the other 0
+                            acc2[q] += sj ? 0.0 : v;
+                            acc2[8 + q] += sj ? v : 0.0;
+                        }
                     }
                 }
             }
@@ -687,18 +683,20 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                         double te[2];
 #pragma unroll
                         for (int e = 0; e < 2; ++e) te[e] = (double)(q.E[e] * fc3);
+                        // FP32-bulk: the full grid, without branches (entries past
+                        // it are neutral and have no slot); FP64: the grid's own size
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
-                            if (l < wt.nl) {
+                            if (P == 1 || l < wt.nl) {
                                 T pw[4];
                                 powers<P>(q.B[l], nsq, wp, wt.nz, pw);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
-                                    if (z < wt.nz) {
+                                    if (P == 1 || z < wt.nz) {
                                         const double pz = (double)((T)wp.coef[z] * pw[z]);
 #pragma unroll
                                         for (int e = 0; e < 2; ++e)
-                                            if (e < wt.ne)
+                                            if (P == 1 || e < wt.ne)
                                                 acc3[(z * 2 + l) * 2 + e] =
                                                     fma(pz, te[e], acc3[(z * 2 + l) * 2 + e]);
                                     }
@@ -799,7 +797,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 double sum = 0.0;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    if (q < wt.nr2) {
+                    if (q < wt.nr2) {  // gv[slot] needs a row of the set
                         const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
                         const double dr = r - wp.rs2[q];
                         const T ex = dexp(-wp.eta2[q] * dr * dr, R);
@@ -841,8 +839,9 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                     const int z = s >> 2, l = (s >> 1) & 1;
                     const int slot = wp.slots[16 + p * 16 + s];
                     const double ww = (work && slot >= 0) ? gv[slot] * wp.coef[z] : 0.0;
-                    L.w[s] = ww;
-                    L.wv[s] = ww * wp.zeta[z] * wp.lam[l];
+                    // kept in the compute type: a loop-invariant copy costs half the registers
+                    reinterpret_cast<T *>(L.w)[s] = (T)ww;
+                    reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
                 }
                 __syncthreads();
                 const int nbmax = tile_max<G>(nb);
@@ -857,18 +856,18 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
-                            if (l < wt.nl) {
+                            if (P == 1 || l < wt.nl) {
                                 T pm[4];
                                 powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
-                                    if (z < wt.nz) {
+                                    if (P == 1 || z < wt.nz) {
                                         const T pm1 = pm[z];
                                         const T pw = pm1 * q.B[l];
 #pragma unroll
                                         for (int e = 0; e < 2; ++e) {
-                                            U[e] += (T)L.w[(z * 2 + l) * 2 + e] * pw;
-                                            V[e] += (T)L.wv[(z * 2 + l) * 2 + e] * pm1;
+                                            U[e] += reinterpret_cast<const T *>(L.w)[(z * 2 + l) * 2 + e] * pw;
+                                            V[e] += reinterpret_cast<const T *>(L.wv)[(z * 2 + l) * 2 + e] * pm1;
                                         }
                                     }
                                 }
@@ -877,7 +876,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         T SA = (T)0, SB = (T)0, SC = (T)0;
 #pragma unroll
                         for (int e = 0; e < 2; ++e) {
-                            if (e < wt.ne) {
+                            if (P == 1 || e < wt.ne) {
                                 SA += V[e] * q.E[e];
                                 const T ue = U[e] * q.E[e];
                                 SB += ue;
@@ -1025,6 +1024,13 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
     for (int l = 0; l < wt->nl; ++l) {
         wp->lam[l] = ls[l];
         wp->lamcode[l] = ls[l] == 1.0 ? 1 : (ls[l] == -1.0 ? -1 : 0);
+    }
+    // entries past the grid stay neutral (lambda 0, zeta 0, coefficient 0, eta
+    // 0): the FP32-bulk kernels evaluate the full 4 x 2 x 2 grid without
+    // branches and those rows have no slot
+    for (int l = 0; l < 2; ++l) {
+        wp->lp[l] = 0.5 * (1.0 + wp->lam[l]);
+        wp->lm[l] = 0.5 * (1.0 - wp->lam[l]);
     }
     for (int e = 0; e < wt->ne; ++e) wp->eta3[e] = es[e];
     for (auto &kv : seen) {
