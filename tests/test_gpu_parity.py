@@ -381,3 +381,54 @@ def test_cpp_drop_in_caller(c1, tmp_path):
     G0 = O.gradient(n, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"],
                     c1["dE_dzeta"])
     assert rel_err(grad, G0) <= TOL64
+
+
+GRIDS = {
+    # 3 g2 rows + a 2 x 1 x 1 g4 grid
+    "small_grid": (["g2", "g4"], [3, 2, 2, 3],
+                   [0.01, 0.0, 0.1, 0.5, 0.3, 1.0, 1.0, -1.0, 0.005, 4.0, -1.0, 0.005]),
+    # g2 only
+    "g2_only": (["g2"], [5, 2], [0.001, 0.0, 0.01, 0.0, 0.05, 1.0, 0.2, 0.0, 0.5, 2.0]),
+    # g4 only, lambda not +/-1, a non-integer zeta
+    "g4_general": (["g4"], [4, 3], [1.0, 0.5, 0.002, 2.5, 0.5, 0.002, 1.0, 0.5, 0.02,
+                                    2.5, 0.5, 0.02]),
+}
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("grid", sorted(GRIDS))
+def test_wave_kernels_on_partial_grids(grid, precision):
+    """Parameter sets that fill only part of the wave kernels' 8-row g2 and
+    4 x 2 x 2 g4 grid (the rest is evaluated with neutral entries and has no
+    slot): wave kernels against the thread-per-centre kernels and the oracle."""
+    import torch
+    from libdescriptor_amd import _capi
+    L = small_two_species(400, seed=17)
+    species = ("A", "B")
+    kinds = GRIDS[grid]
+    ds = sf_desc(species, 5.0, kinds=kinds, resolved=True)
+    t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device="cuda")  # noqa: E731
+    plan = _capi.Plan(ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
+                      t(L["numneigh"]))
+    assert plan.kernel_family == 1
+    n = L["n_contrib"]
+    x = t(L["coords"], torch.float64)
+    g = t(np.random.default_rng(21).normal(size=(n, ds.width)), torch.float64)
+    out = {}
+    for fam in (0, 1):
+        plan.kernel_family = fam
+        d = torch.full((n, ds.width), 0.0, dtype=torch.float64, device="cuda")
+        dx = torch.zeros_like(x)
+        plan.compute(x, d, precision)
+        plan.gradient(x, g, dx, precision)
+        torch.cuda.synchronize()
+        out[fam] = (d.cpu().numpy(), dx.cpu().numpy())
+    tol = TOL64 if precision == 0 else TOL32
+    O = oracle_of(species, 5.0, kinds=kinds, resolved=True)
+    d0 = O.compute(n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    G0 = O.gradient(n, L["species"], L["neighlist"], L["numneigh"], L["coords"], g.cpu().numpy())
+    for fam in (0, 1):
+        sig = np.abs(d0) >= 1e-6 * np.abs(d0).max(axis=0, keepdims=True)
+        rel = np.abs(out[fam][0] - d0)[sig] / np.abs(d0)[sig]
+        assert rel.max() <= tol, (fam, rel.max())
+        assert rel_err(out[fam][1], G0) <= tol
