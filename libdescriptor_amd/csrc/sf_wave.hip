@@ -324,7 +324,7 @@ struct Shared {
 // one group's arrays, all sized for maxn neighbours (lists for tmax pairs)
 struct Grp {
     double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
-    double *dfc, *fx, *fy, *fz, *w, *wv;  // adjoint only
+    double *dfc, *fx, *fy, *fz, *w, *wv, *g;  // adjoint only (g: the centre's dE/dzeta row)
     int *sp, *orig;
     unsigned short *l0;  // active pairs, one region per species-pair block
 };
@@ -337,7 +337,8 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 __host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint)
 {
     const size_t nd = adjoint ? 11 : 7;
-    size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 : 0)) + sizeof(int) * 2 * maxn +
+    size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 + kWaveSlots : 0)) +
+               sizeof(int) * 2 * maxn +
                sizeof(unsigned short) * tmax;
     return (b + 15) & ~size_t(15);
 }
@@ -375,8 +376,9 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, int g, int maxn
         L.fz = p; p += maxn;
         L.w = p; p += 16;
         L.wv = p; p += 16;
+        L.g = p; p += kWaveSlots;
     } else {
-        L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = nullptr;
+        L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
     }
     int *ip = reinterpret_cast<int *>(p);
     L.sp = ip; ip += maxn;
@@ -547,7 +549,7 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 #pragma unroll
         for (int e = 0; e < 2; ++e) t.E[e] = dexp(-wp.eta3[e] * S2, R);
         // rc - r_jk = (rc^2 - r_jk^2) / (rc + r_jk) keeps its relative accuracy near rc
-        const float u = (float)(S.rc2[k] - rjk2) / ((float)rc + rjkf);
+        const float u = (float)(S.rc2[k] - rjk2) * __builtin_amdgcn_rcpf((float)rc + rjkf);
         float sn, cs;
         sincos_quarter(u * (float)S.rcs[k], sn, cs);
         t.fjk = sn * sn;
@@ -759,7 +761,9 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
             xi1 = x[3 * (size_t)i + 1];
             xi2 = x[3 * (size_t)i + 2];
             gv = dc.in_row ? dc.in_row[c] : dE + (size_t)c * width;
+            for (int k = s; k < width; k += G) L.g[k] = gv[k];  // coalesced, read from LDS below
         }
+        __syncthreads();
         const int Nmax = tile_max<G>(N);
 
         // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
@@ -797,12 +801,12 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 double sum = 0.0;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    if (q < wt.nr2) {  // gv[slot] needs a row of the set
+                    if (q < wt.nr2) {  // L.g[slot] needs a row of the set
                         const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
                         const double dr = r - wp.rs2[q];
                         const T ex = dexp(-wp.eta2[q] * dr * dr, R);
                         const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
-                        sum += gv[slot] * (double)dphi;
+                        sum += L.g[slot] * (double)dphi;
                     }
                 }
                 const double ir = 1.0 / r;
@@ -838,7 +842,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 if (s < 16) {
                     const int z = s >> 2, l = (s >> 1) & 1;
                     const int slot = wp.slots[16 + p * 16 + s];
-                    const double ww = (work && slot >= 0) ? gv[slot] * wp.coef[z] : 0.0;
+                    const double ww = (work && slot >= 0) ? L.g[slot] * wp.coef[z] : 0.0;
                     // kept in the compute type: a loop-invariant copy costs half the registers
                     reinterpret_cast<T *>(L.w)[s] = (T)ww;
                     reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
@@ -1008,6 +1012,8 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
         seen[k] = q;
     }
     if ((int)seen.size() != (int)(zs.size() * ls.size() * es.size())) return false;
+    // the adjoint stages a centre's whole descriptor row (<= kWaveSlots entries)
+    if (n2 * wt->nsb + n3 * wt->npb > kWaveSlots) return false;
     wt->has3 = 1;
     wt->nz = (int)zs.size();
     wt->nl = (int)ls.size();
