@@ -1,28 +1,28 @@
-// Lane-group symmetry-function kernels for gfx950.
+// Wave-per-centre symmetry-function kernels for gfx950.
 //
-// A wavefront is split into 64/G groups of G lanes (G = 16, 32 or 64); each
-// group owns one centre atom, so one wave works on 64/G centres at a time and
-// the cross-lane steps below (ballot compaction, loop bookkeeping, the
-// reduction of the accumulators) are paid once per wave for all of them.
-// Persistent grid; tiles of 64/G consecutive centres; each XCD takes one
-// contiguous slice of tiles.  Per tile:
-//   1. pair phase: the group's lanes walk its centre's neighbour-list entries,
-//      form the FP64 displacement and distance, and compact the entries inside
-//      the cutoff (list order kept) into the group's LDS arrays; each lane adds
-//      its g2 rows into FP64 accumulators.
+// One 64-lane wavefront owns one centre atom at a time; a persistent grid
+// walks the centres, each XCD one contiguous slice of them.  Per centre:
+//   1. pair phase: lane s takes neighbour-list entry s (at most 64), forms the
+//      FP64 displacement and distance, and the entries inside the cutoff are
+//      compacted (list order kept) into LDS arrays; each lane adds its g2
+//      rows into FP64 accumulators.
 //   2. triplet enumeration: the lanes walk all pairs (a < b) of compacted
-//      neighbours along diagonals b - a = 1, 2, ... (so one instruction touches
-//      distinct neighbours) and compact the pairs with r_jk < rc_jk into an LDS
-//      list tagged with the species-pair block.
-//   3. triplet phase: the lanes walk the active list only.  cos(theta),
-//      fc_jk, one exp per distinct g4 eta and one squaring chain per distinct
-//      lambda serve all rows of a triplet; each row adds one FP64 fma.
-//   4. the accumulators are summed over the group's lanes with a fixed-order
-//      butterfly (reduce-scatter) and added into the caller's descriptor row
-//      (the reference's += semantics).
+//      neighbours in column-major order and compact the pairs with
+//      r_jk < rc_jk into one LDS region per species-pair block.
+//   3. triplet phase: the lanes walk the active pairs only.  cos(theta),
+//      fc_jk, one exp per distinct g4 eta and the zeta powers of each lambda's
+//      base serve all rows of a triplet; each row adds one FP64 fma.
+//   4. the accumulators are summed over the lanes with a fixed-order
+//      reduce-scatter butterfly (permlane swaps and DPP) and written into the
+//      descriptor row (the reference's += semantics, or = for the serving
+//      loop).
+// Global loads run one centre ahead of their use: the next centre's record at
+// the start of a centre, its neighbour indices after the pair phase, its
+// neighbours' species and coordinates after the enumeration, so the chain
+// offsets -> neighbour list -> coordinates is hidden behind the triplet work.
 // The adjoint kernel forms dE/d r_ij, dE/d r_ik and dE/d r_jk of a triplet
 // from three row sums and adds the resulting vectors into per-neighbour FP64
-// LDS accumulators (ds_add_f64; a group owns its centre, so the order is
+// LDS accumulators (ds_add_f64; the wave owns its centre, so the order is
 // fixed); it then writes one record per neighbour-list edge and one for the
 // centre, at the positions the atomic-free gather in sf_plan.hip reads.
 // Nothing is atomic in global memory, and a centre's results do not depend
@@ -31,7 +31,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -45,12 +44,6 @@ namespace {
 
 constexpr int kW = 64;
 
-template <int G>
-struct Lg;
-template <> struct Lg<16> { static constexpr int v = 4; };
-template <> struct Lg<32> { static constexpr int v = 5; };
-template <> struct Lg<64> { static constexpr int v = 6; };
-
 template <int K>
 struct Log2;
 template <> struct Log2<1> { static constexpr int v = 0; };
@@ -59,26 +52,9 @@ template <> struct Log2<4> { static constexpr int v = 2; };
 template <> struct Log2<8> { static constexpr int v = 3; };
 template <> struct Log2<16> { static constexpr int v = 4; };
 
-// bits of group g in a wave ballot
-template <int G>
-__device__ __forceinline__ unsigned long long group_bits(unsigned long long m, int g)
-{
-    if (G == 64) return m;
-    return (m >> (g * G)) & ((1ull << G) - 1ull);
-}
-
 __device__ __forceinline__ int below(unsigned long long m, int s)
 {
     return __popcll(m & ((1ull << s) - 1ull));
-}
-
-// maximum of a per-group value over the groups of the wave (uniform result)
-template <int G>
-__device__ __forceinline__ int tile_max(int v)
-{
-#pragma unroll
-    for (int m = G; m < kW; m <<= 1) v = max(v, __shfl_xor(v, m));
-    return v;
 }
 
 // Cross-lane exchanges of the reduction butterfly, for partner bit M.
@@ -122,10 +98,10 @@ __device__ __forceinline__ double dpp_partner(double v)
                                             (unsigned)lo));
 }
 
-// Recursive-halving reduce-scatter of K values over the G lanes of a group:
-// step with H live values and partner bit M, all compile-time.  Each step
-// pairs lanes that differ in bit M and agree in the bits already reduced, so
-// every lane's partial sums stay disjoint.
+// Recursive-halving reduce-scatter of K values over the 64 lanes: step with H
+// live values and partner bit M, all compile-time.  Each step pairs lanes that
+// differ in bit M and agree in the bits already reduced, so every lane's
+// partial sums stay disjoint.
 template <int K, int H, int M>
 struct Halve {
     static __device__ __forceinline__ void run(double (&v)[K], int s)
@@ -170,25 +146,24 @@ struct Halve<K, 0, 0> {
     static __device__ __forceinline__ void run(double (&)[K], int) {}
 };
 
-// Afterwards v[0] of sub-lane s holds the group total of value
-// (s >> (log2 G - log2 K)) & (K - 1); the butterfly order is fixed.
-template <int K, int G>
-__device__ __forceinline__ void group_reduce(double (&v)[K], int s)
+// Afterwards v[0] of lane s holds the wave total of value (s >> (6 - log2 K));
+// the butterfly order is fixed.
+template <int K>
+__device__ __forceinline__ void wave_reduce(double (&v)[K], int s)
 {
-    static_assert(K <= G, "more values than lanes");
-    Halve<K, K / 2, G / 2>::run(v, s);
+    Halve<K, K / 2, kW / 2>::run(v, s);
 }
 
 // every slot of a row is reached exactly once per centre, so `assign` may
 // store the sum instead of adding it
-template <int K, int G>
+template <int K>
 __device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *row,
                                            bool assign)
 {
-    group_reduce<K, G>(v, s);
-    constexpr int sh = Lg<G>::v - Log2<K>::v;
+    wave_reduce<K>(v, s);
+    constexpr int sh = 6 - Log2<K>::v;
     if (row && (s & ((1 << sh) - 1)) == 0) {
-        const int slot = slots[(s >> sh) & (K - 1)];
+        const int slot = slots[s >> sh];
         if (slot >= 0) row[slot] = assign ? v[0] : row[slot] + v[0];
     }
 }
@@ -205,8 +180,8 @@ __device__ __forceinline__ void pair_of(int t, int &a, int &b)
     a = t - c * (c - 1) / 2;
 }
 
-// tiles of this block: blocks are dealt to the 8 XCDs round-robin (block b
-// on XCD b % 8); each XCD takes one contiguous slice of tiles
+// centres of this block: blocks are dealt to the 8 XCDs round-robin (block b
+// on XCD b % 8); each XCD takes one contiguous slice of the centres
 struct TileRange {
     int hi, step, first;
 };
@@ -231,7 +206,7 @@ __device__ __forceinline__ TileRange tile_range(int n)
 }
 
 // B^(2^k), k = 0..nsq (nsq uniform): every integer zeta <= 63 of the grid is a
-// product of these, so one squaring chain serves all rows of a triplet
+// product of these
 template <typename T>
 __device__ __forceinline__ void squarings(T b, int nsq, T (&sq)[6])
 {
@@ -321,12 +296,12 @@ struct Shared {
     double *rc, *rcinv, *rc2, *rcs;  // rc, 1/rc, rc^2, (pi/2)/rc per species pair (ns^2 each)
 };
 
-// one group's arrays, all sized for maxn neighbours (lists for tmax pairs)
+// the centre's arrays, sized for maxn neighbours (the pair list for tmax pairs)
 struct Grp {
     double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
-    double *dfc, *fx, *fy, *fz, *w, *wv, *g;  // adjoint only (g: the centre's dE/dzeta row)
-    int *sp, *orig;
-    unsigned short *l0;  // active pairs, one region per species-pair block
+    double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (g: the centre's dE/dzeta row)
+    int *sp, *ip;                               // species; adjoint: record position of the edge
+    unsigned short *l0;                         // active pairs, one region per species-pair block
 };
 
 __host__ __device__ constexpr size_t shared_doubles(int ns)
@@ -338,8 +313,7 @@ __host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint)
 {
     const size_t nd = adjoint ? 11 : 7;
     size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 + kWaveSlots : 0)) +
-               sizeof(int) * 2 * maxn +
-               sizeof(unsigned short) * tmax;
+               sizeof(int) * 2 * maxn + sizeof(unsigned short) * tmax;
     return (b + 15) & ~size_t(15);
 }
 
@@ -355,12 +329,9 @@ __device__ __forceinline__ Shared carve_shared(double *base, int ns)
     return S;
 }
 
-__device__ __forceinline__ Grp carve_group(double *base, int ns, int g, int maxn, int tmax,
-                                           bool adjoint)
+__device__ __forceinline__ Grp carve_group(double *base, int ns, int maxn, bool adjoint)
 {
-    char *q = reinterpret_cast<char *>(base + shared_doubles(ns)) +
-              (size_t)g * group_bytes(maxn, tmax, adjoint);
-    double *p = reinterpret_cast<double *>(q);
+    double *p = base + shared_doubles(ns);
     Grp L;
     L.px = p; p += maxn;
     L.py = p; p += maxn;
@@ -382,7 +353,7 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, int g, int maxn
     }
     int *ip = reinterpret_cast<int *>(p);
     L.sp = ip; ip += maxn;
-    L.orig = ip; ip += maxn;
+    L.ip = ip; ip += maxn;
     L.l0 = reinterpret_cast<unsigned short *>(ip);
     return L;
 }
@@ -402,6 +373,64 @@ __device__ __forceinline__ void stage_shared(const Shared &S, const WaveTab &wt,
     int *dst = reinterpret_cast<int *>(S.par);
     for (int k = lane; k < (int)(sizeof(WaveParams) / 4); k += kW) dst[k] = src[k];
     __syncthreads();
+}
+
+// ---- loads one centre ahead ----
+
+// a centre's record (stage 1)
+struct Ctr {
+    int c, beg, N, si;
+    bool live, work;  // live: c < n_centers; work: live and its request not flagged
+    double x0, x1, x2;
+};
+
+__device__ __forceinline__ Ctr load_centre(int c, int n_centers, const int *centers,
+                                           const int *offs, const int *species, const double *x,
+                                           const DevCounts &dc)
+{
+    Ctr C;
+    C.c = c;
+    C.live = c < n_centers;
+    C.work = false;
+    C.beg = C.N = C.si = 0;
+    C.x0 = C.x1 = C.x2 = 0.0;
+    if (C.live && !(dc.req_flag && dc.req_flag[dc.req_of[c]])) {
+        C.work = true;
+        const int i = centers ? centers[c] : c;
+        C.beg = offs[c];
+        C.N = offs[c + 1] - C.beg;
+        C.si = species[i];
+        C.x0 = x[3 * (size_t)i];
+        C.x1 = x[3 * (size_t)i + 1];
+        C.x2 = x[3 * (size_t)i + 2];
+    }
+    return C;
+}
+
+// lane s's neighbour of a centre: its index (stage 2), then species and
+// absolute coordinates (stage 3)
+struct Nbr {
+    int sj;
+    double p0, p1, p2;
+};
+
+__device__ __forceinline__ int load_index(const Ctr &C, int s, const int *nl)
+{
+    return s < C.N ? nl[C.beg + s] : -1;
+}
+
+__device__ __forceinline__ Nbr load_nbr(int j, const int *species, const double *x)
+{
+    Nbr B;
+    B.sj = 0;
+    B.p0 = B.p1 = B.p2 = 0.0;
+    if (j >= 0) {
+        B.sj = species[j];
+        B.p0 = x[3 * (size_t)j];
+        B.p1 = x[3 * (size_t)j + 1];
+        B.p2 = x[3 * (size_t)j + 2];
+    }
+    return B;
 }
 
 // cutoff value and derivative of a neighbour at distance r < rc; in the
@@ -424,16 +453,15 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
     dfc = -(float)(2.0 * rcs) * s * c;
 }
 
-// Phase 2: the active pairs (a < b, r_jk < rc_jk) of the group's M compacted
+// Phase 2: the active pairs (a < b, r_jk < rc_jk) of the M compacted
 // neighbours, walked in column-major order, compacted into one region of
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
 // n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.
-template <int P, int NPB, int G>
+template <int P, int NPB>
 __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, int ns, int M,
-                                                int n1, int g, int s, int (&cnt)[3],
-                                                int (&base)[3])
+                                                int n1, int s, int (&cnt)[3], int (&base)[3])
 {
     const int n0 = M - n1;
     base[0] = 0;
@@ -441,8 +469,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
     base[2] = base[1] + n0 * n1;
     cnt[0] = cnt[1] = cnt[2] = 0;
     const int T = M * (M - 1) / 2;
-    const int Tmax = tile_max<G>(T);
-    for (int t0 = 0; t0 < Tmax; t0 += G) {
+    for (int t0 = 0; t0 < T; t0 += kW) {
         const int t = t0 + s;
         bool act = false;
         int code = 0, blk = 0;
@@ -458,13 +485,13 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
             if (NPB > 1) blk = pair_index(ns, sa, sb);
         }
         if (NPB == 1) {
-            const unsigned long long m = group_bits<G>(__ballot(act), g);
+            const unsigned long long m = __ballot(act);
             if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
             cnt[0] += __popcll(m);
         } else {
-            const unsigned long long m0 = group_bits<G>(__ballot(act && blk == 0), g);
-            const unsigned long long m1 = group_bits<G>(__ballot(act && blk == 1), g);
-            const unsigned long long m2 = group_bits<G>(__ballot(act && blk == 2), g);
+            const unsigned long long m0 = __ballot(act && blk == 0);
+            const unsigned long long m1 = __ballot(act && blk == 1);
+            const unsigned long long m2 = __ballot(act && blk == 2);
             if (act) {
                 const int pos = blk == 0 ? base[0] + cnt[0] + below(m0, s)
                               : blk == 1 ? base[1] + cnt[1] + below(m1, s)
@@ -478,7 +505,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
     }
 }
 
-// shared quantities of one active triplet (a, b) of a group
+// shared quantities of one active triplet (a, b)
 template <int P>
 struct Trip {
     using T = typename Real<P>::T;
@@ -563,119 +590,101 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
     }
 }
 
-template <int P, int NSB, int NPB, int G>
-__global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
+template <int P, int NSB, int NPB>
+__global__ __launch_bounds__(64, 3) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
                                                  const int *__restrict__ nl,
                                                  const int *__restrict__ species,
                                                  const double *__restrict__ x,
                                                  double *__restrict__ desc, int width, int maxn,
-                                                 int tmax, DevCounts dc)
+                                                 DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
-    constexpr int C = kW / G;
     extern __shared__ double lds_base[];
-    const int lane = threadIdx.x, g = lane / G, s = lane % G;
+    const int s = threadIdx.x;
     const int ns = wt.n_species;
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, g, maxn, tmax, false);
-    stage_shared(S, wt, lane);
+    const Grp L = carve_group(lds_base, ns, maxn, false);
+    stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     const bool assign = dc.out_row != nullptr;
     if (dc.hdr) n_centers = dc.hdr[0];
-    const int n_tiles = (n_centers + C - 1) / C;
-    const TileRange tr = tile_range(n_tiles);
-    for (int tile = tr.first; tile < tr.hi; tile += tr.step) {
-        const int c = tile * C + g;
-        int N = 0, beg = 0, si = 0;
-        double xi0 = 0, xi1 = 0, xi2 = 0;
-        double *row = nullptr;
-        if (c < n_centers && !(dc.req_flag && dc.req_flag[dc.req_of[c]])) {
-            const int i = centers ? centers[c] : c;
-            beg = offs[c];
-            N = offs[c + 1] - beg;
-            si = species[i];
-            xi0 = x[3 * (size_t)i];
-            xi1 = x[3 * (size_t)i + 1];
-            xi2 = x[3 * (size_t)i + 2];
-            row = dc.out_row ? dc.out_row[c] : desc + (size_t)c * width;
-        }
-        const int Nmax = tile_max<G>(N);
+    const TileRange tr = tile_range(n_centers);
+    if (tr.first >= tr.hi) return;
+    // the first centre's loads, in order
+    Ctr C = load_centre(tr.first, n_centers, centers, offs, species, x, dc);
+    Nbr B = load_nbr(load_index(C, s, nl), species, x);
+    for (int c = tr.first; c < tr.hi; c += tr.step) {
+        const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
+        double *row = C.work ? (dc.out_row ? dc.out_row[c] : desc + (size_t)c * width) : nullptr;
 
         // ---- pairs ----
         double acc2[NSB * 8];
 #pragma unroll
         for (int k = 0; k < NSB * 8; ++k) acc2[k] = 0.0;
-        int M = 0, n1 = 0;
-        for (int j0 = 0; j0 < Nmax; j0 += G) {
-            const int jj = j0 + s;
-            bool valid = false;
-            int sj = 0;
-            double p0 = 0, p1 = 0, p2 = 0, d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
-            if (jj < N) {
-                const int j = nl[beg + jj];
-                sj = species[j];
-                p0 = x[3 * (size_t)j];
-                p1 = x[3 * (size_t)j + 1];
-                p2 = x[3 * (size_t)j + 2];
-                d0 = p0 - xi0;
-                d1 = p1 - xi1;
-                d2 = p2 - xi2;
-                r2 = dist2(d0, d1, d2);
-                r = sqrt(r2);
-                valid = r < S.rc[si * ns + sj];  // r > rc is skipped; r == rc adds exactly 0
-            }
-            const unsigned long long m = group_bits<G>(__ballot(valid), g);
-            if (valid) {
-                const int pos = M + below(m, s);
-                const int k = si * ns + sj;
-                T fc, dfc;
-                cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
-                L.px[pos] = p0;
-                L.py[pos] = p1;
-                L.pz[pos] = p2;
-                L.r2[pos] = r2;
-                L.r[pos] = r;
-                L.rinv[pos] = 1.0 / r;
-                L.fc[pos] = (double)fc;
-                L.sp[pos] = sj;
+        bool valid = false;
+        double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
+        const int sj = B.sj;
+        if (s < C.N) {
+            d0 = B.p0 - C.x0;
+            d1 = B.p1 - C.x1;
+            d2 = B.p2 - C.x2;
+            r2 = dist2(d0, d1, d2);
+            r = sqrt(r2);
+            valid = r < S.rc[C.si * ns + sj];  // r > rc is skipped; r == rc adds exactly 0
+        }
+        const unsigned long long m = __ballot(valid);
+        const int M = __popcll(m);
+        const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
+        if (valid) {
+            const int pos = below(m, s);
+            const int k = C.si * ns + sj;
+            T fc, dfc;
+            cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
+            L.px[pos] = B.p0;
+            L.py[pos] = B.p1;
+            L.pz[pos] = B.p2;
+            L.r2[pos] = r2;
+            L.r[pos] = r;
+            L.rinv[pos] = 1.0 / r;
+            L.fc[pos] = (double)fc;
+            L.sp[pos] = sj;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    if (P == 1 || q < wt.nr2) {  // rows past nr2: eta 0, no slot
-                        const double dr = r - wp.rs2[q];
-                        const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
-                        if (NSB == 1) {
-                            acc2[q] += v;
-                        } else {  // the neighbour's species block gets v, the other 0
-                            acc2[q] += sj ? 0.0 : v;
-                            acc2[8 + q] += sj ? v : 0.0;
-                        }
+            for (int q = 0; q < 8; ++q) {
+                if (P == 1 || q < wt.nr2) {  // rows past nr2: eta 0, no slot
+                    const double dr = r - wp.rs2[q];
+                    const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
+                    if (NSB == 1) {
+                        acc2[q] += v;
+                    } else {  // the neighbour's species block gets v, the other 0
+                        acc2[q] += sj ? 0.0 : v;
+                        acc2[8 + q] += sj ? v : 0.0;
                     }
                 }
             }
-            M += __popcll(m);
-            if (NPB > 1) n1 += __popcll(group_bits<G>(__ballot(valid && sj == 1), g));
         }
-        reduce_add<NSB * 8, G>(acc2, s, wp.slots, row, assign);
+        const int jn = load_index(Cn, s, nl);
+        reduce_add<NSB * 8>(acc2, s, wp.slots, row, assign);
         __syncthreads();
 
         // ---- triplets ----
+        Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB, G>(S, L, ns, M, n1, g, s, cnt, base);
+            enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
+            Bn = load_nbr(jn, species, x);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < NPB; ++p) {
                 const int nb = cnt[p];
                 const unsigned short *lst = L.l0 + base[p];
-                const int nbmax = tile_max<G>(nb);
                 double acc3[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
-                for (int t0 = 0; t0 < nbmax; t0 += G) {
+                for (int t0 = 0; t0 < nb; t0 += kW) {
                     const int t = t0 + s;
                     if (t < nb) {
                         const int code = lst[t];
@@ -707,14 +716,18 @@ __global__ __launch_bounds__(64) void k_fwd_wave(WaveTab wt, int n_centers,
                         }
                     }
                 }
-                reduce_add<16, G>(acc3, s, wp.slots + 16 + p * 16, row, assign);
+                reduce_add<16>(acc3, s, wp.slots + 16 + p * 16, row, assign);
             }
+        } else {
+            Bn = load_nbr(jn, species, x);
         }
         __syncthreads();
+        C = Cn;
+        B = Bn;
     }
 }
 
-template <int P, int NSB, int NPB, int G>
+template <int P, int NSB, int NPB>
 __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
@@ -724,115 +737,105 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const double *__restrict__ dE,
                                                  const int *__restrict__ inv_pos,
                                                  double *__restrict__ edge, int n_edges, int width,
-                                                 int maxn, int tmax, DevCounts dc)
+                                                 int maxn, DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
-    constexpr int C = kW / G;
     extern __shared__ double lds_base[];
-    const int lane = threadIdx.x, g = lane / G, s = lane % G;
+    const int s = threadIdx.x;
     const int ns = wt.n_species;
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, g, maxn, tmax, true);
-    stage_shared(S, wt, lane);
+    const Grp L = carve_group(lds_base, ns, maxn, true);
+    stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
     if (dc.hdr) {
         n_centers = dc.hdr[0];
         n_edges = dc.hdr[2];
     }
-    const int n_tiles = (n_centers + C - 1) / C;
-    const TileRange tr = tile_range(n_tiles);
-    for (int tile = tr.first; tile < tr.hi; tile += tr.step) {
-        const int c = tile * C + g;
-        const bool live = c < n_centers;
-        int N = 0, beg = 0, si = 0;
-        double xi0 = 0, xi1 = 0, xi2 = 0;
-        const double *gv = nullptr;  // dE/dzeta row, read only when `work`
-        bool work = false;
+    const TileRange tr = tile_range(n_centers);
+    if (tr.first >= tr.hi) return;
+    // dE/dzeta entry s of a centre's row, and the record position of the centre
+    auto grad_of = [&](const Ctr &C) -> double {
+        if (!C.work || s >= width) return 0.0;
+        return dc.in_row ? dc.in_row[C.c][s] : dE[(size_t)C.c * width + s];
+    };
+    auto self_pos = [&](const Ctr &C) -> int { return C.live ? inv_pos[n_edges + C.c] : 0; };
+    Ctr C = load_centre(tr.first, n_centers, centers, offs, species, x, dc);
+    double gs = grad_of(C);
+    int ips = self_pos(C);
+    int ipe = s < C.N ? inv_pos[C.beg + s] : 0;  // record position of lane s's edge
+    Nbr B = load_nbr(load_index(C, s, nl), species, x);
+    for (int c = tr.first; c < tr.hi; c += tr.step) {
+        const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
+        const double gsn = grad_of(Cn);
+        const int ipsn = self_pos(Cn);
         // a skipped centre (flagged request) writes only its zero self record
-        if (live && !(dc.req_flag && dc.req_flag[dc.req_of[c]])) {
-            work = true;
-            const int i = centers ? centers[c] : c;
-            beg = offs[c];
-            N = offs[c + 1] - beg;
-            si = species[i];
-            xi0 = x[3 * (size_t)i];
-            xi1 = x[3 * (size_t)i + 1];
-            xi2 = x[3 * (size_t)i + 2];
-            gv = dc.in_row ? dc.in_row[c] : dE + (size_t)c * width;
-            for (int k = s; k < width; k += G) L.g[k] = gv[k];  // coalesced, read from LDS below
-        }
+        if (C.work && s < width) L.g[s] = gs;
         __syncthreads();
-        const int Nmax = tile_max<G>(N);
 
         // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
-        int M = 0, n1 = 0;
-        for (int j0 = 0; j0 < Nmax; j0 += G) {
-            const int jj = j0 + s;
-            bool valid = false;
-            int sj = 0;
-            double p0 = 0, p1 = 0, p2 = 0, d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
-            if (jj < N) {
-                const int j = nl[beg + jj];
-                sj = species[j];
-                p0 = x[3 * (size_t)j];
-                p1 = x[3 * (size_t)j + 1];
-                p2 = x[3 * (size_t)j + 2];
-                d0 = p0 - xi0;
-                d1 = p1 - xi1;
-                d2 = p2 - xi2;
-                r2 = dist2(d0, d1, d2);
-                r = sqrt(r2);
-                valid = r < S.rc[si * ns + sj];
-                if (!valid) {
-                    double *ej = edge + 3 * (size_t)inv_pos[beg + jj];
-                    ej[0] = 0.0;
-                    ej[1] = 0.0;
-                    ej[2] = 0.0;
-                }
+        bool valid = false;
+        double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
+        const int sj = B.sj;
+        if (s < C.N) {
+            d0 = B.p0 - C.x0;
+            d1 = B.p1 - C.x1;
+            d2 = B.p2 - C.x2;
+            r2 = dist2(d0, d1, d2);
+            r = sqrt(r2);
+            valid = r < S.rc[C.si * ns + sj];
+            if (!valid) {
+                double *ej = edge + 3 * (size_t)ipe;
+                ej[0] = 0.0;
+                ej[1] = 0.0;
+                ej[2] = 0.0;
             }
-            const unsigned long long m = group_bits<G>(__ballot(valid), g);
-            if (valid) {
-                const int pos = M + below(m, s);
-                const int k = si * ns + sj;
-                T fc, dfc;
-                cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
-                double sum = 0.0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    if (q < wt.nr2) {  // L.g[slot] needs a row of the set
-                        const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
-                        const double dr = r - wp.rs2[q];
-                        const T ex = dexp(-wp.eta2[q] * dr * dr, R);
-                        const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
-                        sum += L.g[slot] * (double)dphi;
-                    }
-                }
-                const double ir = 1.0 / r;
-                L.px[pos] = p0;
-                L.py[pos] = p1;
-                L.pz[pos] = p2;
-                L.r2[pos] = r2;
-                L.r[pos] = r;
-                L.rinv[pos] = ir;
-                L.fc[pos] = (double)fc;
-                L.dfc[pos] = (double)dfc;
-                L.sp[pos] = sj;
-                L.orig[pos] = jj;
-                L.fx[pos] = sum * d0 * ir;
-                L.fy[pos] = sum * d1 * ir;
-                L.fz[pos] = sum * d2 * ir;
-            }
-            M += __popcll(m);
-            if (NPB > 1) n1 += __popcll(group_bits<G>(__ballot(valid && sj == 1), g));
         }
+        const unsigned long long m = __ballot(valid);
+        const int M = __popcll(m);
+        const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
+        if (valid) {
+            const int pos = below(m, s);
+            const int k = C.si * ns + sj;
+            T fc, dfc;
+            cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
+            double sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (q < wt.nr2) {  // L.g[slot] needs a row of the set
+                    const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
+                    const double dr = r - wp.rs2[q];
+                    const T ex = dexp(-wp.eta2[q] * dr * dr, R);
+                    const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
+                    sum += L.g[slot] * (double)dphi;
+                }
+            }
+            const double ir = 1.0 / r;
+            L.px[pos] = B.p0;
+            L.py[pos] = B.p1;
+            L.pz[pos] = B.p2;
+            L.r2[pos] = r2;
+            L.r[pos] = r;
+            L.rinv[pos] = ir;
+            L.fc[pos] = (double)fc;
+            L.dfc[pos] = (double)dfc;
+            L.sp[pos] = sj;
+            L.ip[pos] = ipe;
+            L.fx[pos] = sum * d0 * ir;
+            L.fy[pos] = sum * d1 * ir;
+            L.fz[pos] = sum * d2 * ir;
+        }
+        const int jn = load_index(Cn, s, nl);
+        const int ipen = s < Cn.N ? inv_pos[Cn.beg + s] : 0;
         __syncthreads();
 
         // ---- triplets ----
+        Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB, G>(S, L, ns, M, n1, g, s, cnt, base);
+            enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
+            Bn = load_nbr(jn, species, x);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < NPB; ++p) {
@@ -842,14 +845,13 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 if (s < 16) {
                     const int z = s >> 2, l = (s >> 1) & 1;
                     const int slot = wp.slots[16 + p * 16 + s];
-                    const double ww = (work && slot >= 0) ? L.g[slot] * wp.coef[z] : 0.0;
+                    const double ww = (C.work && slot >= 0) ? L.g[slot] * wp.coef[z] : 0.0;
                     // kept in the compute type: a loop-invariant copy costs half the registers
                     reinterpret_cast<T *>(L.w)[s] = (T)ww;
                     reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
                 }
                 __syncthreads();
-                const int nbmax = tile_max<G>(nb);
-                for (int t0 = 0; t0 < nbmax; t0 += G) {
+                for (int t0 = 0; t0 < nb; t0 += kW) {
                     const int t = t0 + s;
                     if (t < nb) {
                         const int code = lst[t];
@@ -897,77 +899,70 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
                         const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
                                      a3 = (double)Ajk * q.rjkinv;
-                        atomicAdd(&L.fx[a], a1 * (L.px[a] - xi0) - a3 * q.e0);
-                        atomicAdd(&L.fy[a], a1 * (L.py[a] - xi1) - a3 * q.e1);
-                        atomicAdd(&L.fz[a], a1 * (L.pz[a] - xi2) - a3 * q.e2);
-                        atomicAdd(&L.fx[b], a2 * (L.px[b] - xi0) + a3 * q.e0);
-                        atomicAdd(&L.fy[b], a2 * (L.py[b] - xi1) + a3 * q.e1);
-                        atomicAdd(&L.fz[b], a2 * (L.pz[b] - xi2) + a3 * q.e2);
+                        atomicAdd(&L.fx[a], a1 * (L.px[a] - C.x0) - a3 * q.e0);
+                        atomicAdd(&L.fy[a], a1 * (L.py[a] - C.x1) - a3 * q.e1);
+                        atomicAdd(&L.fz[a], a1 * (L.pz[a] - C.x2) - a3 * q.e2);
+                        atomicAdd(&L.fx[b], a2 * (L.px[b] - C.x0) + a3 * q.e0);
+                        atomicAdd(&L.fy[b], a2 * (L.py[b] - C.x1) + a3 * q.e1);
+                        atomicAdd(&L.fz[b], a2 * (L.pz[b] - C.x2) + a3 * q.e2);
                     }
                 }
                 __syncthreads();
             }
+        } else {
+            Bn = load_nbr(jn, species, x);
         }
 
         // ---- edge records and the centre's own record ----
-        const int Mmax = tile_max<G>(M);
         double sv[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int a0 = 0; a0 < Mmax; a0 += G) {
-            const int a = a0 + s;
-            if (a < M) {
-                const double f0 = L.fx[a], f1 = L.fy[a], f2 = L.fz[a];
-                double *ej = edge + 3 * (size_t)inv_pos[beg + L.orig[a]];
-                ej[0] = f0;
-                ej[1] = f1;
-                ej[2] = f2;
-                sv[0] -= f0;
-                sv[1] -= f1;
-                sv[2] -= f2;
-            }
+        if (s < M) {
+            const double f0 = L.fx[s], f1 = L.fy[s], f2 = L.fz[s];
+            double *ej = edge + 3 * (size_t)L.ip[s];
+            ej[0] = f0;
+            ej[1] = f1;
+            ej[2] = f2;
+            sv[0] = -f0;
+            sv[1] = -f1;
+            sv[2] = -f2;
         }
-        group_reduce<4, G>(sv, s);
-        constexpr int sh = Lg<G>::v - 2;
-        if (live && (s & ((1 << sh) - 1)) == 0 && (s >> sh) < 3)
-            edge[3 * (size_t)inv_pos[n_edges + c] + (s >> sh)] = sv[0];
+        wave_reduce<4>(sv, s);
+        if (C.live && (s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ips + (s >> 4)] = sv[0];
         __syncthreads();
+        C = Cn;
+        B = Bn;
+        gs = gsn;
+        ips = ipsn;
+        ipe = ipen;
     }
 }
 
-int grid_for(int n_tiles, int cap)
+int grid_for(int n_centers, int cap)
 {
-    int g = std::min(n_tiles, cap > 0 ? cap : 16384);
+    int g = std::min(n_centers, cap > 0 ? cap : 16384);
     if (g >= 8) g = (g + 7) & ~7;
     return std::max(g, 1);
 }
 
-size_t lds_for(int max_nn, bool adjoint, int ns, int G)
+size_t lds_for(int max_nn, bool adjoint, int ns)
 {
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * shared_doubles(ns) + (size_t)(kW / G) * group_bytes(maxn, tmax, adjoint);
+    return sizeof(double) * shared_doubles(ns) + group_bytes(maxn, tmax, adjoint);
 }
 
 }  // namespace
 
 int wave_group_size(int max_nn, int n_species)
 {
-    if (max_nn > 64) return 0;
-    // LDSF_WAVE_GROUP=16|32|64 fixes the group size (tuning), when its LDS fits
-    if (const char *env = std::getenv("LDSF_WAVE_GROUP")) {
-        const int G = std::atoi(env);
-        if ((G == 16 || G == 32 || G == 64) && lds_for(max_nn, true, n_species, G) <= kWaveMaxLds)
-            return G;
-    }
-    // A whole wave per centre: the LDS per centre is the same for every group
-    // size, so smaller groups only mean fewer resident waves (measured on the
-    // 262,144-atom two-species system: 3.14 / 3.61 / 5.27 ms per fwd+adjoint
-    // step for G = 64 / 32 / 16).
-    return lds_for(max_nn, true, n_species, 64) <= kWaveMaxLds ? 64 : 0;
+    // one wavefront per centre, one lane per neighbour-list entry
+    if (max_nn > kW) return 0;
+    return lds_for(max_nn, true, n_species) <= kWaveMaxLds ? kW : 0;
 }
 
 size_t wave_lds_bytes(int max_nn, bool adjoint, int n_species, int group)
 {
-    return lds_for(max_nn, adjoint, n_species, group);
+    (void)group;
+    return lds_for(max_nn, adjoint, n_species);
 }
 
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
@@ -992,6 +987,10 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
         for (int s = 0; s < wt->nsb; ++s) slots[s * 8 + q] = sf->slot2[q] + s * sf->str2[q];
     }
     wt->nr2 = n2;
+    for (int l = 0; l < 2; ++l) {  // neutral lambda = 0 until the grid sets it
+        wp->lp[l] = 0.5;
+        wp->lm[l] = 0.5;
+    }
     // three-body rows: g4 forming a zeta x lambda x eta grid
     const int n3 = (int)sf->k3.size();
     if (n3 == 0) return true;
@@ -1052,26 +1051,18 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
                            DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
+    if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    const size_t lds = lds_for(max_nn, false, wt.n_species, group);
-    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group), dc.max_blocks)),
-        block(kW);
-#define LDSF_FWD(P, NSB, NPB, G)                                                                  \
-    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, G>), grid, block, lds, s, wt, n_centers, centers, \
-                       offs, nl, species, x, desc, width, maxn, tmax, dc)
-#define LDSF_FWD_G(P, NSB, NPB)                                                                   \
-    do {                                                                                          \
-        if (group == 16) LDSF_FWD(P, NSB, NPB, 16);                                               \
-        else if (group == 32) LDSF_FWD(P, NSB, NPB, 32);                                          \
-        else LDSF_FWD(P, NSB, NPB, 64);                                                           \
-    } while (0)
+    const size_t lds = lds_for(max_nn, false, wt.n_species);
+    const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
+#define LDSF_FWD(P, NSB, NPB)                                                                  \
+    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers, \
+                       offs, nl, species, x, desc, width, maxn, dc)
     if (wt.nsb == 1) {
-        if (precision == 0) LDSF_FWD_G(0, 1, 1); else LDSF_FWD_G(1, 1, 1);
+        if (precision == 0) LDSF_FWD(0, 1, 1); else LDSF_FWD(1, 1, 1);
     } else {
-        if (precision == 0) LDSF_FWD_G(0, 2, 3); else LDSF_FWD_G(1, 2, 3);
+        if (precision == 0) LDSF_FWD(0, 2, 3); else LDSF_FWD(1, 2, 3);
     }
-#undef LDSF_FWD_G
 #undef LDSF_FWD
     return hipGetLastError();
 }
@@ -1082,26 +1073,18 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
                            int width, int max_nn, int group, hipStream_t s, DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
+    if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    const size_t lds = lds_for(max_nn, true, wt.n_species, group);
-    const dim3 grid(grid_for((n_centers + kW / group - 1) / (kW / group), dc.max_blocks)),
-        block(kW);
-#define LDSF_ADJ(P, NSB, NPB, G)                                                                  \
-    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, G>), grid, block, lds, s, wt, n_centers, centers, \
-                       offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, tmax, dc)
-#define LDSF_ADJ_G(P, NSB, NPB)                                                                   \
-    do {                                                                                          \
-        if (group == 16) LDSF_ADJ(P, NSB, NPB, 16);                                               \
-        else if (group == 32) LDSF_ADJ(P, NSB, NPB, 32);                                          \
-        else LDSF_ADJ(P, NSB, NPB, 64);                                                           \
-    } while (0)
+    const size_t lds = lds_for(max_nn, true, wt.n_species);
+    const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
+#define LDSF_ADJ(P, NSB, NPB)                                                                  \
+    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers, \
+                       offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, dc)
     if (wt.nsb == 1) {
-        if (precision == 0) LDSF_ADJ_G(0, 1, 1); else LDSF_ADJ_G(1, 1, 1);
+        if (precision == 0) LDSF_ADJ(0, 1, 1); else LDSF_ADJ(1, 1, 1);
     } else {
-        if (precision == 0) LDSF_ADJ_G(0, 2, 3); else LDSF_ADJ_G(1, 2, 3);
+        if (precision == 0) LDSF_ADJ(0, 2, 3); else LDSF_ADJ(1, 2, 3);
     }
-#undef LDSF_ADJ_G
 #undef LDSF_ADJ
     return hipGetLastError();
 }

@@ -322,37 +322,6 @@ def test_adjoint_edges_then_gather_equals_gradient():
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("group", [16, 32])
-def test_lane_group_sizes_agree(group, monkeypatch):
-    """16 and 32 lanes per centre (4 and 2 centres per wave) against the
-    default whole-wave groups, both forms."""
-    import torch
-    from libdescriptor_amd import _capi
-    L = small_two_species(500, seed=13)
-    ds = sf_desc(("A", "B"), 5.0, resolved=True)
-    t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device="cuda")  # noqa: E731
-    n = L["n_contrib"]
-    x = t(L["coords"], torch.float64)
-    g = t(np.random.default_rng(9).normal(size=(n, ds.width)), torch.float64)
-    args = (ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]), t(L["numneigh"]))
-    monkeypatch.delenv("LDSF_WAVE_GROUP", raising=False)
-    base = _capi.Plan(*args)
-    monkeypatch.setenv("LDSF_WAVE_GROUP", str(group))
-    grp = _capi.Plan(*args)
-    for prec, tol in ((0, TOL64), (1, TOL32)):
-        out = []
-        for plan in (base, grp):
-            assert plan.kernel_family == 1
-            d = torch.zeros((n, ds.width), dtype=torch.float64, device="cuda")
-            dx = torch.zeros_like(x)
-            plan.compute(x, d, prec)
-            plan.gradient(x, g, dx, prec)
-            torch.cuda.synchronize()
-            out.append((d.cpu().numpy(), dx.cpu().numpy()))
-        assert elem_rel_err(out[1][0], out[0][0], 1e-30) <= (1e-13 if prec == 0 else 1e-6)
-        assert rel_err(out[1][1], out[0][1]) <= (1e-13 if prec == 0 else 1e-6)
-
-
 def test_cpp_drop_in_caller(c1, tmp_path):
     """A reference-style C++ program (tests/cpp/drop_in.cpp) through
     Descriptor::compute / gradient / compute_single_atom on the GPU."""
