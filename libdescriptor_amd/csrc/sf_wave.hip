@@ -173,11 +173,12 @@ __device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slo
 // lands on the wrong side of a column boundary.
 __device__ __forceinline__ void pair_of(int t, int &a, int &b)
 {
-    int c = (int)((1.0f + __builtin_sqrtf((float)(8 * t + 1))) * 0.5f);
-    c -= (c * (c - 1) / 2 > t) ? 1 : 0;
-    c += ((c + 1) * c / 2 <= t) ? 1 : 0;
+    // v_sqrt_f32 (1 ulp) is enough: the two corrections absorb an off-by-one
+    int c = (int)((1.0f + __builtin_amdgcn_sqrtf((float)(8 * t + 1))) * 0.5f);
+    c -= ((c * (c - 1)) >> 1 > t) ? 1 : 0;
+    c += (((c + 1) * c) >> 1 <= t) ? 1 : 0;
     b = c;
-    a = t - c * (c - 1) / 2;
+    a = t - ((c * (c - 1)) >> 1);
 }
 
 // centres of this block: blocks are dealt to the 8 XCDs round-robin (block b
@@ -465,10 +466,10 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
 {
     const int n0 = M - n1;
     base[0] = 0;
-    base[1] = n0 * (n0 - 1) / 2;
+    base[1] = (n0 * (n0 - 1)) >> 1;
     base[2] = base[1] + n0 * n1;
     cnt[0] = cnt[1] = cnt[2] = 0;
-    const int T = M * (M - 1) / 2;
+    const int T = (M * (M - 1)) >> 1;
     for (int t0 = 0; t0 < T; t0 += kW) {
         const int t = t0 + s;
         bool act = false;
@@ -482,7 +483,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
             // only within an ulp of rc_jk, where fc_jk vanishes quadratically
             act = (P == 0) ? (sqrt(rjk2) < S.rc[sa * ns + sb]) : (rjk2 < S.rc2[sa * ns + sb]);
             code = a | (b << 6);
-            if (NPB > 1) blk = pair_index(ns, sa, sb);
+            if (NPB > 1) blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
         }
         if (NPB == 1) {
             const unsigned long long m = __ballot(act);
@@ -561,7 +562,7 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
         }
     } else {
         const double r2a = L.r2[a], r2b = L.r2[b];
-        const float rjkf = __builtin_sqrtf((float)rjk2);
+        const float rjkf = __builtin_amdgcn_sqrtf((float)rjk2);
         t.rjk = rjkf;
         const double h = 0.5 * t.ria * t.rib;
         const float inv = (float)h;
