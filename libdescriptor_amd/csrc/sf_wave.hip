@@ -154,17 +154,16 @@ __device__ __forceinline__ void wave_reduce(double (&v)[K], int s)
     Halve<K, K / 2, kW / 2>::run(v, s);
 }
 
-// every slot of a row is reached exactly once per centre, so `assign` may
-// store the sum instead of adding it
+// sums the K values over the wave and adds them into the centre's row staged
+// in LDS (each slot of a row is reached exactly once per centre)
 template <int K>
-__device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *row,
-                                           bool assign)
+__device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *lrow)
 {
     wave_reduce<K>(v, s);
     constexpr int sh = 6 - Log2<K>::v;
-    if (row && (s & ((1 << sh) - 1)) == 0) {
+    if ((s & ((1 << sh) - 1)) == 0) {
         const int slot = slots[s >> sh];
-        if (slot >= 0) row[slot] = assign ? v[0] : row[slot] + v[0];
+        if (slot >= 0) lrow[slot] += v[0];
     }
 }
 
@@ -301,6 +300,7 @@ struct Shared {
 struct Grp {
     double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
     double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (g: the centre's dE/dzeta row)
+    double *d;                                  // forward only: the centre's descriptor row
     int *sp, *ip;                               // species; adjoint: record position of the edge
     unsigned short *l0;                         // active pairs, one region per species-pair block
 };
@@ -313,7 +313,7 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 __host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint)
 {
     const size_t nd = adjoint ? 11 : 7;
-    size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 + kWaveSlots : 0)) +
+    size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 + kWaveSlots : kWaveSlots)) +
                sizeof(int) * 2 * maxn + sizeof(unsigned short) * tmax;
     return (b + 15) & ~size_t(15);
 }
@@ -349,8 +349,10 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, int maxn, bool 
         L.w = p; p += 16;
         L.wv = p; p += 16;
         L.g = p; p += kWaveSlots;
+        L.d = nullptr;
     } else {
         L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
+        L.d = p; p += kWaveSlots;
     }
     int *ip = reinterpret_cast<int *>(p);
     L.sp = ip; ip += maxn;
@@ -615,12 +617,22 @@ __global__ __launch_bounds__(64, 3) void k_fwd_wave(WaveTab wt, int n_centers,
     if (dc.hdr) n_centers = dc.hdr[0];
     const TileRange tr = tile_range(n_centers);
     if (tr.first >= tr.hi) return;
+    // entry s of a centre's descriptor row before this call: += keeps it
+    // (the reference's semantics), the serving loop's = starts from zero
+    auto row_in = [&](const Ctr &C) -> double {
+        return (!assign && C.work && s < width) ? desc[(size_t)C.c * width + s] : 0.0;
+    };
     // the first centre's loads, in order
     Ctr C = load_centre(tr.first, n_centers, centers, offs, species, x, dc);
+    double rin = row_in(C);
     Nbr B = load_nbr(load_index(C, s, nl), species, x);
     for (int c = tr.first; c < tr.hi; c += tr.step) {
         const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
+        const double rinn = row_in(Cn);
         double *row = C.work ? (dc.out_row ? dc.out_row[c] : desc + (size_t)c * width) : nullptr;
+        // the row is summed in LDS and stored once, coalesced, at the end
+        if (s < kWaveSlots) L.d[s] = rin;
+        __syncthreads();
 
         // ---- pairs ----
         double acc2[NSB * 8];
@@ -668,7 +680,7 @@ __global__ __launch_bounds__(64, 3) void k_fwd_wave(WaveTab wt, int n_centers,
             }
         }
         const int jn = load_index(Cn, s, nl);
-        reduce_add<NSB * 8>(acc2, s, wp.slots, row, assign);
+        reduce_add<NSB * 8>(acc2, s, wp.slots, L.d);
         __syncthreads();
 
         // ---- triplets ----
@@ -717,14 +729,16 @@ __global__ __launch_bounds__(64, 3) void k_fwd_wave(WaveTab wt, int n_centers,
                         }
                     }
                 }
-                reduce_add<16>(acc3, s, wp.slots + 16 + p * 16, row, assign);
+                reduce_add<16>(acc3, s, wp.slots + 16 + p * 16, L.d);
             }
         } else {
             Bn = load_nbr(jn, species, x);
         }
         __syncthreads();
+        if (row && s < width) row[s] = L.d[s];
         C = Cn;
         B = Bn;
+        rin = rinn;
     }
 }
 
