@@ -296,7 +296,7 @@ struct Shared {
     double *rc, *rcinv, *rc2, *rcs;  // rc, 1/rc, rc^2, (pi/2)/rc per species pair (ns^2 each)
 };
 
-// the centre's arrays, sized for maxn neighbours (the pair list for tmax pairs)
+// the centre's arrays: 64 entries per neighbour array, then the pair list
 struct Grp {
     double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
     double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (g: the centre's dE/dzeta row)
@@ -310,11 +310,18 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
     return (sizeof(WaveParams) + 7) / 8 + 4 * (size_t)ns * ns;
 }
 
-__host__ __device__ inline size_t group_bytes(int maxn, int tmax, bool adjoint)
+// The per-neighbour arrays sit at fixed offsets (64 entries each) from one
+// base, so one address per neighbour index serves every array through the
+// LDS instructions' immediate offsets; the pair list (tmax entries) follows.
+// (stride 65, not 64: arrays read or updated together fall in different banks)
+constexpr int kStr = kW + 1;
+constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
+constexpr int kAdjDoubles = 11 * kStr + 16 + 16 + kWaveSlots;  // px..fz, w, wv, g
+
+__host__ __device__ inline size_t group_bytes(int tmax, bool adjoint)
 {
-    const size_t nd = adjoint ? 11 : 7;
-    size_t b = sizeof(double) * (nd * maxn + (adjoint ? 32 + kWaveSlots : kWaveSlots)) +
-               sizeof(int) * 2 * maxn + sizeof(unsigned short) * tmax;
+    size_t b = sizeof(double) * (adjoint ? kAdjDoubles : kFwdDoubles) +
+               sizeof(int) * (adjoint ? 2 : 1) * kW + sizeof(unsigned short) * tmax;
     return (b + 15) & ~size_t(15);
 }
 
@@ -330,34 +337,34 @@ __device__ __forceinline__ Shared carve_shared(double *base, int ns)
     return S;
 }
 
-__device__ __forceinline__ Grp carve_group(double *base, int ns, int maxn, bool adjoint)
+__device__ __forceinline__ Grp carve_group(double *base, int ns, bool adjoint)
 {
     double *p = base + shared_doubles(ns);
     Grp L;
-    L.px = p; p += maxn;
-    L.py = p; p += maxn;
-    L.pz = p; p += maxn;
-    L.r2 = p; p += maxn;
-    L.r = p; p += maxn;
-    L.rinv = p; p += maxn;
-    L.fc = p; p += maxn;
+    L.px = p;
+    L.py = p + kStr;
+    L.pz = p + 2 * kStr;
+    L.r2 = p + 3 * kStr;
+    L.r = p + 4 * kStr;
+    L.rinv = p + 5 * kStr;
+    L.fc = p + 6 * kStr;
     if (adjoint) {
-        L.dfc = p; p += maxn;
-        L.fx = p; p += maxn;
-        L.fy = p; p += maxn;
-        L.fz = p; p += maxn;
-        L.w = p; p += 16;
-        L.wv = p; p += 16;
-        L.g = p; p += kWaveSlots;
+        L.dfc = p + 7 * kStr;
+        L.fx = p + 8 * kStr;
+        L.fy = p + 9 * kStr;
+        L.fz = p + 10 * kStr;
+        L.w = p + 11 * kStr;
+        L.wv = p + 11 * kStr + 16;
+        L.g = p + 11 * kStr + 32;
         L.d = nullptr;
     } else {
         L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
-        L.d = p; p += kWaveSlots;
+        L.d = p + 7 * kStr;
     }
-    int *ip = reinterpret_cast<int *>(p);
-    L.sp = ip; ip += maxn;
-    L.ip = ip; ip += maxn;
-    L.l0 = reinterpret_cast<unsigned short *>(ip);
+    int *ip = reinterpret_cast<int *>(p + (adjoint ? kAdjDoubles : kFwdDoubles));
+    L.sp = ip;
+    L.ip = adjoint ? ip + kW : nullptr;
+    L.l0 = reinterpret_cast<unsigned short *>(ip + (adjoint ? 2 : 1) * kW);
     return L;
 }
 
@@ -594,7 +601,7 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 }
 
 template <int P, int NSB, int NPB>
-__global__ __launch_bounds__(64, 3) void k_fwd_wave(WaveTab wt, int n_centers,
+__global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
                                                  const int *__restrict__ nl,
@@ -609,7 +616,7 @@ __global__ __launch_bounds__(64, 3) void k_fwd_wave(WaveTab wt, int n_centers,
     const int s = threadIdx.x;
     const int ns = wt.n_species;
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, maxn, false);
+    const Grp L = carve_group(lds_base, ns, false);
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
@@ -760,7 +767,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     const int s = threadIdx.x;
     const int ns = wt.n_species;
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, maxn, true);
+    const Grp L = carve_group(lds_base, ns, true);
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
@@ -962,7 +969,7 @@ size_t lds_for(int max_nn, bool adjoint, int ns)
 {
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * shared_doubles(ns) + group_bytes(maxn, tmax, adjoint);
+    return sizeof(double) * shared_doubles(ns) + group_bytes(tmax, adjoint);
 }
 
 }  // namespace
