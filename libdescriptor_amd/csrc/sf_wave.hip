@@ -7,7 +7,7 @@
 //      compacted (list order kept) into LDS arrays; each lane adds its g2
 //      rows into FP64 accumulators.
 //   2. triplet enumeration: the lanes walk all pairs (a < b) of compacted
-//      neighbours in column-major order and compact the pairs with
+//      neighbours along diagonals b - a = 1, 2, ... and compact the pairs with
 //      r_jk < rc_jk into one LDS region per species-pair block.
 //   3. triplet phase: the lanes walk the active pairs only.  cos(theta),
 //      fc_jk, one exp per distinct g4 eta and the zeta powers of each lambda's
@@ -167,17 +167,22 @@ __device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slo
     }
 }
 
-// Pair t of the column-major upper triangle, t = b (b - 1) / 2 + a with
-// a < b: b from the square root, corrected by one where the float estimate
-// lands on the wrong side of a column boundary.
-__device__ __forceinline__ void pair_of(int t, int &a, int &b)
+// Pair t of the M neighbours in diagonal order: diagonal d = b - a holds the
+// M - d pairs (a, a + d) and starts at C(d) = (d - 1)(2M - d) / 2, so d is a
+// root of a quadratic in t, corrected by one where the float estimate lands on
+// the wrong side of a diagonal boundary (v_sqrt_f32's 1 ulp is enough).
+// Along a diagonal the a's and the b's are all distinct, so the lanes of one
+// instruction touch distinct neighbours: conflict-free LDS reads, and the
+// adjoint's LDS accumulations into a and b rarely collide.
+__device__ __forceinline__ void pair_of(int t, int M, int &a, int &b)
 {
-    // v_sqrt_f32 (1 ulp) is enough: the two corrections absorb an off-by-one
-    int c = (int)((1.0f + __builtin_amdgcn_sqrtf((float)(8 * t + 1))) * 0.5f);
-    c -= ((c * (c - 1)) >> 1 > t) ? 1 : 0;
-    c += (((c + 1) * c) >> 1 <= t) ? 1 : 0;
-    b = c;
-    a = t - ((c * (c - 1)) >> 1);
+    const int D = 2 * M + 1;
+    int d = (int)(((float)D - __builtin_amdgcn_sqrtf((float)(D * D - 8 * M - 8 * t))) * 0.5f);
+    d = max(d, 1);
+    d += ((d * (2 * M - d - 1)) >> 1 <= t) ? 1 : 0;         // C(d + 1) <= t
+    d -= (((d - 1) * (2 * M - d)) >> 1 > t) ? 1 : 0;       // C(d) > t
+    a = t - (((d - 1) * (2 * M - d)) >> 1);
+    b = a + d;
 }
 
 // centres of this block: blocks are dealt to the 8 XCDs round-robin (block b
@@ -464,7 +469,7 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
 }
 
 // Phase 2: the active pairs (a < b, r_jk < rc_jk) of the M compacted
-// neighbours, walked in column-major order, compacted into one region of
+// neighbours, walked in diagonal order, compacted into one region of
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
@@ -485,7 +490,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
         int code = 0, blk = 0;
         if (t < T) {
             int a, b;
-            pair_of(t, a, b);
+            pair_of(t, M, a, b);
             const double rjk2 = dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
             const int sa = L.sp[a], sb = L.sp[b];
             // the reference tests sqrt(r_jk^2) < rc_jk; the squared test differs
