@@ -304,7 +304,8 @@ struct Shared {
 // the centre's arrays: 64 entries per neighbour array, then the pair list
 struct Grp {
     double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
-    double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (g: the centre's dE/dzeta row)
+    double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (w, wv: row weights of up to
+                                                // 3 blocks; g: the centre's dE/dzeta row)
     double *d;                                  // forward only: the centre's descriptor row
     int *sp, *ip;                               // species; adjoint: record position of the edge
     unsigned short *l0;                         // active pairs, one region per species-pair block
@@ -321,7 +322,7 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 // (stride 65, not 64: arrays read or updated together fall in different banks)
 constexpr int kStr = kW + 1;
 constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
-constexpr int kAdjDoubles = 11 * kStr + 16 + 16 + kWaveSlots;  // px..fz, w, wv, g
+constexpr int kAdjDoubles = 11 * kStr + 48 + 48 + kWaveSlots;  // px..fz, w, wv, g
 
 __host__ __device__ inline size_t group_bytes(int tmax, bool adjoint)
 {
@@ -359,8 +360,8 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, bool adjoint)
         L.fy = p + 9 * kStr;
         L.fz = p + 10 * kStr;
         L.w = p + 11 * kStr;
-        L.wv = p + 11 * kStr + 16;
-        L.g = p + 11 * kStr + 32;
+        L.wv = p + 11 * kStr + 48;
+        L.g = p + 11 * kStr + 96;
         L.d = nullptr;
     } else {
         L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
@@ -857,85 +858,86 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         const int ipen = s < Cn.N ? inv_pos[Cn.beg + s] : 0;
         __syncthreads();
 
-        // ---- triplets ----
+        // ---- triplets: one pass over the active pairs of every block ----
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
             enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x);
+            // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda,
+            // kept in the compute type (a loop-invariant copy costs half the registers)
+            if (s < 16 * NPB) {
+                const int k = s & 15, z = k >> 2, l = (k >> 1) & 1;
+                const int slot = wp.slots[16 + s];
+                const double ww = (C.work && slot >= 0) ? L.g[slot] * wp.coef[z] : 0.0;
+                reinterpret_cast<T *>(L.w)[s] = (T)ww;
+                reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
+            }
             __syncthreads();
+            const int c01 = cnt[0] + cnt[1], nall = c01 + cnt[2];
+            for (int t0 = 0; t0 < nall; t0 += kW) {
+                const int t = t0 + s;
+                if (t < nall) {
+                    // position t of the concatenated block lists
+                    const int p = t < cnt[0] ? 0 : (t < c01 ? 1 : 2);
+                    const int code = L.l0[p == 0 ? base[0] + t
+                                          : p == 1 ? base[1] + (t - cnt[0])
+                                                   : base[2] + (t - c01)];
+                    const T *w = reinterpret_cast<const T *>(L.w) + 16 * p;
+                    const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * p;
+                    const int a = code & 63, b = (code >> 6) & 63;
+                    Trip<P> q;
+                    triplet<P, true>(S, wp, wt, L, ns, a, b, q);
+                    const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
+                    T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
-            for (int p = 0; p < NPB; ++p) {
-                const int nb = cnt[p];
-                const unsigned short *lst = L.l0 + base[p];
-                // row weights of this block: dE/dzeta 2^(1-zeta), and times zeta lambda
-                if (s < 16) {
-                    const int z = s >> 2, l = (s >> 1) & 1;
-                    const int slot = wp.slots[16 + p * 16 + s];
-                    const double ww = (C.work && slot >= 0) ? L.g[slot] * wp.coef[z] : 0.0;
-                    // kept in the compute type: a loop-invariant copy costs half the registers
-                    reinterpret_cast<T *>(L.w)[s] = (T)ww;
-                    reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
-                }
-                __syncthreads();
-                for (int t0 = 0; t0 < nb; t0 += kW) {
-                    const int t = t0 + s;
-                    if (t < nb) {
-                        const int code = lst[t];
-                        const int a = code & 63, b = (code >> 6) & 63;
-                        Trip<P> q;
-                        triplet<P, true>(S, wp, wt, L, ns, a, b, q);
-                        const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
-                        T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
+                    for (int l = 0; l < 2; ++l) {
+                        if (P == 1 || l < wt.nl) {
+                            T pm[4];
+                            powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
 #pragma unroll
-                        for (int l = 0; l < 2; ++l) {
-                            if (P == 1 || l < wt.nl) {
-                                T pm[4];
-                                powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
+                            for (int z = 0; z < 4; ++z) {
+                                if (P == 1 || z < wt.nz) {
+                                    const T pm1 = pm[z];
+                                    const T pw = pm1 * q.B[l];
 #pragma unroll
-                                for (int z = 0; z < 4; ++z) {
-                                    if (P == 1 || z < wt.nz) {
-                                        const T pm1 = pm[z];
-                                        const T pw = pm1 * q.B[l];
-#pragma unroll
-                                        for (int e = 0; e < 2; ++e) {
-                                            U[e] += reinterpret_cast<const T *>(L.w)[(z * 2 + l) * 2 + e] * pw;
-                                            V[e] += reinterpret_cast<const T *>(L.wv)[(z * 2 + l) * 2 + e] * pm1;
-                                        }
+                                    for (int e = 0; e < 2; ++e) {
+                                        U[e] += w[(z * 2 + l) * 2 + e] * pw;
+                                        V[e] += wv[(z * 2 + l) * 2 + e] * pm1;
                                     }
                                 }
                             }
                         }
-                        T SA = (T)0, SB = (T)0, SC = (T)0;
-#pragma unroll
-                        for (int e = 0; e < 2; ++e) {
-                            if (P == 1 || e < wt.ne) {
-                                SA += V[e] * q.E[e];
-                                const T ue = U[e] * q.E[e];
-                                SB += ue;
-                                SC += (T)wp.eta3[e] * ue;
-                            }
-                        }
-                        // d cos(theta) / d r_ij, d r_ik, d r_jk
-                        const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
-                        const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
-                        const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
-                        const T fc3 = q.fa * q.fb * q.fjk;
-                        const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
-                        const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
-                        const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
-                        const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
-                                     a3 = (double)Ajk * q.rjkinv;
-                        atomicAdd(&L.fx[a], a1 * (L.px[a] - C.x0) - a3 * q.e0);
-                        atomicAdd(&L.fy[a], a1 * (L.py[a] - C.x1) - a3 * q.e1);
-                        atomicAdd(&L.fz[a], a1 * (L.pz[a] - C.x2) - a3 * q.e2);
-                        atomicAdd(&L.fx[b], a2 * (L.px[b] - C.x0) + a3 * q.e0);
-                        atomicAdd(&L.fy[b], a2 * (L.py[b] - C.x1) + a3 * q.e1);
-                        atomicAdd(&L.fz[b], a2 * (L.pz[b] - C.x2) + a3 * q.e2);
                     }
+                    T SA = (T)0, SB = (T)0, SC = (T)0;
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        if (P == 1 || e < wt.ne) {
+                            SA += V[e] * q.E[e];
+                            const T ue = U[e] * q.E[e];
+                            SB += ue;
+                            SC += (T)wp.eta3[e] * ue;
+                        }
+                    }
+                    // d cos(theta) / d r_ij, d r_ik, d r_jk
+                    const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
+                    const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
+                    const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
+                    const T fc3 = q.fa * q.fb * q.fjk;
+                    const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
+                    const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
+                    const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
+                    const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
+                                 a3 = (double)Ajk * q.rjkinv;
+                    atomicAdd(&L.fx[a], a1 * (L.px[a] - C.x0) - a3 * q.e0);
+                    atomicAdd(&L.fy[a], a1 * (L.py[a] - C.x1) - a3 * q.e1);
+                    atomicAdd(&L.fz[a], a1 * (L.pz[a] - C.x2) - a3 * q.e2);
+                    atomicAdd(&L.fx[b], a2 * (L.px[b] - C.x0) + a3 * q.e0);
+                    atomicAdd(&L.fy[b], a2 * (L.py[b] - C.x1) + a3 * q.e1);
+                    atomicAdd(&L.fz[b], a2 * (L.pz[b] - C.x2) + a3 * q.e2);
                 }
-                __syncthreads();
             }
+            __syncthreads();
         } else {
             Bn = load_nbr(jn, species, x);
         }
