@@ -703,16 +703,33 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x);
             __syncthreads();
-#pragma unroll
-            for (int p = 0; p < NPB; ++p) {
-                const int nb = cnt[p];
-                const unsigned short *lst = L.l0 + base[p];
+            // the lanes are split over the species-pair blocks in proportion to
+            // their active pairs: the fewest passes I with sum_p ceil(cnt_p / I)
+            // <= 64 lanes; lane s serves block p with stride lanes(p)
+            const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
+            int passes = (nall + kW - 1) / kW, ln[3] = {kW, 0, 0};
+            if (NPB > 1 && nall > 0) {
+                auto lanes_for = [&](int I) {
+                    ln[0] = (cnt[0] + I - 1) / I;
+                    ln[1] = (cnt[1] + I - 1) / I;
+                    ln[2] = (cnt[2] + I - 1) / I;
+                    return ln[0] + ln[1] + ln[2];
+                };
+                while (lanes_for(passes) > kW) ++passes;
+            }
+            const int pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
+            const int u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
+            const int stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
+            const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
+            const unsigned short *lst =
+                L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
+            {
                 double acc3[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
-                for (int t0 = 0; t0 < nb; t0 += kW) {
-                    const int t = t0 + s;
-                    if (t < nb) {
+                for (int i = 0; i < passes; ++i) {
+                    const int t = u + i * stride;
+                    if (u < stride && t < nb) {
                         const int code = lst[t];
                         Trip<P> q;
                         triplet<P, false>(S, wp, wt, L, ns, code & 63, (code >> 6) & 63, q);
@@ -742,7 +759,18 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                         }
                     }
                 }
-                reduce_add<16>(acc3, s, wp.slots + 16 + p * 16, L.d);
+                if (NPB == 1) {
+                    reduce_add<16>(acc3, s, wp.slots + 16, L.d);
+                } else {
+#pragma unroll
+                    for (int p = 0; p < NPB; ++p) {
+                        if (cnt[p] == 0) continue;
+                        double v[16];
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) v[k] = pl == p ? acc3[k] : 0.0;
+                        reduce_add<16>(v, s, wp.slots + 16 + p * 16, L.d);
+                    }
+                }
             }
         } else {
             Bn = load_nbr(jn, species, x);
