@@ -52,9 +52,12 @@ template <> struct Log2<4> { static constexpr int v = 2; };
 template <> struct Log2<8> { static constexpr int v = 3; };
 template <> struct Log2<16> { static constexpr int v = 4; };
 
-__device__ __forceinline__ int below(unsigned long long m, int s)
+// bits of the wave mask m set in the lanes below this one (v_mbcnt: the lane
+// is implicit; every caller's s is its lane)
+__device__ __forceinline__ int below(unsigned long long m, int)
 {
-    return __popcll(m & ((1ull << s) - 1ull));
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 // Cross-lane exchanges of the reduction butterfly, for partner bit M.
@@ -167,6 +170,13 @@ __device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slo
     }
 }
 
+// species-pair index a * ns + b (ns is small: v_mul_u32_u24, or a shift
+// where ns is a compile-time 2)
+__device__ __forceinline__ int pidx(int a, int b, int ns)
+{
+    return (int)__umul24(a, ns) + b;
+}
+
 // Pair t of the M neighbours in diagonal order: diagonal d = b - a holds the
 // M - d pairs (a, a + d) and starts at C(d) = (d - 1)(2M - d) / 2, so d is a
 // root of a quadratic in t, corrected by one where the float estimate lands on
@@ -176,12 +186,13 @@ __device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slo
 // adjoint's LDS accumulations into a and b rarely collide.
 __device__ __forceinline__ void pair_of(int t, int M, int &a, int &b)
 {
+    // every product is below 2^24 (M <= 64): v_mul_u32_u24 at full rate
     const int D = 2 * M + 1;
     int d = (int)(((float)D - __builtin_amdgcn_sqrtf((float)(D * D - 8 * M - 8 * t))) * 0.5f);
     d = max(d, 1);
-    d += ((d * (2 * M - d - 1)) >> 1 <= t) ? 1 : 0;         // C(d + 1) <= t
-    d -= (((d - 1) * (2 * M - d)) >> 1 > t) ? 1 : 0;       // C(d) > t
-    a = t - (((d - 1) * (2 * M - d)) >> 1);
+    d += ((int)__umul24(d, 2 * M - d - 1) >> 1 <= t) ? 1 : 0;   // C(d + 1) <= t
+    d -= ((int)__umul24(d - 1, 2 * M - d) >> 1 > t) ? 1 : 0;    // C(d) > t
+    a = t - ((int)__umul24(d - 1, 2 * M - d) >> 1);
     b = a + d;
 }
 
@@ -496,7 +507,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
             const int sa = L.sp[a], sb = L.sp[b];
             // the reference tests sqrt(r_jk^2) < rc_jk; the squared test differs
             // only within an ulp of rc_jk, where fc_jk vanishes quadratically
-            act = (P == 0) ? (sqrt(rjk2) < S.rc[sa * ns + sb]) : (rjk2 < S.rc2[sa * ns + sb]);
+            act = (P == 0) ? (sqrt(rjk2) < S.rc[pidx(sa, sb, ns)]) : (rjk2 < S.rc2[pidx(sa, sb, ns)]);
             code = a | (b << 6);
             if (NPB > 1) blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
         }
@@ -509,10 +520,10 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
             const unsigned long long m1 = __ballot(act && blk == 1);
             const unsigned long long m2 = __ballot(act && blk == 2);
             if (act) {
-                const int pos = blk == 0 ? base[0] + cnt[0] + below(m0, s)
-                              : blk == 1 ? base[1] + cnt[1] + below(m1, s)
-                                         : base[2] + cnt[2] + below(m2, s);
-                L.l0[pos] = (unsigned short)code;
+                const int p0 = base[0] + cnt[0] + below(m0, s);
+                const int p1 = base[1] + cnt[1] + below(m1, s);
+                const int p2 = base[2] + cnt[2] + below(m2, s);
+                L.l0[blk == 0 ? p0 : (blk == 1 ? p1 : p2)] = (unsigned short)code;
             }
             cnt[0] += __popcll(m0);
             cnt[1] += __popcll(m1);
@@ -539,7 +550,7 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 {
     using T = typename Real<P>::T;
     const Real<P> R;
-    const int k = L.sp[a] * ns + L.sp[b];
+    const int k = pidx(L.sp[a], L.sp[b], ns);
     const double rc = S.rc[k], rcinv = S.rcinv[k];
     t.ra = L.r[a];
     t.rb = L.r[b];
@@ -620,7 +631,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
     const Real<P> R;
     extern __shared__ double lds_base[];
     const int s = threadIdx.x;
-    const int ns = wt.n_species;
+    const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
     const Shared S = carve_shared(lds_base, ns);
     const Grp L = carve_group(lds_base, ns, false);
     stage_shared(S, wt, s);
@@ -660,14 +671,14 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             d2 = B.p2 - C.x2;
             r2 = dist2(d0, d1, d2);
             r = sqrt(r2);
-            valid = r < S.rc[C.si * ns + sj];  // r > rc is skipped; r == rc adds exactly 0
+            valid = r < S.rc[pidx(C.si, sj, ns)];  // r > rc is skipped; r == rc adds exactly 0
         }
         const unsigned long long m = __ballot(valid);
         const int M = __popcll(m);
         const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
         if (valid) {
             const int pos = below(m, s);
-            const int k = C.si * ns + sj;
+            const int k = pidx(C.si, sj, ns);
             T fc, dfc;
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
             L.px[pos] = B.p0;
@@ -703,33 +714,39 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x);
             __syncthreads();
-            // the lanes are split over the species-pair blocks in proportion to
-            // their active pairs: the fewest passes I with sum_p ceil(cnt_p / I)
-            // <= 64 lanes; lane s serves block p with stride lanes(p)
+            // the blocks are dealt columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48;
+            // column s & 15) in proportion to their active pairs: the fewest
+            // passes I with sum_p ceil(cnt_p / 4I) <= 16 columns.  Lane s serves
+            // block p with stride 4 * cols(p); a column's 4 lanes sum with two
+            // register swaps, the columns of a block in LDS.
             const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
-            int passes = (nall + kW - 1) / kW, ln[3] = {kW, 0, 0};
+            int passes = (nall + kW - 1) / kW, nc[3] = {16, 0, 0};
             if (NPB > 1 && nall > 0) {
-                auto lanes_for = [&](int I) {
-                    ln[0] = (cnt[0] + I - 1) / I;
-                    ln[1] = (cnt[1] + I - 1) / I;
-                    ln[2] = (cnt[2] + I - 1) / I;
-                    return ln[0] + ln[1] + ln[2];
+                auto cols_for = [&](int I) {
+                    nc[0] = (cnt[0] + 4 * I - 1) / (4 * I);
+                    nc[1] = (cnt[1] + 4 * I - 1) / (4 * I);
+                    nc[2] = (cnt[2] + 4 * I - 1) / (4 * I);
+                    return nc[0] + nc[1] + nc[2];
                 };
-                while (lanes_for(passes) > kW) ++passes;
+                while (cols_for(passes) > 16) ++passes;
             }
-            const int pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
-            const int u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
-            const int stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
+            const int col = s & 15, rowq = s >> 4;
+            const int pl = NPB == 1 ? 0 : (col < nc[0] ? 0 : (col < nc[0] + nc[1] ? 1 : 2));
+            const int cs = pl == 0 ? 0 : (pl == 1 ? nc[0] : nc[0] + nc[1]);
+            const int ncol = pl == 0 ? nc[0] : (pl == 1 ? nc[1] : nc[2]);
+            const int u = NPB == 1 ? s : (col - cs) + rowq * ncol;
+            const int stride = NPB == 1 ? kW : 4 * ncol;
             const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
             const unsigned short *lst =
                 L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
+            const bool mine = NPB == 1 || col < cs + ncol;
             {
                 double acc3[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
                 for (int i = 0; i < passes; ++i) {
                     const int t = u + i * stride;
-                    if (u < stride && t < nb) {
+                    if (mine && t < nb) {
                         const int code = lst[t];
                         Trip<P> q;
                         triplet<P, false>(S, wp, wt, L, ns, code & 63, (code >> 6) & 63, q);
@@ -762,13 +779,33 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 if (NPB == 1) {
                     reduce_add<16>(acc3, s, wp.slots + 16, L.d);
                 } else {
+                    // the column's 4 lanes: lane s keeps values 4 (s >> 4) .. +3
 #pragma unroll
-                    for (int p = 0; p < NPB; ++p) {
-                        if (cnt[p] == 0) continue;
-                        double v[16];
+                    for (int q = 0; q < 8; ++q) {
+                        double a = acc3[q], b = acc3[q + 8];
+                        xswap<32>(a, b);
+                        acc3[q] = a + b;
+                    }
 #pragma unroll
-                        for (int k = 0; k < 16; ++k) v[k] = pl == p ? acc3[k] : 0.0;
-                        reduce_add<16>(v, s, wp.slots + 16 + p * 16, L.d);
+                    for (int q = 0; q < 4; ++q) {
+                        double a = acc3[q], b = acc3[q + 4];
+                        xswap<16>(a, b);
+                        acc3[q] = a + b;
+                    }
+                    // the neighbour arrays are dead here: column c's 16 sums
+                    // at buf[16 c ..], then lane (p, k) = (s >> 4, s & 15) adds
+                    // its block's columns in order
+                    double *buf = L.px;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) buf[16 * col + 4 * rowq + q] = acc3[q];
+                    __syncthreads();
+                    if (rowq < NPB) {
+                        const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
+                        const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
+                        double sum = 0.0;
+                        for (int c = c0; c < c1; ++c) sum += buf[16 * c + col];
+                        const int slot = wp.slots[16 + s];
+                        if (c1 > c0 && slot >= 0) L.d[slot] += sum;
                     }
                 }
             }
@@ -799,7 +836,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     const Real<P> R;
     extern __shared__ double lds_base[];
     const int s = threadIdx.x;
-    const int ns = wt.n_species;
+    const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
     const Shared S = carve_shared(lds_base, ns);
     const Grp L = carve_group(lds_base, ns, true);
     stage_shared(S, wt, s);
@@ -840,7 +877,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
             d2 = B.p2 - C.x2;
             r2 = dist2(d0, d1, d2);
             r = sqrt(r2);
-            valid = r < S.rc[C.si * ns + sj];
+            valid = r < S.rc[pidx(C.si, sj, ns)];
             if (!valid) {
                 double *ej = edge + 3 * (size_t)ipe;
                 ej[0] = 0.0;
@@ -853,7 +890,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
         if (valid) {
             const int pos = below(m, s);
-            const int k = C.si * ns + sj;
+            const int k = pidx(C.si, sj, ns);
             T fc, dfc;
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
             double sum = 0.0;
