@@ -324,7 +324,8 @@ struct Grp {
 
 __host__ __device__ constexpr size_t shared_doubles(int ns)
 {
-    return (sizeof(WaveParams) + 7) / 8 + 4 * (size_t)ns * ns;
+    // an even count: the neighbour arrays that follow hold 16-byte vectors
+    return ((sizeof(WaveParams) + 15) / 16) * 2 + 4 * (size_t)ns * ns;
 }
 
 // The per-neighbour arrays sit at fixed offsets (64 entries each) from one
@@ -480,6 +481,149 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
     dfc = -(float)(2.0 * rcs) * s * c;
 }
 
+// ---- the FP32-bulk triplet path ----
+//
+// Per compacted neighbour a the pair phase stores, in the space of the FP64
+// neighbour arrays (which this form does not use):
+//   D[a]       = (x_a - x_i, y_a - y_i)              double2
+//   D[64 + a]  = (z_a - z_i, r_a)                     double2
+//   V[a]       forward: (sqrt(1/2) / r_a, g_0, g_1, species bits)   float4
+//              with g_e = fc(r_a) exp(-eta_e r_a^2)
+//              adjoint: (1 / r_a, fc(r_a), dfc/dr(r_a), species bits)
+//   W[a]       adjoint: (exp(-eta_0 r_a^2), exp(-eta_1 r_a^2))     float2
+// r_jk^2 and the base 1 + lambda cos(theta) are formed in FP64 (the base's
+// cancellation-free numerator would lose ~1e-6 absolute from FP32
+// displacements, which zeta = 16 turns into 1e-5 relative error of a small
+// row); everything after them -- exp, log, powers, cutoff, products -- is
+// FP32.  exp(-eta (r_ij^2 + r_ik^2 + r_jk^2)) factors into the two
+// neighbours' stored terms and one exp of r_jk.
+__device__ __forceinline__ float uniform_f(double v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)v)));
+}
+
+__device__ __forceinline__ double uniform_d(double v)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// wave-uniform parameters (scalar registers) of the FP32-bulk path
+struct Fast {
+    float zeta[4];
+    double lp[2], lm[2], lam[2];  // the base's numerator is FP64
+    float eta[2];                 // eta_e
+    float etal[2];                // eta_e log2(e)
+    // rc, rc^2, (pi/2)/rc per species-pair block (two species), else block 0;
+    // separate scalars, so that a per-lane choice stays a register select
+    float rc0, rc1, rc2, rs0, rs1, rs2;
+    double rq0, rq1, rq2;
+};
+
+// the value of block p (0, 1 or 2)
+template <typename V>
+__device__ __forceinline__ V blk3(int p, V v0, V v1, V v2)
+{
+    return p == 0 ? v0 : (p == 1 ? v1 : v2);
+}
+
+template <int NPB>
+__device__ __forceinline__ Fast load_fast(const Shared &S, const WaveParams &wp)
+{
+    Fast F;
+#pragma unroll
+    for (int z = 0; z < 4; ++z) F.zeta[z] = uniform_f(wp.zeta[z]);
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        F.lp[l] = uniform_d(wp.lp[l]);
+        F.lm[l] = uniform_d(wp.lm[l]);
+        F.lam[l] = uniform_d(wp.lam[l]);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        F.eta[e] = uniform_f(wp.eta3[e]);
+        F.etal[e] = uniform_f(wp.eta3[e] * 1.4426950408889634);
+    }
+    // blocks 0-0, 0-1, 1-1 of two species: pair indices 0, 1, 3
+    const int k1 = NPB > 1 ? 1 : 0, k2 = NPB > 1 ? 3 : 0;
+    F.rc0 = uniform_f(S.rc[0]);
+    F.rc1 = uniform_f(S.rc[k1]);
+    F.rc2 = uniform_f(S.rc[k2]);
+    F.rq0 = uniform_d(S.rc2[0]);
+    F.rq1 = uniform_d(S.rc2[k1]);
+    F.rq2 = uniform_d(S.rc2[k2]);
+    F.rs0 = uniform_f(S.rcs[0]);
+    F.rs1 = uniform_f(S.rcs[k1]);
+    F.rs2 = uniform_f(S.rcs[k2]);
+    return F;
+}
+
+// sin(y) and cos(y) on [0, pi/2] from their Taylor series to y^13 / y^12 in
+// Horner form (truncation below 7e-9)
+__device__ __forceinline__ float sin_q(float y, float y2)
+{
+    float t = fmaf(y2, 1.0f / 6227020800.0f, -1.0f / 39916800.0f);
+    t = fmaf(y2, t, 1.0f / 362880.0f);
+    t = fmaf(y2, t, -1.0f / 5040.0f);
+    t = fmaf(y2, t, 1.0f / 120.0f);
+    t = fmaf(y2, t, -1.0f / 6.0f);
+    return fmaf(y * y2, t, y);
+}
+
+__device__ __forceinline__ float cos_q(float y2)
+{
+    float t = fmaf(y2, 1.0f / 479001600.0f, -1.0f / 3628800.0f);
+    t = fmaf(y2, t, 1.0f / 40320.0f);
+    t = fmaf(y2, t, -1.0f / 720.0f);
+    t = fmaf(y2, t, 1.0f / 24.0f);
+    t = fmaf(y2, t, -0.5f);
+    return fmaf(y2, t, 1.0f);
+}
+
+// |e|^2 of e = d_b - d_a; the enumeration and the triplet phase evaluate the
+// same expression, so an active pair has r_jk^2 < rc^2 in both
+__device__ __forceinline__ double rjk2_of(const double2 &a0, const double2 &a1, const double2 &b0,
+                                          const double2 &b1, double &ex, double &ey, double &ez)
+{
+    ex = b0.x - a0.x;
+    ey = b0.y - a0.y;
+    ez = b1.x - a1.x;
+    return fma(ez, ez, fma(ey, ey, ex * ex));
+}
+
+// geometry of an active pair (a, b) of the FP32-bulk path
+struct TripF {
+    double ex, ey, ez;  // d_b - d_a
+    double rjk2d;
+    float rjk2, rjk, rjkinv;
+    float sn, y;        // sin(y), y = (pi/2)(rc - r_jk)/rc: fc_jk = sin^2 y
+};
+
+__device__ __forceinline__ void geom_f(const double2 &a0, const double2 &a1, const double2 &b0,
+                                       const double2 &b1, float rc, double rc2, float rcs,
+                                       TripF &t)
+{
+    t.rjk2d = rjk2_of(a0, a1, b0, b1, t.ex, t.ey, t.ez);
+    t.rjk2 = (float)t.rjk2d;
+    t.rjkinv = __builtin_amdgcn_rsqf(t.rjk2);
+    t.rjk = t.rjk2 * t.rjkinv;
+    // rc - r_jk = (rc^2 - r_jk^2) / (rc + r_jk) keeps its relative accuracy near rc
+    t.y = (float)(rc2 - t.rjk2d) * __builtin_amdgcn_rcpf(rc + t.rjk) * rcs;
+    t.sn = sin_q(t.y, t.y * t.y);
+}
+
+// the base 1 + lambda_l cos(theta) = (lp (r_ij + r_ik)^2 - lm (r_ij - r_ik)^2 -
+// lambda r_jk^2) / (2 r_ij r_ik), numerator in FP64; a base <= 0 (rounding of
+// an exactly collinear triplet) becomes 1e-30, whose powers vanish
+__device__ __forceinline__ float base_f(const Fast &F, int l, double sp, double sm, double rjk2,
+                                        float inv)
+{
+    const double num = fma(-F.lam[l], rjk2, fma(F.lp[l], sp, -F.lm[l] * sm));
+    return fmaxf((float)num * inv, 1e-30f);
+}
+
 // Phase 2: the active pairs (a < b, r_jk < rc_jk) of the M compacted
 // neighbours, walked in diagonal order, compacted into one region of
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
@@ -487,8 +631,9 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
 // n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.
 template <int P, int NPB>
-__device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, int ns, int M,
-                                                int n1, int s, int (&cnt)[3], int (&base)[3])
+__device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, const Fast &F,
+                                                int ns, int M, int n1, int s, int (&cnt)[3],
+                                                int (&base)[3])
 {
     const int n0 = M - n1;
     base[0] = 0;
@@ -503,11 +648,28 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, i
         if (t < T) {
             int a, b;
             pair_of(t, M, a, b);
-            const double rjk2 = dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
-            const int sa = L.sp[a], sb = L.sp[b];
-            // the reference tests sqrt(r_jk^2) < rc_jk; the squared test differs
-            // only within an ulp of rc_jk, where fc_jk vanishes quadratically
-            act = (P == 0) ? (sqrt(rjk2) < S.rc[pidx(sa, sb, ns)]) : (rjk2 < S.rc2[pidx(sa, sb, ns)]);
+            int sa, sb;
+            if constexpr (P == 0) {
+                const double rjk2 =
+                    dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
+                sa = L.sp[a];
+                sb = L.sp[b];
+                act = sqrt(rjk2) < S.rc[pidx(sa, sb, ns)];
+            } else {
+                // the triplet phase forms the same r_jk^2 from the same operands:
+                // an active pair has r_jk^2 < rc^2 there too.  (The reference
+                // tests r_jk < rc_jk; the two differ only within an ulp of rc_jk,
+                // where fc_jk vanishes quadratically.)
+                const double2 *D = reinterpret_cast<const double2 *>(L.px);
+                const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
+                double ex, ey, ez;
+                const double rjk2 = rjk2_of(D[a], D[kW + a], D[b], D[kW + b], ex, ey, ez);
+                sa = __float_as_int(V[a].w);
+                sb = __float_as_int(V[b].w);
+                const double rc2 = NPB > 1 ? blk3(sa + sb, F.rq0, F.rq1, F.rq2)
+                                           : (ns == 1 ? F.rq0 : S.rc2[pidx(sa, sb, ns)]);
+                act = rjk2 < rc2;
+            }
             code = a | (b << 6);
             if (NPB > 1) blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
         }
@@ -617,6 +779,76 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
     }
 }
 
+// One active triplet of the FP32-bulk adjoint: the derivatives of the row
+// sums with respect to r_ij, r_ik and r_jk, turned into vectors and added
+// into the two neighbours' FP64 LDS accumulators.  w / wv: the block's row
+// weights dE/dzeta 2^(1-zeta) and those times zeta lambda.
+template <int NPB>
+__device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, const Grp &L, int ns,
+                                              float rc, double rc2, float rcs, int a, int b,
+                                              const float *w, const float *wv)
+{
+    const double2 *D = reinterpret_cast<const double2 *>(L.px);
+    const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
+    const float2 *W = reinterpret_cast<const float2 *>(V + kW);
+    const double2 a0 = D[a], a1 = D[kW + a], b0 = D[b], b1 = D[kW + b];
+    const float4 va = V[a], vb = V[b];
+    const float2 ea = W[a], eb = W[b];
+    if (NPB == 1 && ns > 1) {  // one block, cutoff per species pair
+        const int k = pidx(__float_as_int(va.w), __float_as_int(vb.w), ns);
+        rc = (float)S.rc[k];
+        rc2 = S.rc2[k];
+        rcs = (float)S.rcs[k];
+    }
+    TripF g;
+    geom_f(a0, a1, b0, b1, rc, rc2, rcs, g);
+    const float cs = cos_q(g.y * g.y);
+    const float fjk = g.sn * g.sn, dfjk = -2.0f * rcs * g.sn * cs;
+    const float ra = (float)a1.y, rb = (float)b1.y, ria = va.x, rib = vb.x;
+    const float fa = va.y, fb = vb.y, dfa = va.z, dfb = vb.z;
+    const float inv = 0.5f * ria * rib;  // 1 / (2 r_ij r_ik)
+    const double sp = (a1.y + b1.y) * (a1.y + b1.y), sm = (a1.y - b1.y) * (a1.y - b1.y);
+    float E[2];
+    E[0] = ea.x * eb.x * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2);
+    E[1] = ea.y * eb.y * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2);
+    float U[2] = {0.0f, 0.0f}, Vs[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
+        const float lb = __builtin_amdgcn_logf(bl);
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+            const float pm = __builtin_amdgcn_exp2f((F.zeta[z] - 1.0f) * lb);  // B^(zeta - 1)
+            const float pw = pm * bl;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                U[e] = fmaf(w[(z * 2 + l) * 2 + e], pw, U[e]);
+                Vs[e] = fmaf(wv[(z * 2 + l) * 2 + e], pm, Vs[e]);
+            }
+        }
+    }
+    const float ue0 = U[0] * E[0], ue1 = U[1] * E[1];
+    const float SA = fmaf(Vs[0], E[0], Vs[1] * E[1]);
+    const float SB = ue0 + ue1;
+    const float SC = fmaf(F.eta[0], ue0, F.eta[1] * ue1);
+    // d cos(theta) / d r_ij = (r_ij^2 - r_ik^2 + r_jk^2) / (2 r_ij^2 r_ik), whose
+    // numerator is -2 d_a . (d_b - d_a); likewise for r_ik with 2 d_b . (d_b - d_a)
+    const float nij = (float)(-2.0 * fma(a1.x, g.ez, fma(a0.y, g.ey, a0.x * g.ex)));
+    const float nik = (float)(2.0 * fma(b1.x, g.ez, fma(b0.y, g.ey, b0.x * g.ex)));
+    const float dcij = nij * inv * ria, dcik = nik * inv * rib, dcjk = -2.0f * g.rjk * inv;
+    const float fc3 = fa * fb * fjk;
+    const float Aij = fmaf(fc3, fmaf(dcij, SA, -2.0f * ra * SC), fb * fjk * dfa * SB);
+    const float Aik = fmaf(fc3, fmaf(dcik, SA, -2.0f * rb * SC), fa * fjk * dfb * SB);
+    const float Ajk = fmaf(fc3, fmaf(dcjk, SA, -2.0f * g.rjk * SC), fa * fb * dfjk * SB);
+    const double a1c = (double)(Aij * ria), a2c = (double)(Aik * rib), a3c = (double)(Ajk * g.rjkinv);
+    atomicAdd(&L.fx[a], fma(a1c, a0.x, -a3c * g.ex));
+    atomicAdd(&L.fy[a], fma(a1c, a0.y, -a3c * g.ey));
+    atomicAdd(&L.fz[a], fma(a1c, a1.x, -a3c * g.ez));
+    atomicAdd(&L.fx[b], fma(a2c, b0.x, a3c * g.ex));
+    atomicAdd(&L.fy[b], fma(a2c, b0.y, a3c * g.ey));
+    atomicAdd(&L.fz[b], fma(a2c, b1.x, a3c * g.ez));
+}
+
 template <int P, int NSB, int NPB>
 __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
@@ -637,6 +869,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
+    const Fast F = P == 1 ? load_fast<NPB>(S, wp) : Fast();
     const bool assign = dc.out_row != nullptr;
     if (dc.hdr) n_centers = dc.hdr[0];
     const TileRange tr = tile_range(n_centers);
@@ -681,14 +914,24 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             const int k = pidx(C.si, sj, ns);
             T fc, dfc;
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
-            L.px[pos] = B.p0;
-            L.py[pos] = B.p1;
-            L.pz[pos] = B.p2;
-            L.r2[pos] = r2;
-            L.r[pos] = r;
-            L.rinv[pos] = 1.0 / r;
-            L.fc[pos] = (double)fc;
-            L.sp[pos] = sj;
+            if constexpr (P == 1) {
+                double2 *D = reinterpret_cast<double2 *>(L.px);
+                float4 *V = reinterpret_cast<float4 *>(D + 2 * kW);
+                D[pos] = make_double2(d0, d1);
+                D[kW + pos] = make_double2(d2, r);
+                V[pos] = make_float4((float)(0.70710678118654752 / r),
+                                     (float)fc * dexp(-wp.eta3[0] * r2, R),
+                                     (float)fc * dexp(-wp.eta3[1] * r2, R), __int_as_float(sj));
+            } else {
+                L.px[pos] = B.p0;
+                L.py[pos] = B.p1;
+                L.pz[pos] = B.p2;
+                L.r2[pos] = r2;
+                L.r[pos] = r;
+                L.rinv[pos] = 1.0 / r;
+                L.fc[pos] = (double)fc;
+                L.sp[pos] = sj;
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 if (P == 1 || q < wt.nr2) {  // rows past nr2: eta 0, no slot
@@ -711,7 +954,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
+            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x);
             __syncthreads();
             // the blocks are dealt columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48;
@@ -742,34 +985,86 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             const bool mine = NPB == 1 || col < cs + ncol;
             {
                 double acc3[16];
+                if constexpr (P == 1) {
+                    // this lane's species-pair cutoff
+                    const float rcl = blk3(pl, F.rc0, F.rc1, F.rc2);
+                    const double rc2l = blk3(pl, F.rq0, F.rq1, F.rq2);
+                    const float rcsl = blk3(pl, F.rs0, F.rs1, F.rs2);
+                    const bool rc_pair = NPB == 1 && ns > 1;  // cutoff per species pair
+                    const double2 *D = reinterpret_cast<const double2 *>(L.px);
+                    const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
+                    float acc[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
-                for (int i = 0; i < passes; ++i) {
-                    const int t = u + i * stride;
-                    if (mine && t < nb) {
-                        const int code = lst[t];
-                        Trip<P> q;
-                        triplet<P, false>(S, wp, wt, L, ns, code & 63, (code >> 6) & 63, q);
-                        const T fc3 = q.fa * q.fb * q.fjk;
-                        double te[2];
+                    for (int k = 0; k < 16; ++k) acc[k] = 0.0f;
+                    for (int i = 0; i < passes; ++i) {
+                        const int t = u + i * stride;
+                        if (mine && t < nb) {
+                            const int code = lst[t];
+                            const int a = code & 63, b = (code >> 6) & 63;
+                            const double2 a0 = D[a], a1 = D[kW + a], b0 = D[b], b1 = D[kW + b];
+                            const float4 va = V[a], vb = V[b];
+                            float rc = rcl, rcs = rcsl;
+                            double rc2 = rc2l;
+                            if (rc_pair) {
+                                const int k = pidx(__float_as_int(va.w), __float_as_int(vb.w), ns);
+                                rc = (float)S.rc[k];
+                                rc2 = S.rc2[k];
+                                rcs = (float)S.rcs[k];
+                            }
+                            TripF g;
+                            geom_f(a0, a1, b0, b1, rc, rc2, rcs, g);
+                            const float fjk = g.sn * g.sn;
+                            const float inv = va.x * vb.x;  // 1 / (2 r_ij r_ik)
+                            const double sp = (a1.y + b1.y) * (a1.y + b1.y);
+                            const double sm = (a1.y - b1.y) * (a1.y - b1.y);
+                            float te[2];
+                            te[0] = va.y * vb.y * (fjk * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2));
+                            te[1] = va.z * vb.z * (fjk * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2));
 #pragma unroll
-                        for (int e = 0; e < 2; ++e) te[e] = (double)(q.E[e] * fc3);
-                        // FP32-bulk: the full grid, without branches (entries past
-                        // it are neutral and have no slot); FP64: the grid's own size
-#pragma unroll
-                        for (int l = 0; l < 2; ++l) {
-                            if (P == 1 || l < wt.nl) {
-                                T pw[4];
-                                powers<P>(q.B[l], nsq, wp, wt.nz, pw);
+                            for (int l = 0; l < 2; ++l) {
+                                const float lb = __builtin_amdgcn_logf(base_f(F, l, sp, sm, g.rjk2d, inv));
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
-                                    if (P == 1 || z < wt.nz) {
-                                        const double pz = (double)((T)wp.coef[z] * pw[z]);
+                                    const float pw = __builtin_amdgcn_exp2f(F.zeta[z] * lb);
 #pragma unroll
-                                        for (int e = 0; e < 2; ++e)
-                                            if (P == 1 || e < wt.ne)
-                                                acc3[(z * 2 + l) * 2 + e] =
-                                                    fma(pz, te[e], acc3[(z * 2 + l) * 2 + e]);
+                                    for (int e = 0; e < 2; ++e)
+                                        acc[(z * 2 + l) * 2 + e] = fmaf(pw, te[e], acc[(z * 2 + l) * 2 + e]);
+                                }
+                            }
+                        }
+                    }
+                    // a lane's few terms are summed in FP32, the lanes in FP64
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) acc3[k] = (double)acc[k] * wp.coef[k >> 2];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
+                    for (int i = 0; i < passes; ++i) {
+                        const int t = u + i * stride;
+                        if (mine && t < nb) {
+                            const int code = lst[t];
+                            Trip<P> q;
+                            triplet<P, false>(S, wp, wt, L, ns, code & 63, (code >> 6) & 63, q);
+                            const T fc3 = q.fa * q.fb * q.fjk;
+                            double te[2];
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) te[e] = (double)(q.E[e] * fc3);
+                            // the grid's own size: l < nl, z < nz, e < ne
+#pragma unroll
+                            for (int l = 0; l < 2; ++l) {
+                                if (l < wt.nl) {
+                                    T pw[4];
+                                    powers<P>(q.B[l], nsq, wp, wt.nz, pw);
+#pragma unroll
+                                    for (int z = 0; z < 4; ++z) {
+                                        if (z < wt.nz) {
+                                            const double pz = (double)((T)wp.coef[z] * pw[z]);
+#pragma unroll
+                                            for (int e = 0; e < 2; ++e)
+                                                if (e < wt.ne)
+                                                    acc3[(z * 2 + l) * 2 + e] =
+                                                        fma(pz, te[e], acc3[(z * 2 + l) * 2 + e]);
+                                        }
                                     }
                                 }
                             }
@@ -842,6 +1137,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
+    const Fast F = P == 1 ? load_fast<NPB>(S, wp) : Fast();
     if (dc.hdr) {
         n_centers = dc.hdr[0];
         n_edges = dc.hdr[2];
@@ -905,15 +1201,25 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 }
             }
             const double ir = 1.0 / r;
-            L.px[pos] = B.p0;
-            L.py[pos] = B.p1;
-            L.pz[pos] = B.p2;
-            L.r2[pos] = r2;
-            L.r[pos] = r;
-            L.rinv[pos] = ir;
-            L.fc[pos] = (double)fc;
-            L.dfc[pos] = (double)dfc;
-            L.sp[pos] = sj;
+            if constexpr (P == 1) {
+                double2 *D = reinterpret_cast<double2 *>(L.px);
+                float4 *V = reinterpret_cast<float4 *>(D + 2 * kW);
+                float2 *W = reinterpret_cast<float2 *>(V + kW);
+                D[pos] = make_double2(d0, d1);
+                D[kW + pos] = make_double2(d2, r);
+                V[pos] = make_float4((float)ir, fc, dfc, __int_as_float(sj));
+                W[pos] = make_float2(dexp(-wp.eta3[0] * r2, R), dexp(-wp.eta3[1] * r2, R));
+            } else {
+                L.px[pos] = B.p0;
+                L.py[pos] = B.p1;
+                L.pz[pos] = B.p2;
+                L.r2[pos] = r2;
+                L.r[pos] = r;
+                L.rinv[pos] = ir;
+                L.fc[pos] = (double)fc;
+                L.dfc[pos] = (double)dfc;
+                L.sp[pos] = sj;
+            }
             L.ip[pos] = ipe;
             L.fx[pos] = sum * d0 * ir;
             L.fy[pos] = sum * d1 * ir;
@@ -923,11 +1229,11 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         const int ipen = s < Cn.N ? inv_pos[Cn.beg + s] : 0;
         __syncthreads();
 
-        // ---- triplets: one pass over the active pairs of every block ----
+        // ---- triplets: the active pairs of every block, lanes split over the blocks ----
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB>(S, L, ns, M, n1, s, cnt, base);
+            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x);
             // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda,
             // kept in the compute type (a loop-invariant copy costs half the registers)
@@ -939,67 +1245,90 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
             }
             __syncthreads();
-            const int c01 = cnt[0] + cnt[1], nall = c01 + cnt[2];
-            for (int t0 = 0; t0 < nall; t0 += kW) {
-                const int t = t0 + s;
-                if (t < nall) {
-                    // position t of the concatenated block lists
-                    const int p = t < cnt[0] ? 0 : (t < c01 ? 1 : 2);
-                    const int code = L.l0[p == 0 ? base[0] + t
-                                          : p == 1 ? base[1] + (t - cnt[0])
-                                                   : base[2] + (t - c01)];
-                    const T *w = reinterpret_cast<const T *>(L.w) + 16 * p;
-                    const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * p;
+            // the lanes are split over the species-pair blocks in proportion to
+            // their active pairs: the fewest passes I with sum_p ceil(cnt_p / I)
+            // <= 64 lanes; lane s serves block p with stride lanes(p)
+            const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
+            int passes = (nall + kW - 1) / kW, ln[3] = {kW, 0, 0};
+            if (NPB > 1 && nall > 0) {
+                auto lanes_for = [&](int I) {
+                    ln[0] = (cnt[0] + I - 1) / I;
+                    ln[1] = (cnt[1] + I - 1) / I;
+                    ln[2] = (cnt[2] + I - 1) / I;
+                    return ln[0] + ln[1] + ln[2];
+                };
+                while (lanes_for(passes) > kW) ++passes;
+            }
+            const int pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
+            const int u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
+            const int stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
+            const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
+            const unsigned short *lst =
+                L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
+            const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
+            const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
+            // this lane's species-pair cutoff (FP32-bulk)
+            const float rcl = P == 1 ? blk3(pl, F.rc0, F.rc1, F.rc2) : 0.0f;
+            const double rc2l = P == 1 ? blk3(pl, F.rq0, F.rq1, F.rq2) : 0.0;
+            const float rcsl = P == 1 ? blk3(pl, F.rs0, F.rs1, F.rs2) : 0.0f;
+            for (int i = 0; i < passes; ++i) {
+                const int t = u + i * stride;
+                if (u < stride && t < nb) {
+                    const int code = lst[t];
                     const int a = code & 63, b = (code >> 6) & 63;
-                    Trip<P> q;
-                    triplet<P, true>(S, wp, wt, L, ns, a, b, q);
-                    const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
-                    T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
+                    if constexpr (P == 1) {
+                        adj_triplet_f<NPB>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv);
+                    } else {
+                        Trip<P> q;
+                        triplet<P, true>(S, wp, wt, L, ns, a, b, q);
+                        const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
+                        T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
-                    for (int l = 0; l < 2; ++l) {
-                        if (P == 1 || l < wt.nl) {
-                            T pm[4];
-                            powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
+                        for (int l = 0; l < 2; ++l) {
+                            if (l < wt.nl) {
+                                T pm[4];
+                                powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
 #pragma unroll
-                            for (int z = 0; z < 4; ++z) {
-                                if (P == 1 || z < wt.nz) {
-                                    const T pm1 = pm[z];
-                                    const T pw = pm1 * q.B[l];
+                                for (int z = 0; z < 4; ++z) {
+                                    if (z < wt.nz) {
+                                        const T pm1 = pm[z];
+                                        const T pw = pm1 * q.B[l];
 #pragma unroll
-                                    for (int e = 0; e < 2; ++e) {
-                                        U[e] += w[(z * 2 + l) * 2 + e] * pw;
-                                        V[e] += wv[(z * 2 + l) * 2 + e] * pm1;
+                                        for (int e = 0; e < 2; ++e) {
+                                            U[e] += w[(z * 2 + l) * 2 + e] * pw;
+                                            V[e] += wv[(z * 2 + l) * 2 + e] * pm1;
+                                        }
                                     }
                                 }
                             }
                         }
-                    }
-                    T SA = (T)0, SB = (T)0, SC = (T)0;
+                        T SA = (T)0, SB = (T)0, SC = (T)0;
 #pragma unroll
-                    for (int e = 0; e < 2; ++e) {
-                        if (P == 1 || e < wt.ne) {
-                            SA += V[e] * q.E[e];
-                            const T ue = U[e] * q.E[e];
-                            SB += ue;
-                            SC += (T)wp.eta3[e] * ue;
+                        for (int e = 0; e < 2; ++e) {
+                            if (e < wt.ne) {
+                                SA += V[e] * q.E[e];
+                                const T ue = U[e] * q.E[e];
+                                SB += ue;
+                                SC += (T)wp.eta3[e] * ue;
+                            }
                         }
+                        // d cos(theta) / d r_ij, d r_ik, d r_jk
+                        const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
+                        const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
+                        const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
+                        const T fc3 = q.fa * q.fb * q.fjk;
+                        const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
+                        const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
+                        const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
+                        const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
+                                     a3 = (double)Ajk * q.rjkinv;
+                        atomicAdd(&L.fx[a], a1 * (L.px[a] - C.x0) - a3 * q.e0);
+                        atomicAdd(&L.fy[a], a1 * (L.py[a] - C.x1) - a3 * q.e1);
+                        atomicAdd(&L.fz[a], a1 * (L.pz[a] - C.x2) - a3 * q.e2);
+                        atomicAdd(&L.fx[b], a2 * (L.px[b] - C.x0) + a3 * q.e0);
+                        atomicAdd(&L.fy[b], a2 * (L.py[b] - C.x1) + a3 * q.e1);
+                        atomicAdd(&L.fz[b], a2 * (L.pz[b] - C.x2) + a3 * q.e2);
                     }
-                    // d cos(theta) / d r_ij, d r_ik, d r_jk
-                    const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
-                    const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
-                    const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
-                    const T fc3 = q.fa * q.fb * q.fjk;
-                    const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
-                    const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
-                    const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
-                    const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
-                                 a3 = (double)Ajk * q.rjkinv;
-                    atomicAdd(&L.fx[a], a1 * (L.px[a] - C.x0) - a3 * q.e0);
-                    atomicAdd(&L.fy[a], a1 * (L.py[a] - C.x1) - a3 * q.e1);
-                    atomicAdd(&L.fz[a], a1 * (L.pz[a] - C.x2) - a3 * q.e2);
-                    atomicAdd(&L.fx[b], a2 * (L.px[b] - C.x0) + a3 * q.e0);
-                    atomicAdd(&L.fy[b], a2 * (L.py[b] - C.x1) + a3 * q.e1);
-                    atomicAdd(&L.fz[b], a2 * (L.pz[b] - C.x2) + a3 * q.e2);
                 }
             }
             __syncthreads();
