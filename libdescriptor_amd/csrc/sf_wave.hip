@@ -86,6 +86,18 @@ __device__ __forceinline__ void xswap(double &a, double &b)
     b = __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
 }
 
+template <int M>
+__device__ __forceinline__ void xswapf(float &a, float &b)
+{
+    static_assert(M == 32 || M == 16, "permlane swaps exchange 32- or 16-lane halves");
+    const auto r = M == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
+                                                              false, false)
+                           : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
+                                                              false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
 // M <= 8: the partner lane's value through DPP: lane ^ 1 and lane ^ 2 (quad
 // permutations), the mirror lane in its 8-lane half row (differs in bit 2)
 // and in its 16-lane row (differs in bit 3).
@@ -177,23 +189,34 @@ __device__ __forceinline__ int pidx(int a, int b, int ns)
     return (int)__umul24(a, ns) + b;
 }
 
-// Pair t of the M neighbours in diagonal order: diagonal d = b - a holds the
-// M - d pairs (a, a + d) and starts at C(d) = (d - 1)(2M - d) / 2, so d is a
-// root of a quadratic in t, corrected by one where the float estimate lands on
-// the wrong side of a diagonal boundary (v_sqrt_f32's 1 ulp is enough).
+// Codes a | b << 6 of all pairs a < b of M compacted neighbours (M = 2..64)
+// in diagonal order: diagonal d = b - a = 1, 2, ... holds the M - d pairs
+// (a, a + d), a ascending.  The pairs of M start at C(M, 3) = M(M-1)(M-2)/6.
 // Along a diagonal the a's and the b's are all distinct, so the lanes of one
 // instruction touch distinct neighbours: conflict-free LDS reads, and the
-// adjoint's LDS accumulations into a and b rarely collide.
-__device__ __forceinline__ void pair_of(int t, int M, int &a, int &b)
+// adjoint's LDS accumulations into a and b rarely collide.  The table is
+// built at compile time (87 KB of read-only device memory); a lane reads its
+// pair's code with one coalesced load, issued a pass ahead.
+constexpr int kPairTabSize = 64 * 63 * 62 / 6 + 64 * 63 / 2;  // C(65, 3)
+struct PairTab {
+    unsigned short v[kPairTabSize];
+};
+
+constexpr PairTab make_pair_tab()
 {
-    // every product is below 2^24 (M <= 64): v_mul_u32_u24 at full rate
-    const int D = 2 * M + 1;
-    int d = (int)(((float)D - __builtin_amdgcn_sqrtf((float)(D * D - 8 * M - 8 * t))) * 0.5f);
-    d = max(d, 1);
-    d += ((int)__umul24(d, 2 * M - d - 1) >> 1 <= t) ? 1 : 0;   // C(d + 1) <= t
-    d -= ((int)__umul24(d - 1, 2 * M - d) >> 1 > t) ? 1 : 0;    // C(d) > t
-    a = t - ((int)__umul24(d - 1, 2 * M - d) >> 1);
-    b = a + d;
+    PairTab t{};
+    int o = 0;
+    for (int M = 2; M <= 64; ++M)
+        for (int d = 1; d < M; ++d)
+            for (int a = 0; a + d < M; ++a) t.v[o++] = (unsigned short)(a | ((a + d) << 6));
+    return t;
+}
+
+__device__ const PairTab kPairTab = make_pair_tab();
+
+__device__ __forceinline__ const unsigned short *pair_codes(int M)
+{
+    return kPairTab.v + (M * (M - 1) * (M - 2)) / 6;
 }
 
 // centres of this block: blocks are dealt to the 8 XCDs round-robin (block b
@@ -632,8 +655,8 @@ __device__ __forceinline__ float base_f(const Fast &F, int l, double sp, double 
 // n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.
 template <int P, int NPB>
 __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, const Fast &F,
-                                                int ns, int M, int n1, int s, int (&cnt)[3],
-                                                int (&base)[3])
+                                                int ns, int M, int n1, int s, int code0,
+                                                int (&cnt)[3], int (&base)[3])
 {
     const int n0 = M - n1;
     base[0] = 0;
@@ -641,13 +664,16 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
     base[2] = base[1] + n0 * n1;
     cnt[0] = cnt[1] = cnt[2] = 0;
     const int T = (M * (M - 1)) >> 1;
+    const unsigned short *tab = pair_codes(M);
+    int next = code0;  // this lane's code of the first pass, loaded by the caller
     for (int t0 = 0; t0 < T; t0 += kW) {
         const int t = t0 + s;
+        const int code = next;
+        if (t0 + kW < T) next = t + kW < T ? tab[t + kW] : 0;
         bool act = false;
-        int code = 0, blk = 0;
+        int blk = 0;
         if (t < T) {
-            int a, b;
-            pair_of(t, M, a, b);
+            const int a = code & 63, b = code >> 6;
             int sa, sb;
             if constexpr (P == 0) {
                 const double rjk2 =
@@ -670,17 +696,16 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                                            : (ns == 1 ? F.rq0 : S.rc2[pidx(sa, sb, ns)]);
                 act = rjk2 < rc2;
             }
-            code = a | (b << 6);
             if (NPB > 1) blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
         }
         if (NPB == 1) {
-            const unsigned long long m = __ballot(act);
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(act);
             if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
             cnt[0] += __popcll(m);
         } else {
-            const unsigned long long m0 = __ballot(act && blk == 0);
-            const unsigned long long m1 = __ballot(act && blk == 1);
-            const unsigned long long m2 = __ballot(act && blk == 2);
+            const unsigned long long m0 = __builtin_amdgcn_ballot_w64(act && blk == 0);
+            const unsigned long long m1 = __builtin_amdgcn_ballot_w64(act && blk == 1);
+            const unsigned long long m2 = __builtin_amdgcn_ballot_w64(act && blk == 2);
             if (act) {
                 const int p0 = base[0] + cnt[0] + below(m0, s);
                 const int p1 = base[1] + cnt[1] + below(m1, s);
@@ -909,6 +934,8 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         const unsigned long long m = __ballot(valid);
         const int M = __popcll(m);
         const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
+        // the enumeration's first pair codes, a phase ahead
+        const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
         if (valid) {
             const int pos = below(m, s);
             const int k = pidx(C.si, sj, ns);
@@ -954,7 +981,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, cnt, base);
+            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, code0, cnt, base);
             Bn = load_nbr(jn, species, x);
             __syncthreads();
             // the blocks are dealt columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48;
@@ -984,7 +1011,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
             const bool mine = NPB == 1 || col < cs + ncol;
             {
-                double acc3[16];
+                double v4[4];
                 if constexpr (P == 1) {
                     // this lane's species-pair cutoff
                     const float rcl = blk3(pl, F.rc0, F.rc1, F.rc2);
@@ -1033,10 +1060,25 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                             }
                         }
                     }
-                    // a lane's few terms are summed in FP32, the lanes in FP64
+                    // a lane's few terms and its column's 4 lanes (value halves
+                    // swapped between lane halves, then rows) are summed in FP32,
+                    // the columns in FP64: lane s keeps values 4 (s >> 4) .. +3
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) acc3[k] = (double)acc[k] * wp.coef[k >> 2];
+                    for (int q = 0; q < 8; ++q) {
+                        float a = acc[q], b = acc[q + 8];
+                        xswapf<32>(a, b);
+                        acc[q] = a + b;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float a = acc[q], b = acc[q + 4];
+                        xswapf<16>(a, b);
+                        acc[q] = a + b;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v4[q] = (double)acc[q];
                 } else {
+                    double acc3[16];
 #pragma unroll
                     for (int k = 0; k < 16; ++k) acc3[k] = 0.0;
                     for (int i = 0; i < passes; ++i) {
@@ -1070,10 +1112,6 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                             }
                         }
                     }
-                }
-                if (NPB == 1) {
-                    reduce_add<16>(acc3, s, wp.slots + 16, L.d);
-                } else {
                     // the column's 4 lanes: lane s keeps values 4 (s >> 4) .. +3
 #pragma unroll
                     for (int q = 0; q < 8; ++q) {
@@ -1087,21 +1125,26 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                         xswap<16>(a, b);
                         acc3[q] = a + b;
                     }
-                    // the neighbour arrays are dead here: column c's 16 sums
-                    // at buf[16 c ..], then lane (p, k) = (s >> 4, s & 15) adds
-                    // its block's columns in order
-                    double *buf = L.px;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) buf[16 * col + 4 * rowq + q] = acc3[q];
-                    __syncthreads();
-                    if (rowq < NPB) {
-                        const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
-                        const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
-                        double sum = 0.0;
-                        for (int c = c0; c < c1; ++c) sum += buf[16 * c + col];
-                        const int slot = wp.slots[16 + s];
-                        if (c1 > c0 && slot >= 0) L.d[slot] += sum;
-                    }
+                    for (int q = 0; q < 4; ++q) v4[q] = acc3[q];
+                }
+                // the neighbour arrays are dead here: value k of column c at
+                // buf[20 k + c] (20 doubles a value: the 64 lanes' stores fall on
+                // two bank sets), then lane (p, k) = (s >> 4, s & 15) adds its
+                // block's columns in order
+                double *buf = L.px;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) buf[20 * (4 * rowq + q) + col] = v4[q];
+                __syncthreads();
+                if (rowq < NPB) {
+                    const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
+                    const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
+                    double sum = 0.0;
+                    for (int c = c0; c < c1; ++c) sum += buf[20 * col + c];
+                    const int slot = wp.slots[16 + s];
+                    // the FP32-bulk rows take their 2^(1 - zeta) here
+                    if (P == 1) sum *= wp.coef[col >> 2];
+                    if (c1 > c0 && slot >= 0) L.d[slot] += sum;
                 }
             }
         } else {
@@ -1184,6 +1227,8 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         const unsigned long long m = __ballot(valid);
         const int M = __popcll(m);
         const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
+        // the enumeration's first pair codes, a phase ahead
+        const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
         if (valid) {
             const int pos = below(m, s);
             const int k = pidx(C.si, sj, ns);
@@ -1233,7 +1278,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, cnt, base);
+            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, code0, cnt, base);
             Bn = load_nbr(jn, species, x);
             // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda,
             // kept in the compute type (a loop-invariant copy costs half the registers)
