@@ -8,7 +8,7 @@ neighbor.py adds NeighborList / assemble_forces)::
 
 Everything computes on the GPU through hand-written HIP kernels; importing
 fails loudly if the native library has not been built
-(``python -m libdescriptor_amd.build``).
+(``python libdescriptor_amd/build.py``).
 """
 import os
 
@@ -29,7 +29,7 @@ try:
 except ImportError as e:  # pragma: no cover - exercised only without a build
     raise ImportError(
         "libdescriptor_amd: the native module is not built; run "
-        "`python -m libdescriptor_amd.build` (" + str(e) + ")") from e
+        "`python libdescriptor_amd/build.py` (" + str(e) + ")") from e
 
 from .neighbor import NeighborList, NeighborListError, assemble_forces  # noqa: F401,E402
 

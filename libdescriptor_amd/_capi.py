@@ -92,7 +92,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libdescriptor_amd: libldsf.so is not built; run "
-                              "`python -m libdescriptor_amd.build`")
+                              "`python libdescriptor_amd/build.py`")
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _PROTOS.items():
             f = getattr(L, name)

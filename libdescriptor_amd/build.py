@@ -6,7 +6,8 @@
 * ``libdescriptor_amd/libdescriptor<ext>.so`` -- the pybind11 module, linked to
   libldsf.so with an $ORIGIN rpath.
 
-Usage: ``python -m libdescriptor_amd.build`` (or ``build()`` from Python).
+Usage: ``python libdescriptor_amd/build.py`` (or ``build()`` from Python). It is run
+as a file: the package refuses to import until this has built it.
 """
 import os
 import subprocess
