@@ -214,8 +214,9 @@ def main():
         if plan.kernel_family == 1 else ("k_adj_general" if dom == "adjoint" else "k_fwd_general")
     traffic = measured_traffic(dom_kernel, n) if args.n == 262144 else None
 
-    batched = serving = None
+    batched = serving = layout = None
     if not args.no_extra:
+        layout = bench_layout(L, dev)
         batched = bench_batched(args, ds)
         serving = bench_serving(args)
 
@@ -254,11 +255,56 @@ def main():
                      "traffic": traffic,
                      "hbm_GBps": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None},
         "cpu_baseline": cpu,
+        "layout": layout,
         "batched": batched,
         "serving": serving,
         "kernel_family": plan.kernel_family,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_layout(L, dev):
+    """The input layout of the same system built on the GPU (padding rows and
+    the KLIFF-order neighbour list of the contributing atoms, ldsf_layout_create)
+    against the host producers: the product's C++ (OpenMP) and, where
+    oracle/_ref was built, the reference's own single-threaded code."""
+    import torch
+    from libdescriptor_amd import _capi, workloads as w
+    from oracle import ref
+    n = L["n_contrib"]
+    pos = np.ascontiguousarray(L["coords"][:n])
+    sp = np.ascontiguousarray(L["species"][:n])
+    cell = np.eye(3) * (n / 0.05) ** (1.0 / 3.0)
+    pbc = np.array([1, 1, 1], np.int32)
+    xd = torch.tensor(pos, device=dev)
+    sd = torch.tensor(sp, device=dev)
+    times = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        D = _capi.DeviceLayout(xd, sd, cell, pbc, 5.0)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    dev_ms = float(np.median(times[1:]) * 1e3)
+    same = (np.array_equal(D.coords.cpu().numpy(), L["coords"]) and
+            np.array_equal(D.numneigh.cpu().numpy(), L["numneigh"]) and
+            np.array_equal(D.neighlist.cpu().numpy(), L["neighlist"]))
+    t = time.perf_counter()
+    w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+    host_ms = (time.perf_counter() - t) * 1e3
+    out = {"rows": D.n_contrib + D.n_padding, "edges": D.n_neigh, "device_ms": dev_ms,
+           "equal_to_host_producer": bool(same), "host_product_ms": host_ms,
+           "host_threads": min(len(os.sched_getaffinity(0)), 16)}
+    if ref.available():
+        t = time.perf_counter()
+        pc, _, _ = ref.create_paddings(5.0, cell, pbc, pos, sp)
+        need = np.r_[np.ones(n), np.zeros(len(pc))].astype(np.int32)
+        ref.neighlist(np.concatenate([pos, pc]), 5.0, need, n)
+        out["reference_ms"] = (time.perf_counter() - t) * 1e3
+        out["reference_threads"] = 1
+    log(f"layout: device {dev_ms:.1f} ms (equal to host: {same}), host C++ {host_ms:.0f} ms"
+        + (f", reference {out['reference_ms']:.0f} ms" if "reference_ms" in out else ""))
+    return out
 
 
 def bench_batched(args, ds):

@@ -276,6 +276,51 @@ LDSF_API long ldsf_nbl_total(const ldsf_neighlist *nl, int k, int n_query);
 LDSF_API int ldsf_nbl_flatten(const ldsf_neighlist *nl, int k, int n_query, int *numneigh,
                               int *neighlist);
 
+/* ------------------------------------------------------------------------
+ * Device-resident layout producer (GPU twin of the producers above)
+ * ------------------------------------------------------------------------ */
+typedef struct ldsf_layout ldsf_layout;
+
+/* NeighborList(conf, infl_dist) (python_package/libdescriptor/neighbor.py:35-78)
+ * built on the current device: padding atoms as nbl_create_paddings
+ * (neighbor_list.cpp:283-418) appended after the n_contrib contributing rows,
+ * then the full neighbour list of the contributing rows as nbl_build
+ * (neighbor_list.cpp:78-243, needNeighbors = 1 for contributing rows only,
+ * one cutoff = infl_dist) in the flat layout of get_numneigh_and_neighlist_1D
+ * (neighbor.py:108-154).  Same rows, order and FP64 values as the host
+ * producers.  cell[9] (rows = cell vectors) and pbc[3] are host arrays;
+ * d_coords[n_contrib][3] (float64) and d_species[n_contrib] (int32) are device
+ * arrays.  Errors as the host producers: LDSF_ERR_SINGULAR_CELL,
+ * LDSF_ERR_NEIGHBOR (collision, too many cells).  Synchronises `stream`. */
+LDSF_API int ldsf_layout_create(ldsf_layout **out, int n_contrib, double cutoff,
+                                const double *cell, const int *pbc, const double *d_coords,
+                                const int *d_species, void *stream);
+LDSF_API void ldsf_layout_destroy(ldsf_layout *layout);
+/* sizes: rows = n_contrib + n_padding; n_neigh = sum(numneigh) */
+LDSF_API int ldsf_layout_info(const ldsf_layout *layout, int *n_contrib, int *n_padding,
+                              long *n_neigh, int *max_neighbors);
+/* device arrays owned by the layout: coords[rows][3], species[rows],
+ * image[rows] (NeighborList.get_image: a contributing row's image is itself),
+ * numneigh[n_contrib], neighlist[n_neigh] */
+LDSF_API int ldsf_layout_arrays(const ldsf_layout *layout, const double **coords,
+                                const int **species, const int **image, const int **numneigh,
+                                const int **neighlist);
+/* copies the layout's arrays into caller-owned device buffers (any may be NULL)
+ * of the sizes ldsf_layout_info reports, ordered on `stream` */
+LDSF_API int ldsf_layout_copy(const ldsf_layout *layout, double *d_coords, int *d_species,
+                              int *d_image, int *d_numneigh, int *d_neighlist, void *stream);
+/* assemble_forces(forces, n, padding_image) (neighbor.py:227-263) on the
+ * device: d_out[n_contrib][3] = the contributing rows of d_forces[rows][3]
+ * plus the rows of their padding images, summed in the reference's order
+ * (per atom 0 + sum of its image rows, then added, when n < n_padding; else
+ * the image rows added one by one in padding order). */
+LDSF_API int ldsf_layout_assemble_forces(const ldsf_layout *layout, const double *d_forces,
+                                         double *d_out, void *stream);
+/* the same fold for any padding_image (device int32 [n_padding], values in
+ * [0, n_contrib)); d_forces[n_contrib + n_padding][3].  Synchronises `stream`. */
+LDSF_API int ldsf_assemble_forces(int n_contrib, int n_padding, const double *d_forces,
+                                  const int *d_padding_image, double *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

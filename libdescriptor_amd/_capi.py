@@ -61,6 +61,13 @@ _PROTOS = {
     "ldsf_nbl_num_particles": (_i, [_vp]),
     "ldsf_nbl_total": (_l, [_vp, _i, _i]),
     "ldsf_nbl_flatten": (_i, [_vp, _i, _i, _vp, _vp]),
+    "ldsf_layout_create": (_i, [_vp, _i, _d, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_layout_destroy": (None, [_vp]),
+    "ldsf_layout_info": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "ldsf_layout_arrays": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_layout_copy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_layout_assemble_forces": (_i, [_vp, _vp, _vp, _vp]),
+    "ldsf_assemble_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "ldsf_server_create": (_i, [_vp, _vp, _vp]),
     "ldsf_server_num_lanes": (_i, [_vp]),
     "ldsf_server_submit": (_i, [_vp, _vp, _vp]),
@@ -198,6 +205,83 @@ class Plan:
         if h and _lib is not None:
             _lib.ldsf_plan_destroy(h)
             self.handle = None
+
+
+class DeviceLayout:
+    """The device layout producer (ldsf_layout): padding atoms and the KLIFF-order
+    neighbour list of the contributing atoms, built on the GPU from device
+    coordinates (the GPU twin of NeighborList, python_package/libdescriptor/
+    neighbor.py:35-154).  After construction the arrays are torch tensors on
+    the same device: coords (rows, 3) float64, species / image (rows,) int32,
+    numneigh (n_contrib,) int32, neighlist (n_neigh,) int32."""
+
+    def __init__(self, coords, species, cell, pbc, cutoff, stream=None):
+        import numpy as np
+        import torch
+        if coords.dtype != torch.float64 or species.dtype != torch.int32:
+            raise TypeError("coords must be float64 and species int32 device tensors")
+        coords = coords.contiguous()
+        species = species.contiguous()
+        n = int(species.numel())
+        if coords.numel() != 3 * n:
+            raise ValueError("coords must have shape (n, 3) matching species")
+        cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+        pbc = np.ascontiguousarray(pbc, dtype=np.int32).reshape(3)
+        st = _stream_ptr(stream)
+        out = ctypes.c_void_p()
+        check(lib().ldsf_layout_create(ctypes.byref(out), n, float(cutoff), cell.ctypes.data,
+                                       pbc.ctypes.data, _ptr(coords), _ptr(species), st))
+        self.handle = out.value
+        nc, npd, nn, mx = ctypes.c_int(), ctypes.c_int(), ctypes.c_long(), ctypes.c_int()
+        check(lib().ldsf_layout_info(self.handle, ctypes.byref(nc), ctypes.byref(npd),
+                                     ctypes.byref(nn), ctypes.byref(mx)))
+        self.n_contrib, self.n_padding = nc.value, npd.value
+        self.n_neigh, self.max_neighbors = nn.value, mx.value
+        rows = self.n_contrib + self.n_padding
+        dev = coords.device
+        self.coords = torch.empty((rows, 3), dtype=torch.float64, device=dev)
+        self.species = torch.empty(rows, dtype=torch.int32, device=dev)
+        self.image = torch.empty(rows, dtype=torch.int32, device=dev)
+        self.numneigh = torch.empty(self.n_contrib, dtype=torch.int32, device=dev)
+        self.neighlist = torch.empty(max(self.n_neigh, 1), dtype=torch.int32, device=dev)
+        check(lib().ldsf_layout_copy(self.handle, _ptr(self.coords), _ptr(self.species),
+                                     _ptr(self.image), _ptr(self.numneigh), _ptr(self.neighlist),
+                                     st))
+        self.neighlist = self.neighlist[:self.n_neigh]
+
+    def assemble_forces(self, forces, out=None, stream=None):
+        """assemble_forces (neighbor.py:227-263) on the device: (rows, 3) -> (n_contrib, 3)."""
+        import torch
+        forces = forces.contiguous()
+        if forces.dtype != torch.float64 or forces.shape != (self.n_contrib + self.n_padding, 3):
+            raise ValueError("forces must be float64 of shape (rows, 3)")
+        if out is None:
+            out = torch.empty((self.n_contrib, 3), dtype=torch.float64, device=forces.device)
+        check(lib().ldsf_layout_assemble_forces(self.handle, _ptr(forces), _ptr(out),
+                                                _stream_ptr(stream)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.ldsf_layout_destroy(h)
+            self.handle = None
+
+
+def assemble_forces_device(forces, n, padding_image, stream=None):
+    """assemble_forces (neighbor.py:227-263) for device tensors: forces (n + n_pad, 3)
+    float64, padding_image (n_pad,) int32 -> (n, 3)."""
+    import torch
+    forces = forces.contiguous()
+    padding_image = padding_image.contiguous()
+    n_pad = int(padding_image.numel())
+    if forces.dtype != torch.float64 or forces.shape != (n + n_pad, 3) or \
+            padding_image.dtype != torch.int32:
+        raise ValueError("forces must be float64 (n + n_pad, 3), padding_image int32")
+    out = torch.empty((n, 3), dtype=torch.float64, device=forces.device)
+    check(lib().ldsf_assemble_forces(int(n), n_pad, _ptr(forces), _ptr(padding_image), _ptr(out),
+                                     _stream_ptr(stream)))
+    return out
 
 
 def _stream_ptr(stream):

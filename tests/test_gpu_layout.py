@@ -1,0 +1,205 @@
+"""The device layout producer (ldsf_layout_*: padding atoms, KLIFF-order
+neighbour list, flat layout) and the device ghost fold (assemble_forces)
+against the reference's compiled neighbour code (golden fixtures, oracle/_ref),
+the numpy restatement (oracle.nbl) and brute-force membership.  Integer and
+index outputs and padding coordinates are compared bit for bit; the fold is
+compared bit for bit with the reference's summation order."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return torch.device("cuda:0")
+
+
+def device_layout(pos, cell, pbc, species, rc):
+    from libdescriptor_amd import _capi
+    d = _dev()
+    x = torch.tensor(np.ascontiguousarray(pos, dtype=np.float64), device=d)
+    s = torch.tensor(np.ascontiguousarray(species, dtype=np.int32), device=d)
+    L = _capi.DeviceLayout(x, s, cell, pbc, rc)
+    torch.cuda.synchronize()
+    return L
+
+
+def as_np(L):
+    return (L.coords.cpu().numpy(), L.species.cpu().numpy(), L.image.cpu().numpy(),
+            L.numneigh.cpu().numpy(), L.neighlist.cpu().numpy())
+
+
+def test_golden_c1_layout_bit_exact(c1):
+    print("case: C1 64-atom diamond Si, rc 4.5", flush=True)
+    L = device_layout(c1["positions"], c1["cell"], [1, 1, 1], np.zeros(64, np.int32), 4.5)
+    coords, sp, image, nn, nl = as_np(L)
+    np.testing.assert_array_equal(coords, c1["coords"])
+    np.testing.assert_array_equal(image, np.r_[np.arange(64), c1["image"]])
+    np.testing.assert_array_equal(sp, c1["species"])
+    np.testing.assert_array_equal(nn, c1["numneigh"])
+    np.testing.assert_array_equal(nl, c1["neighlist"])
+    assert L.max_neighbors == int(c1["numneigh"].max())
+
+
+def test_golden_two_species_layout():
+    print("case: two-species golden, rc 5.0", flush=True)
+    t = golden("two_species_ref.npz")
+    L = float(t["L"])
+    Ld = device_layout(t["positions"], np.eye(3) * L, [1, 1, 1], t["species0"], 5.0)
+    coords, sp, image, nn, nl = as_np(Ld)
+    np.testing.assert_array_equal(coords, t["coords"])
+    np.testing.assert_array_equal(sp, t["species"])
+    np.testing.assert_array_equal(nn, t["numneigh"])
+    np.testing.assert_array_equal(nl, t["neighlist"])
+
+
+CELLS = [
+    (np.eye(3) * 9.0, [1, 1, 1]),
+    (np.array([[8.0, 0, 0], [2.5, 7.5, 0], [1.0, -1.5, 9.0]]), [1, 1, 1]),  # triclinic
+    (np.eye(3) * 7.0, [1, 0, 1]),                                          # one open direction
+    (np.eye(3) * 6.0, [0, 0, 0]),                                          # cluster
+    (np.eye(3) * 3.1, [1, 1, 1]),                                          # cell < cutoff: 2 image shells
+]
+
+
+@pytest.mark.parametrize("case", range(len(CELLS)))
+def test_random_cells_match_reference_order(case):
+    from oracle import nbl, ref
+    cell, pbc = CELLS[case]
+    print(f"case: random cell {case}", flush=True)
+    rng = np.random.default_rng(case)
+    nat = 40 if case < 4 else 6
+    pos = rng.random((nat, 3)) @ cell
+    sp = rng.integers(0, 2, nat).astype(np.int32)
+    rc = 4.1
+    coords, spd, image, nn, nl = as_np(device_layout(pos, cell, pbc, sp, rc))
+    pc, ps, pi = nbl.create_paddings(rc, cell, pbc, pos, sp)
+    np.testing.assert_array_equal(coords, np.concatenate([pos, pc]))
+    np.testing.assert_array_equal(spd, np.concatenate([sp, ps]))
+    np.testing.assert_array_equal(image, np.concatenate([np.arange(nat), pi]))
+    need = np.r_[np.ones(nat), np.zeros(len(pc))].astype(np.int32)
+    nn2, nl2 = nbl.neighlist(coords, rc, need, nat)
+    np.testing.assert_array_equal(nn, nn2)
+    np.testing.assert_array_equal(nl, nl2)
+    off = np.r_[0, np.cumsum(nn)]
+    for i, s in enumerate(nbl.brute_force_sets(coords, rc, nat)):
+        np.testing.assert_array_equal(np.sort(nl[off[i]:off[i + 1]]), s)
+    if ref.available():
+        pc3, _, pi3 = ref.create_paddings(rc, cell, pbc, pos, sp)
+        np.testing.assert_array_equal(coords[nat:], pc3)
+        np.testing.assert_array_equal(image[nat:], pi3)
+        _, nl3 = ref.neighlist(coords, rc, need, nat)
+        np.testing.assert_array_equal(nl, nl3)
+
+
+def test_full_size_two_species_layout():
+    """262,144 atoms (the benchmark's system): every row, index and count equal to
+    the compiled reference's producers (oracle/_ref) where present, else to the
+    host producer, plus the numpy restatement on the first 3000 centres."""
+    from libdescriptor_amd import workloads
+    from oracle import nbl, ref
+    print("case: 262144-atom two-species system, rc 5.0", flush=True)
+    pos, cell, pbc, sp = workloads.random_system(262144, 0.05, 1.8, seed=0)
+    coords, spd, image, nn, nl = as_np(device_layout(pos, cell, pbc, sp, 5.0))
+    n = len(pos)
+    if ref.available():
+        pc, ps, pi = ref.create_paddings(5.0, cell, pbc, pos, sp)
+        need = np.r_[np.ones(n), np.zeros(len(pc))].astype(np.int32)
+        nn2, nl2 = ref.neighlist(np.concatenate([pos, pc]), 5.0, need, n)
+    else:
+        H = workloads.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+        pc, pi = H["coords"][n:], H["image"][n:]
+        ps = H["species"][n:]
+        nn2, nl2 = H["numneigh"], H["neighlist"]
+    np.testing.assert_array_equal(coords[n:], pc)
+    np.testing.assert_array_equal(spd[n:], ps)
+    np.testing.assert_array_equal(image[n:], pi)
+    np.testing.assert_array_equal(nn, nn2)
+    np.testing.assert_array_equal(nl, nl2)
+    k = 3000
+    need = np.r_[np.ones(k), np.zeros(len(coords) - k)].astype(np.int32)
+    nn3, nl3 = nbl.neighlist(coords, 5.0, need, k)
+    np.testing.assert_array_equal(nn[:k], nn3)
+    np.testing.assert_array_equal(nl[:int(nn3.sum())], nl3)
+
+
+def test_errors_collision_and_singular_cell():
+    from libdescriptor_amd import _capi
+    print("case: collision and singular cell", flush=True)
+    pos = np.array([[0.0, 0, 0], [1e-6, 0, 0], [3.0, 0, 0]])
+    with pytest.raises(_capi.LdsfError, match="Collision of atoms 1 and 2"):
+        device_layout(pos, np.eye(3) * 20.0, [0, 0, 0], np.zeros(3, np.int32), 4.0)
+    with pytest.raises(_capi.LdsfError, match="determinant"):
+        device_layout(pos + [5.0, 0, 0], np.zeros((3, 3)), [1, 1, 1], np.zeros(3, np.int32), 4.0)
+
+
+@pytest.mark.parametrize("which", ["c1", "large"])
+def test_assemble_forces_bitwise(which, c1):
+    """Both summation orders of neighbor.py:227-263: per-atom np.sum when
+    n < n_padding (C1: 64 atoms, 327 padding rows), one add per padding row
+    otherwise (a 4000-atom periodic system)."""
+    from libdescriptor_amd import _capi, workloads
+    from oracle import nbl
+    print(f"case: assemble_forces {which}", flush=True)
+    if which == "c1":
+        L = device_layout(c1["positions"], c1["cell"], [1, 1, 1], np.zeros(64, np.int32), 4.5)
+    else:
+        pos, cell, pbc, sp = workloads.random_system(4000, 0.05, 1.8, seed=3)
+        L = device_layout(pos, cell, pbc, sp, 5.0)
+    rows = L.n_contrib + L.n_padding
+    assert (L.n_contrib < L.n_padding) == (which == "c1")
+    rng = np.random.default_rng(7)
+    # magnitudes over many decades, so that a different order rounds differently
+    f = rng.normal(size=(rows, 3)) * 10.0 ** rng.integers(-6, 6, size=(rows, 3))
+    f[5] = -0.0  # sign of zero follows the reference's order too
+    ft = torch.tensor(f, device=_dev())
+    out = L.assemble_forces(ft).cpu().numpy()
+    img = L.image.cpu().numpy()[L.n_contrib:]
+    exp = nbl.assemble_forces(f, L.n_contrib, img)
+    np.testing.assert_array_equal(out.view(np.int64), exp.view(np.int64))
+    out2 = _capi.assemble_forces_device(ft, L.n_contrib,
+                                        torch.tensor(img, device=_dev())).cpu().numpy()
+    np.testing.assert_array_equal(out2.view(np.int64), exp.view(np.int64))
+
+
+def test_assemble_forces_rejects_bad_image():
+    from libdescriptor_amd import _capi
+    print("case: assemble_forces bad image", flush=True)
+    d = _dev()
+    f = torch.zeros((5, 3), dtype=torch.float64, device=d)
+    with pytest.raises(_capi.LdsfError):
+        _capi.assemble_forces_device(f, 3, torch.tensor([0, 3], dtype=torch.int32, device=d))
+
+
+def test_device_layout_feeds_the_plan(c1):
+    """Positions on the device -> layout -> plan -> descriptors and dE/dr ->
+    fold, all on the GPU, equal to the golden descriptors and the oracle's
+    folded gradient."""
+    import libdescriptor_amd as ld
+    from libdescriptor_amd.autograd import SymmetryFunctionsLayer
+    from libdescriptor_amd.workloads import behler_si_params
+    from oracle import nbl, orc
+    print("case: C1 device layout -> plan -> fold", flush=True)
+    d = _dev()
+    L = device_layout(c1["positions"], c1["cell"], [1, 1, 1], np.zeros(64, np.int32), 4.5)
+    types, sizes, params = behler_si_params()
+    ds = ld.DescriptorKind.init_descriptor(ld.AvailableDescriptors.SymmetryFunctions, ["Si"], "cos",
+                                           4.5, types, sizes, params)
+    layer = SymmetryFunctionsLayer(ds, L.species, L.neighlist, L.numneigh, device=d)
+    x = L.coords.clone().requires_grad_(True)
+    desc = layer(x)
+    w = torch.tensor(c1["dE_dzeta"], device=d)
+    (desc * w).sum().backward()
+    forces = L.assemble_forces(x.grad).cpu().numpy()
+    assert float(np.abs(desc.detach().cpu().numpy() - c1["desc"]).max()
+                 / np.abs(c1["desc"]).max()) <= 1e-12
+    O = orc.OracleSF(["Si"], 4.5, types, sizes, params)
+    G = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"], c1["dE_dzeta"])
+    F0 = nbl.assemble_forces(G, 64, c1["image"])
+    assert float(np.abs(forces - F0).max() / np.abs(F0).max()) <= 1e-12
