@@ -100,8 +100,11 @@ LDSF_API int ldsf_compute_batch(ldsf_sf *sf, int n_configurations, const int *n_
                                 const double *coordinates, double *desc);
 
 /* Descriptor::gradient (Descriptors.hpp:58-60, Descriptors.cpp:165-190).
- * d_coordinates += J^T dE_dzeta.  desc is accepted for signature parity and
- * is not read or written (the reference's callers pass a throwaway copy). */
+ * d_coordinates += J^T dE_dzeta, and desc += the descriptors: the reference's
+ * Enzyme call runs the forward too (enzyme_dup desc), so the same holds here,
+ * from one fused pass; desc may be NULL (the reference's callers pass a
+ * throwaway copy, python_bindings.cpp:236-238).  The same holds for the
+ * single-atom and batch forms below. */
 LDSF_API int ldsf_gradient(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
                            const int *number_of_neighs, const double *coordinates,
                            double *d_coordinates, double *desc, const double *dE_dzeta);
@@ -171,6 +174,18 @@ LDSF_API int ldsf_plan_compute(ldsf_plan *plan, const double *d_coordinates, dou
 LDSF_API int ldsf_plan_gradient(ldsf_plan *plan, const double *d_coordinates,
                                 const double *d_dE_dzeta, double *d_dcoords, int precision,
                                 void *stream);
+
+/* The reference's gradient semantics on the device: d_desc[n_centers][width]
+ * += descriptors and d_dcoords[n_total][3] += J^T d_dE_dzeta, from one fused
+ * kernel that shares the pair and triplet work between the two (then the
+ * gather).  ldsf_plan_forward_adjoint_edges is its first half, the fused
+ * kernel alone; ldsf_plan_gather completes it. */
+LDSF_API int ldsf_plan_forward_gradient(ldsf_plan *plan, const double *d_coordinates,
+                                        const double *d_dE_dzeta, double *d_desc,
+                                        double *d_dcoords, int precision, void *stream);
+LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
+                                             const double *d_dE_dzeta, double *d_desc,
+                                             int precision, void *stream);
 
 /* The two halves of ldsf_plan_gradient.  ldsf_plan_adjoint_edges runs the
  * adjoint kernel and keeps one dE/dr vector per neighbour-list edge and per

@@ -46,6 +46,8 @@ _PROTOS = {
     "ldsf_plan_gradient": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_adjoint_edges": (_i, [_vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_gather": (_i, [_vp, _vp, _vp]),
+    "ldsf_plan_forward_gradient": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_forward_adjoint_edges": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_kernel_family": (_i, [_vp]),
     "ldsf_plan_set_kernel_family": (_i, [_vp, _i]),
     "ldsf_nbl_create_paddings": (_i, [_i, _d, _vp, _vp, _vp, _vp, _vp]),
@@ -195,6 +197,19 @@ class Plan:
         """First half of gradient(): the adjoint kernel alone."""
         check(lib().ldsf_plan_adjoint_edges(self.handle, _ptr(coords), _ptr(dE_dzeta),
                                             int(precision), _stream_ptr(stream)))
+
+    def forward_gradient(self, coords, dE_dzeta, desc, dcoords, precision=FP64, stream=None):
+        """The reference's gradient: desc += descriptors, dcoords += J^T dE_dzeta,
+        from one fused kernel and the gather."""
+        check(lib().ldsf_plan_forward_gradient(self.handle, _ptr(coords), _ptr(dE_dzeta),
+                                               _ptr(desc), _ptr(dcoords), int(precision),
+                                               _stream_ptr(stream)))
+
+    def forward_adjoint_edges(self, coords, dE_dzeta, desc, precision=FP64, stream=None):
+        """First half of forward_gradient(): the fused kernel alone."""
+        check(lib().ldsf_plan_forward_adjoint_edges(self.handle, _ptr(coords), _ptr(dE_dzeta),
+                                                    _ptr(desc), int(precision),
+                                                    _stream_ptr(stream)))
 
     def gather(self, dcoords, stream=None):
         """Second half of gradient(): dcoords += the per-row sums of the records."""

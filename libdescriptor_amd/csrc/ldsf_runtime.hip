@@ -313,6 +313,35 @@ int plan_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, int p
     return LDSF_OK;
 }
 
+// the fused kernel: desc += descriptors and the adjoint's records, from one
+// pass over the pairs and triplets (general family: the two kernels in turn)
+int plan_forward_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, double *desc,
+                               int precision, hipStream_t s)
+{
+    ldsf::SFTab tab;
+    if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
+    const long n_rec = plan->n_edges + plan->n_centers;
+    if (!plan->edge && n_rec > 0)
+        HIP_TRY(hipMalloc((void **)&plan->edge, sizeof(double) * 3 * n_rec), "hipMalloc(edge)");
+    hipError_t e;
+    if (plan->family == 1 && plan->wave_ok) {
+        e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
+                                  plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
+                                  plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,
+                                  plan->wave_group, s, ldsf::DevCounts(), desc);
+    } else {
+        e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
+                                     plan->nl, plan->species, x, desc, s);
+        if (e == hipSuccess)
+            e = ldsf::launch_adj_general(tab, precision, plan->n_centers, plan->centers,
+                                         plan->offs, plan->nl, plan->species, x, dE,
+                                         plan->inv_pos, plan->edge, (int)plan->n_edges, s);
+    }
+    HIP_TRY(e, "fused kernel launch");
+    plan->edges_ready = true;
+    return LDSF_OK;
+}
+
 int plan_gather(ldsf_plan *plan, double *dcoords, hipStream_t s)
 {
     if (!plan->edges_ready && plan->n_edges + plan->n_centers > 0)
@@ -326,6 +355,13 @@ int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dc
                   int precision, hipStream_t s)
 {
     if (int rc = plan_adjoint_edges(plan, x, dE, precision, s)) return rc;
+    return plan_gather(plan, dcoords, s);
+}
+
+int plan_forward_gradient(ldsf_plan *plan, const double *x, const double *dE, double *desc,
+                          double *dcoords, int precision, hipStream_t s)
+{
+    if (int rc = plan_forward_adjoint_edges(plan, x, dE, desc, precision, s)) return rc;
     return plan_gather(plan, dcoords, s);
 }
 
@@ -400,17 +436,24 @@ int host_run(HostCall &c, double *desc_io, const double *dE, double *dcoords_io,
     if (desc_io) {
         HIP_TRY(d_desc.alloc((size_t)c.n_centers * W), "hipMalloc");
         HIP_TRY(hipMemcpy(d_desc.p, desc_io, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
-        if ((rc = plan_compute(plan, d_x.p, d_desc.p, precision, s))) return rc;
-        HIP_TRY(hipMemcpy(desc_io, d_desc.p, sizeof(double) * c.n_centers * W, hipMemcpyDeviceToHost), "hipMemcpy");
     }
     if (dE) {
         HIP_TRY(d_dE.alloc((size_t)c.n_centers * W), "hipMalloc");
         HIP_TRY(d_dx.alloc(3 * (size_t)c.n_rows), "hipMalloc");
         HIP_TRY(hipMemcpy(d_dE.p, dE, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
         HIP_TRY(hipMemcpy(d_dx.p, dcoords_io, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
-        if ((rc = plan_gradient(plan, d_x.p, d_dE.p, d_dx.p, precision, s))) return rc;
-        HIP_TRY(hipMemcpy(dcoords_io, d_dx.p, sizeof(double) * 3 * c.n_rows, hipMemcpyDeviceToHost), "hipMemcpy");
     }
+    if (desc_io && dE)  // the reference's gradient: the forward runs too (one fused pass)
+        rc = plan_forward_gradient(plan, d_x.p, d_dE.p, d_desc.p, d_dx.p, precision, s);
+    else if (desc_io)
+        rc = plan_compute(plan, d_x.p, d_desc.p, precision, s);
+    else if (dE)
+        rc = plan_gradient(plan, d_x.p, d_dE.p, d_dx.p, precision, s);
+    if (rc) return rc;
+    if (desc_io)
+        HIP_TRY(hipMemcpy(desc_io, d_desc.p, sizeof(double) * c.n_centers * W, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (dE)
+        HIP_TRY(hipMemcpy(dcoords_io, d_dx.p, sizeof(double) * 3 * c.n_rows, hipMemcpyDeviceToHost), "hipMemcpy");
     HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     return LDSF_OK;
 }
@@ -530,6 +573,30 @@ LDSF_API int ldsf_plan_adjoint_edges(ldsf_plan *plan, const double *d_coordinate
     return plan_adjoint_edges(plan, d_coordinates, d_dE_dzeta, precision, (hipStream_t)stream);
 }
 
+LDSF_API int ldsf_plan_forward_gradient(ldsf_plan *plan, const double *d_coordinates,
+                                        const double *d_dE_dzeta, double *d_desc,
+                                        double *d_dcoords, int precision, void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta || !d_desc || !d_dcoords)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_gradient: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_gradient: unknown precision");
+    return plan_forward_gradient(plan, d_coordinates, d_dE_dzeta, d_desc, d_dcoords, precision,
+                                 (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
+                                             const double *d_dE_dzeta, double *d_desc,
+                                             int precision, void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta || !d_desc)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_adjoint_edges: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_adjoint_edges: unknown precision");
+    return plan_forward_adjoint_edges(plan, d_coordinates, d_dE_dzeta, d_desc, precision,
+                                      (hipStream_t)stream);
+}
+
 LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream)
 {
     if (!plan || (plan->n_total > 0 && !d_dcoords))
@@ -574,11 +641,10 @@ LDSF_API int ldsf_gradient(ldsf_sf *sf, int n_atoms, const int *species, const i
                            const int *number_of_neighs, const double *coordinates,
                            double *d_coordinates, double *desc, const double *dE_dzeta)
 {
-    (void)desc;
     if (!sf || (n_atoms > 0 && (!d_coordinates || !dE_dzeta)))
         return ldsf::fail(LDSF_ERR_ARG, "gradient: bad arguments");
     HostCall c{sf, n_atoms, {}, species, neighbor_list, number_of_neighs, coordinates};
-    return host_run(c, nullptr, dE_dzeta, d_coordinates, sf->precision);
+    return host_run(c, desc, dE_dzeta, d_coordinates, sf->precision);
 }
 
 LDSF_API int ldsf_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
@@ -587,12 +653,11 @@ LDSF_API int ldsf_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms, cons
                                        double *desc, const double *dE_dzeta)
 {
     (void)n_atoms;
-    (void)desc;
     if (!sf || !d_coordinates || !dE_dzeta || index < 0)
         return ldsf::fail(LDSF_ERR_ARG, "gradient_single_atom: bad arguments");
     int nn = number_of_neighs;
     HostCall c{sf, 1, {index}, species, neighbor_list, &nn, coordinates};
-    return host_run(c, nullptr, dE_dzeta, d_coordinates, sf->precision);
+    return host_run(c, desc, dE_dzeta, d_coordinates, sf->precision);
 }
 
 LDSF_API int ldsf_gradient_batch(ldsf_sf *sf, int n_configurations, const int *n_atoms,
@@ -601,7 +666,6 @@ LDSF_API int ldsf_gradient_batch(ldsf_sf *sf, int n_configurations, const int *n
                                  const double *coordinates, double *d_coordinates, double *desc,
                                  const double *dE_dzeta)
 {
-    (void)desc;
     if (!sf || n_configurations < 0 || (n_configurations > 0 && (!n_atoms || !config_ptr)))
         return ldsf::fail(LDSF_ERR_ARG, "gradient_batch: bad arguments");
     int n = 0, rc = 0;
@@ -609,7 +673,7 @@ LDSF_API int ldsf_gradient_batch(ldsf_sf *sf, int n_configurations, const int *n
     if (rc) return rc;
     if (n > 0 && (!d_coordinates || !dE_dzeta)) return ldsf::fail(LDSF_ERR_ARG, "gradient_batch: null output");
     HostCall c{sf, n, std::move(ctr), species, neighbor_list, number_of_neighs, coordinates};
-    return host_run(c, nullptr, dE_dzeta, d_coordinates, sf->precision);
+    return host_run(c, desc, dE_dzeta, d_coordinates, sf->precision);
 }
 
 LDSF_API int ldsf_jacobian(ldsf_sf *sf, int n_atoms, int n_total_atoms, const int *species,

@@ -79,11 +79,14 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
                            DevCounts dc = DevCounts());
 // The adjoint kernels write the vector of edge e (and of centre c, as edge
 // n_edges + c) at record inv_pos[e] of `edge`, the order the gather reads.
+// With desc (or dc.out_row) set, the fused kernel runs: it also adds each
+// centre's descriptors into desc (or assigns them to dc.out_row[c]), from the
+// same pair and triplet work.
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
                            int width, int max_nn, int group, hipStream_t s,
-                           DevCounts dc = DevCounts());
+                           DevCounts dc = DevCounts(), double *desc = nullptr);
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,

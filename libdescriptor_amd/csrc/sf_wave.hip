@@ -358,11 +358,20 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 constexpr int kStr = kW + 1;
 constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
 constexpr int kAdjDoubles = 11 * kStr + 48 + 48 + kWaveSlots;  // px..fz, w, wv, g
+constexpr int kFusedDoubles = kAdjDoubles + kWaveSlots;        // ... and d
 
-__host__ __device__ inline size_t group_bytes(int tmax, bool adjoint)
+// kernel kinds: the forward, the adjoint, and the fused forward + adjoint
+enum { kFwd = 0, kAdj = 1, kFused = 2 };
+
+__host__ __device__ constexpr int kind_doubles(int kind)
 {
-    size_t b = sizeof(double) * (adjoint ? kAdjDoubles : kFwdDoubles) +
-               sizeof(int) * (adjoint ? 2 : 1) * kW + sizeof(unsigned short) * tmax;
+    return kind == kFwd ? kFwdDoubles : (kind == kAdj ? kAdjDoubles : kFusedDoubles);
+}
+
+__host__ __device__ inline size_t group_bytes(int tmax, int kind)
+{
+    size_t b = sizeof(double) * kind_doubles(kind) + sizeof(int) * (kind == kFwd ? 1 : 2) * kW +
+               sizeof(unsigned short) * tmax;
     return (b + 15) & ~size_t(15);
 }
 
@@ -378,8 +387,9 @@ __device__ __forceinline__ Shared carve_shared(double *base, int ns)
     return S;
 }
 
-__device__ __forceinline__ Grp carve_group(double *base, int ns, bool adjoint)
+__device__ __forceinline__ Grp carve_group(double *base, int ns, int kind)
 {
+    const bool adjoint = kind != kFwd;
     double *p = base + shared_doubles(ns);
     Grp L;
     L.px = p;
@@ -397,12 +407,12 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, bool adjoint)
         L.w = p + 11 * kStr;
         L.wv = p + 11 * kStr + 48;
         L.g = p + 11 * kStr + 96;
-        L.d = nullptr;
+        L.d = kind == kFused ? p + kAdjDoubles : nullptr;
     } else {
         L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
         L.d = p + 7 * kStr;
     }
-    int *ip = reinterpret_cast<int *>(p + (adjoint ? kAdjDoubles : kFwdDoubles));
+    int *ip = reinterpret_cast<int *>(p + kind_doubles(kind));
     L.sp = ip;
     L.ip = adjoint ? ip + kW : nullptr;
     L.l0 = reinterpret_cast<unsigned short *>(ip + (adjoint ? 2 : 1) * kW);
@@ -807,11 +817,13 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 // One active triplet of the FP32-bulk adjoint: the derivatives of the row
 // sums with respect to r_ij, r_ik and r_jk, turned into vectors and added
 // into the two neighbours' FP64 LDS accumulators.  w / wv: the block's row
-// weights dE/dzeta 2^(1-zeta) and those times zeta lambda.
-template <int NPB>
+// weights dE/dzeta 2^(1-zeta) and those times zeta lambda.  FWD: the triplet's
+// descriptor terms B^zeta exp(..) fc fc fc (without 2^(1-zeta)) are also added
+// into the lane's row accumulators `acc`, from the same powers.
+template <int NPB, bool FWD>
 __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, const Grp &L, int ns,
                                               float rc, double rc2, float rcs, int a, int b,
-                                              const float *w, const float *wv)
+                                              const float *w, const float *wv, float (&acc)[16])
 {
     const double2 *D = reinterpret_cast<const double2 *>(L.px);
     const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
@@ -837,6 +849,12 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     E[0] = ea.x * eb.x * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2);
     E[1] = ea.y * eb.y * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2);
     float U[2] = {0.0f, 0.0f}, Vs[2] = {0.0f, 0.0f};
+    float te[2] = {0.0f, 0.0f};
+    if (FWD) {
+        const float fc3f = fa * fb * fjk;
+        te[0] = E[0] * fc3f;
+        te[1] = E[1] * fc3f;
+    }
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
@@ -849,6 +867,7 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
             for (int e = 0; e < 2; ++e) {
                 U[e] = fmaf(w[(z * 2 + l) * 2 + e], pw, U[e]);
                 Vs[e] = fmaf(wv[(z * 2 + l) * 2 + e], pm, Vs[e]);
+                if (FWD) acc[(z * 2 + l) * 2 + e] = fmaf(pw, te[e], acc[(z * 2 + l) * 2 + e]);
             }
         }
     }
@@ -890,7 +909,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
     const int s = threadIdx.x;
     const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, false);
+    const Grp L = carve_group(lds_base, ns, kFwd);
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
@@ -1158,8 +1177,13 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
     }
 }
 
-template <int P, int NSB, int NPB>
-__global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
+// The adjoint kernel.  FWD = true is the fused forward + adjoint: the
+// descriptor row is accumulated from the same pair and triplet work (the
+// reference's gradient adds the forward into desc as well,
+// src/Descriptors.cpp:165-190), its triplet lanes dealt the forward kernel's
+// 4-lane columns so that the row sums reduce the same way.
+template <int P, int NSB, int NPB, bool FWD>
+__global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
                                                  const int *__restrict__ nl,
@@ -1167,7 +1191,8 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const double *__restrict__ x,
                                                  const double *__restrict__ dE,
                                                  const int *__restrict__ inv_pos,
-                                                 double *__restrict__ edge, int n_edges, int width,
+                                                 double *__restrict__ edge, int n_edges,
+                                                 double *__restrict__ desc, int width,
                                                  int maxn, DevCounts dc)
 {
     using T = typename Real<P>::T;
@@ -1176,7 +1201,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
     const int s = threadIdx.x;
     const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, true);
+    const Grp L = carve_group(lds_base, ns, FWD ? kFused : kAdj);
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
@@ -1193,20 +1218,34 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         return dc.in_row ? dc.in_row[C.c][s] : dE[(size_t)C.c * width + s];
     };
     auto self_pos = [&](const Ctr &C) -> int { return C.live ? inv_pos[n_edges + C.c] : 0; };
+    // FWD: entry s of the centre's descriptor row before this call (+=), or 0
+    // where the serving loop assigns (=)
+    const bool assign = dc.out_row != nullptr;
+    auto row_in = [&](const Ctr &C) -> double {
+        return (FWD && !assign && C.work && s < width) ? desc[(size_t)C.c * width + s] : 0.0;
+    };
     Ctr C = load_centre(tr.first, n_centers, centers, offs, species, x, dc);
     double gs = grad_of(C);
+    double rin = row_in(C);
     int ips = self_pos(C);
     int ipe = s < C.N ? inv_pos[C.beg + s] : 0;  // record position of lane s's edge
     Nbr B = load_nbr(load_index(C, s, nl), species, x);
     for (int c = tr.first; c < tr.hi; c += tr.step) {
         const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
         const double gsn = grad_of(Cn);
+        const double rinn = row_in(Cn);
         const int ipsn = self_pos(Cn);
+        double *row = (FWD && C.work) ? (dc.out_row ? dc.out_row[c] : desc + (size_t)c * width)
+                                      : nullptr;
         // a skipped centre (flagged request) writes only its zero self record
         if (C.work && s < width) L.g[s] = gs;
+        if (FWD && s < kWaveSlots) L.d[s] = rin;
         __syncthreads();
 
         // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
+        double acc2[NSB * 8];
+#pragma unroll
+        for (int k = 0; k < NSB * 8; ++k) acc2[k] = 0.0;
         bool valid = false;
         double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
         const int sj = B.sj;
@@ -1243,6 +1282,15 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                     const T ex = dexp(-wp.eta2[q] * dr * dr, R);
                     const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
                     sum += L.g[slot] * (double)dphi;
+                    if (FWD) {  // the g2 term, as the forward kernel forms it
+                        const double v = (double)(ex * fc);
+                        if (NSB == 1) {
+                            acc2[q] += v;
+                        } else {
+                            acc2[q] += sj ? 0.0 : v;
+                            acc2[8 + q] += sj ? v : 0.0;
+                        }
+                    }
                 }
             }
             const double ir = 1.0 / r;
@@ -1272,6 +1320,7 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         }
         const int jn = load_index(Cn, s, nl);
         const int ipen = s < Cn.N ? inv_pos[Cn.beg + s] : 0;
+        if (FWD) reduce_add<NSB * 8>(acc2, s, wp.slots, L.d);
         __syncthreads();
 
         // ---- triplets: the active pairs of every block, lanes split over the blocks ----
@@ -1290,43 +1339,84 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
             }
             __syncthreads();
-            // the lanes are split over the species-pair blocks in proportion to
-            // their active pairs: the fewest passes I with sum_p ceil(cnt_p / I)
-            // <= 64 lanes; lane s serves block p with stride lanes(p)
             const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
-            int passes = (nall + kW - 1) / kW, ln[3] = {kW, 0, 0};
-            if (NPB > 1 && nall > 0) {
-                auto lanes_for = [&](int I) {
-                    ln[0] = (cnt[0] + I - 1) / I;
-                    ln[1] = (cnt[1] + I - 1) / I;
-                    ln[2] = (cnt[2] + I - 1) / I;
-                    return ln[0] + ln[1] + ln[2];
-                };
-                while (lanes_for(passes) > kW) ++passes;
+            int passes = (nall + kW - 1) / kW;
+            int pl, u, stride, nb;
+            const unsigned short *lst;
+            bool mine;
+            // FWD: columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48; column s & 15)
+            // dealt to the blocks in proportion to their active pairs, the
+            // fewest passes I with sum_p ceil(cnt_p / 4I) <= 16 columns.
+            // Otherwise lanes dealt in proportion: the fewest passes I with
+            // sum_p ceil(cnt_p / I) <= 64 lanes; lane s serves block p with
+            // stride lanes(p).
+            int nc[3] = {16, 0, 0};
+            const int col = s & 15, rowq = s >> 4;
+            if (FWD) {
+                if (NPB > 1 && nall > 0) {
+                    auto cols_for = [&](int I) {
+                        nc[0] = (cnt[0] + 4 * I - 1) / (4 * I);
+                        nc[1] = (cnt[1] + 4 * I - 1) / (4 * I);
+                        nc[2] = (cnt[2] + 4 * I - 1) / (4 * I);
+                        return nc[0] + nc[1] + nc[2];
+                    };
+                    while (cols_for(passes) > 16) ++passes;
+                }
+                pl = NPB == 1 ? 0 : (col < nc[0] ? 0 : (col < nc[0] + nc[1] ? 1 : 2));
+                const int cs = pl == 0 ? 0 : (pl == 1 ? nc[0] : nc[0] + nc[1]);
+                const int ncol = pl == 0 ? nc[0] : (pl == 1 ? nc[1] : nc[2]);
+                u = NPB == 1 ? s : (col - cs) + rowq * ncol;
+                stride = NPB == 1 ? kW : 4 * ncol;
+                mine = NPB == 1 || col < cs + ncol;
+            } else {
+                int ln[3] = {kW, 0, 0};
+                if (NPB > 1 && nall > 0) {
+                    auto lanes_for = [&](int I) {
+                        ln[0] = (cnt[0] + I - 1) / I;
+                        ln[1] = (cnt[1] + I - 1) / I;
+                        ln[2] = (cnt[2] + I - 1) / I;
+                        return ln[0] + ln[1] + ln[2];
+                    };
+                    while (lanes_for(passes) > kW) ++passes;
+                }
+                pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
+                u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
+                stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
+                mine = u < stride;
             }
-            const int pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
-            const int u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
-            const int stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
-            const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
-            const unsigned short *lst =
-                L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
+            nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
+            lst = L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
             const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
             const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
             // this lane's species-pair cutoff (FP32-bulk)
             const float rcl = P == 1 ? blk3(pl, F.rc0, F.rc1, F.rc2) : 0.0f;
             const double rc2l = P == 1 ? blk3(pl, F.rq0, F.rq1, F.rq2) : 0.0;
             const float rcsl = P == 1 ? blk3(pl, F.rs0, F.rs1, F.rs2) : 0.0f;
+            // FWD: the lane's row sums (FP32-bulk: FP32 per lane, as the forward kernel)
+            float accf[16];
+            double accd[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                accf[k] = 0.0f;
+                accd[k] = 0.0;
+            }
             for (int i = 0; i < passes; ++i) {
                 const int t = u + i * stride;
-                if (u < stride && t < nb) {
+                if (mine && t < nb) {
                     const int code = lst[t];
                     const int a = code & 63, b = (code >> 6) & 63;
                     if constexpr (P == 1) {
-                        adj_triplet_f<NPB>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv);
+                        adj_triplet_f<NPB, FWD>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accf);
                     } else {
                         Trip<P> q;
                         triplet<P, true>(S, wp, wt, L, ns, a, b, q);
                         const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
+                        const T fc3 = q.fa * q.fb * q.fjk;
+                        double te[2] = {0.0, 0.0};
+                        if (FWD) {
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) te[e] = (double)(q.E[e] * fc3);
+                        }
                         T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
@@ -1342,6 +1432,14 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                                         for (int e = 0; e < 2; ++e) {
                                             U[e] += w[(z * 2 + l) * 2 + e] * pw;
                                             V[e] += wv[(z * 2 + l) * 2 + e] * pm1;
+                                        }
+                                        if (FWD) {
+                                            const double pz = (double)((T)wp.coef[z] * pw);
+#pragma unroll
+                                            for (int e = 0; e < 2; ++e)
+                                                if (e < wt.ne)
+                                                    accd[(z * 2 + l) * 2 + e] =
+                                                        fma(pz, te[e], accd[(z * 2 + l) * 2 + e]);
                                         }
                                     }
                                 }
@@ -1361,7 +1459,6 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                         const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
                         const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
                         const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
-                        const T fc3 = q.fa * q.fb * q.fjk;
                         const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
                         const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
                         const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
@@ -1377,6 +1474,58 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
                 }
             }
             __syncthreads();
+            if (FWD) {
+                // the row sums: a column's 4 lanes with two register swaps (FP32-bulk:
+                // in FP32), then the columns of a block in FP64 through LDS, in
+                // order -- the forward kernel's reduction; lane s keeps values
+                // 4 (s >> 4) .. +3.  The neighbour arrays are dead here.
+                double v4[4];
+                if constexpr (P == 1) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        float a = accf[q], b = accf[q + 8];
+                        xswapf<32>(a, b);
+                        accf[q] = a + b;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float a = accf[q], b = accf[q + 4];
+                        xswapf<16>(a, b);
+                        accf[q] = a + b;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v4[q] = (double)accf[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        double a = accd[q], b = accd[q + 8];
+                        xswap<32>(a, b);
+                        accd[q] = a + b;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        double a = accd[q], b = accd[q + 4];
+                        xswap<16>(a, b);
+                        accd[q] = a + b;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v4[q] = accd[q];
+                }
+                double *buf = L.px;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) buf[20 * (4 * rowq + q) + col] = v4[q];
+                __syncthreads();
+                if (rowq < NPB) {
+                    const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
+                    const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
+                    double sum = 0.0;
+                    for (int cc = c0; cc < c1; ++cc) sum += buf[20 * col + cc];
+                    const int slot = wp.slots[16 + s];
+                    // the FP32-bulk rows take their 2^(1 - zeta) here
+                    if (P == 1) sum *= wp.coef[col >> 2];
+                    if (c1 > c0 && slot >= 0) L.d[slot] += sum;
+                }
+            }
         } else {
             Bn = load_nbr(jn, species, x);
         }
@@ -1396,9 +1545,11 @@ __global__ __launch_bounds__(64) void k_adj_wave(WaveTab wt, int n_centers,
         wave_reduce<4>(sv, s);
         if (C.live && (s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ips + (s >> 4)] = sv[0];
         __syncthreads();
+        if (FWD && row && s < width) row[s] = L.d[s];
         C = Cn;
         B = Bn;
         gs = gsn;
+        rin = rinn;
         ips = ipsn;
         ipe = ipen;
     }
@@ -1411,11 +1562,11 @@ int grid_for(int n_centers, int cap)
     return std::max(g, 1);
 }
 
-size_t lds_for(int max_nn, bool adjoint, int ns)
+size_t lds_for(int max_nn, int kind, int ns)
 {
     const int maxn = std::max(max_nn, 1);
     const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * shared_doubles(ns) + group_bytes(tmax, adjoint);
+    return sizeof(double) * shared_doubles(ns) + group_bytes(tmax, kind);
 }
 
 }  // namespace
@@ -1424,13 +1575,13 @@ int wave_group_size(int max_nn, int n_species)
 {
     // one wavefront per centre, one lane per neighbour-list entry
     if (max_nn > kW) return 0;
-    return lds_for(max_nn, true, n_species) <= kWaveMaxLds ? kW : 0;
+    return lds_for(max_nn, kFused, n_species) <= kWaveMaxLds ? kW : 0;
 }
 
 size_t wave_lds_bytes(int max_nn, bool adjoint, int n_species, int group)
 {
     (void)group;
-    return lds_for(max_nn, adjoint, n_species);
+    return lds_for(max_nn, adjoint ? kAdj : kFwd, n_species);
 }
 
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
@@ -1521,7 +1672,7 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (n_centers <= 0) return hipSuccess;
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const size_t lds = lds_for(max_nn, false, wt.n_species);
+    const size_t lds = lds_for(max_nn, kFwd, wt.n_species);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
 #define LDSF_FWD(P, NSB, NPB)                                                                  \
     hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers, \
@@ -1538,20 +1689,31 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
-                           int width, int max_nn, int group, hipStream_t s, DevCounts dc)
+                           int width, int max_nn, int group, hipStream_t s, DevCounts dc,
+                           double *desc)
 {
     if (n_centers <= 0) return hipSuccess;
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const size_t lds = lds_for(max_nn, true, wt.n_species);
+    const bool fused = desc != nullptr || dc.out_row != nullptr;
+    const size_t lds = lds_for(max_nn, fused ? kFused : kAdj, wt.n_species);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
-#define LDSF_ADJ(P, NSB, NPB)                                                                  \
-    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers, \
-                       offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, dc)
-    if (wt.nsb == 1) {
-        if (precision == 0) LDSF_ADJ(0, 1, 1); else LDSF_ADJ(1, 1, 1);
+#define LDSF_ADJ(P, NSB, NPB, FWD)                                                              \
+    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD>), grid, block, lds, s, wt, n_centers,      \
+                       centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, width, \
+                       maxn, dc)
+    if (fused) {
+        if (wt.nsb == 1) {
+            if (precision == 0) LDSF_ADJ(0, 1, 1, true); else LDSF_ADJ(1, 1, 1, true);
+        } else {
+            if (precision == 0) LDSF_ADJ(0, 2, 3, true); else LDSF_ADJ(1, 2, 3, true);
+        }
     } else {
-        if (precision == 0) LDSF_ADJ(0, 2, 3); else LDSF_ADJ(1, 2, 3);
+        if (wt.nsb == 1) {
+            if (precision == 0) LDSF_ADJ(0, 1, 1, false); else LDSF_ADJ(1, 1, 1, false);
+        } else {
+            if (precision == 0) LDSF_ADJ(0, 2, 3, false); else LDSF_ADJ(1, 2, 3, false);
+        }
     }
 #undef LDSF_ADJ
     return hipGetLastError();

@@ -1,0 +1,162 @@
+"""The fused forward + adjoint (ldsf_plan_forward_gradient, and the host
+ldsf_gradient, which re-adds the forward into desc as the reference's Enzyme
+call does, src/Descriptors.cpp:165-190) against the oracle and against the
+separate forward and adjoint kernels.
+
+Tolerances: FP64 descriptors elementwise relative <= 1e-12 against the oracle
+and <= 1e-14 against the separate forward kernel (the fused kernel forms
+B^zeta as B^(zeta-1) B); FP64 gradients normwise <= 1e-12; FP32-bulk
+descriptors elementwise relative <= 1e-5 on entries carrying signal
+(|ref| >= 1e-6 of the block maximum), gradients normwise <= 1e-5."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import libdescriptor_amd as ld
+from conftest import behler, elem_rel_err, rel_err
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+TOL64, TOL32 = 1e-12, 1e-5
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return torch.device("cuda:0")
+
+
+def signal_rel(a, ref, blocks):
+    worst = 0.0
+    for lo, hi in blocks:
+        r = ref[:, lo:hi]
+        scale = np.abs(r).max()
+        if scale == 0:
+            continue
+        sig = np.abs(r) >= 1e-6 * scale
+        worst = max(worst, float((np.abs(a[:, lo:hi] - r)[sig] / np.abs(r)[sig]).max()))
+    return worst
+
+
+def two_species(n, seed):
+    from libdescriptor_amd import workloads as w
+    pos, cell, pbc, sp = w.random_system(n, 0.05, 1.8, seed=seed)
+    return w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+
+
+class Setup:
+    def __init__(self, L, species=("A", "B"), rc=5.0, resolved=True):
+        d = _dev()
+        types, sizes, params = behler()
+        self.ds = ld.DescriptorKind.init_symmetry_functions(list(species), rc, types, sizes,
+                                                            list(params), species_resolved=resolved)
+        self.O = orc.OracleSF(list(species), rc, types, sizes, params, species_resolved=resolved)
+        from libdescriptor_amd import _capi
+        t = lambda a, dt=torch.int32: torch.tensor(np.asarray(a), dtype=dt, device=d)  # noqa: E731
+        self.L = L
+        self.n = L["n_contrib"]
+        self.x = t(L["coords"], torch.float64)
+        self.plan = _capi.Plan(self.ds._handle, len(L["coords"]), t(L["species"]),
+                               t(L["neighlist"]), t(L["numneigh"]))
+        self.W = self.ds.width
+        self.g = t(np.random.default_rng(5).normal(size=(self.n, self.W)), torch.float64)
+
+    def split(self, prec):
+        desc = torch.zeros((self.n, self.W), dtype=torch.float64, device=self.x.device)
+        dx = torch.zeros_like(self.x)
+        self.plan.compute(self.x, desc, prec)
+        self.plan.gradient(self.x, self.g, dx, prec)
+        torch.cuda.synchronize()
+        return desc.cpu().numpy(), dx.cpu().numpy()
+
+    def fused(self, prec, desc0=None):
+        desc = torch.zeros((self.n, self.W), dtype=torch.float64, device=self.x.device) \
+            if desc0 is None else torch.tensor(desc0, device=self.x.device)
+        dx = torch.zeros_like(self.x)
+        self.plan.forward_gradient(self.x, self.g, desc, dx, prec)
+        torch.cuda.synchronize()
+        return desc.cpu().numpy(), dx.cpu().numpy()
+
+    def oracle(self):
+        L = self.L
+        d0 = self.O.compute(self.n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
+        G0 = self.O.gradient(self.n, L["species"], L["neighlist"], L["numneigh"], L["coords"],
+                             self.g.cpu().numpy())
+        return d0, G0
+
+
+BLOCKS = [(0, 8), (8, 16), (16, 32), (32, 48), (48, 64)]
+
+
+@pytest.mark.parametrize("family", [1, 0])
+def test_fused_two_species_against_oracle(family):
+    print(f"case: 400-atom two-species, family {family}", flush=True)
+    S = Setup(two_species(400, 11))
+    if family == 0:
+        S.plan.kernel_family = 0
+    d0, G0 = S.oracle()
+    d, G = S.fused(0)
+    assert elem_rel_err(d, d0, 1e-300) <= TOL64
+    assert rel_err(G, G0) <= TOL64
+    ds_, Gs = S.split(0)
+    assert elem_rel_err(d, ds_, 1e-300) <= 1e-14
+    assert rel_err(G, Gs) <= 1e-14
+    d32, G32 = S.fused(1)
+    assert signal_rel(d32, d0, BLOCKS) <= TOL32
+    assert rel_err(G32, G0) <= TOL32
+    # deterministic
+    d2, G2 = S.fused(1)
+    np.testing.assert_array_equal(d2, d32)
+    np.testing.assert_array_equal(G2, G32)
+
+
+def test_fused_accumulates_into_desc(c1):
+    """desc += descriptors (the reference's += semantics), single species."""
+    print("case: C1 fused +=", flush=True)
+    L = {k: c1[k] for k in ("coords", "species", "neighlist", "numneigh")}
+    L["n_contrib"] = 64
+    S = Setup(L, species=("Si",), rc=4.5, resolved=False)
+    base = np.random.default_rng(2).normal(size=(64, 24))
+    d, G = S.fused(0, base)
+    assert elem_rel_err(d - base, c1["desc"], 1e-300) <= 1e-11
+    d0, G0 = S.oracle()
+    assert rel_err(G, G0) <= TOL64
+
+
+def test_host_gradient_readds_forward(c1):
+    """ldsf_gradient with a desc buffer: desc += descriptors, d_coords += J^T g."""
+    from libdescriptor_amd import _capi
+    print("case: C1 host ldsf_gradient with desc", flush=True)
+    _dev()
+    ds = ld.DescriptorKind.init_descriptor(ld.AvailableDescriptors.SymmetryFunctions, ["Si"], "cos",
+                                           4.5, *behler())
+    sp = np.ascontiguousarray(c1["species"], np.int32)
+    nl = np.ascontiguousarray(c1["neighlist"], np.int32)
+    nn = np.ascontiguousarray(c1["numneigh"], np.int32)
+    x = np.ascontiguousarray(c1["coords"])
+    g = np.ascontiguousarray(c1["dE_dzeta"])
+    desc = np.ones((64, 24))
+    dx = np.zeros_like(x)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    _capi.check(_capi.lib().ldsf_gradient(ctypes.c_void_p(ds._handle), 64, p(sp), p(nl), p(nn),
+                                          p(x), p(dx), p(desc), p(g)))
+    assert elem_rel_err(desc - 1.0, c1["desc"], 1e-300) <= 1e-11
+    G0 = orc.OracleSF(["Si"], 4.5, *behler()).gradient(64, sp, nl, nn, x, g)
+    assert rel_err(dx, G0) <= TOL64
+
+
+def test_fused_full_size_262144():
+    """The benchmark's system: fused against the separate kernels, both forms."""
+    from libdescriptor_amd import workloads as w
+    print("case: 262144-atom two-species fused vs split", flush=True)
+    S = Setup(w.config_c3(n=262144, seed=0))
+    d64, G64 = S.split(0)
+    df, Gf = S.fused(0)
+    assert elem_rel_err(df, d64, 1e-300) <= 1e-13
+    assert rel_err(Gf, G64) <= 1e-13
+    d32, G32 = S.fused(1)
+    assert signal_rel(d32, d64, BLOCKS) <= TOL32
+    assert rel_err(G32, G64) <= TOL32
