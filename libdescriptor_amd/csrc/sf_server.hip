@@ -20,9 +20,10 @@
 //   lane: rows, centres, neighbour entries offset to batch rows; per centre
 //   the addresses of its descriptor and dE/dzeta rows in the request's own
 //   buffers) -> neighbour sums check -> CSR offsets per request -> transposed
-//   CSR (count, scan, fill, per-row sort, inverse) -> forward (straight into
-//   each request's desc) -> adjoint -> gather straight into each request's
-//   d_coordinates -> flags D2H.
+//   CSR (count, scan, fill, per-row sort, inverse) -> the fused forward +
+//   adjoint kernel (descriptors straight into each request's desc; the
+//   general kernels of the largest bucket run as forward, then adjoint) ->
+//   gather straight into each request's d_coordinates -> flags D2H.
 // No memset nodes (zeroing is done by kernels).  A request with an index,
 // species code or neighbour count out of range is flagged on the device; the
 // kernels skip its centres, so nothing reads out of bounds, and its wait
@@ -515,9 +516,8 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
             return e;
     } else {
         const int maxn = 8 * (bucket + 1);
-        if ((e = launch_fwd_wave(S->wave, prec, c.lane_centers, L.centres, L.offs, L.nl,
-                                 L.species, L.x, nullptr, S->width, maxn, 64, s, dc)))
-            return e;
+        // the fused kernel: descriptors (assigned through dc.out_row) and the
+        // adjoint's records from one pass over the pairs and triplets
         if ((e = launch_adj_wave(S->wave, prec, c.lane_centers, L.centres, L.offs, L.nl,
                                  L.species, L.x, nullptr, L.inv_pos, L.edge, c.lane_edges, S->width,
                                  maxn, 64, s, dc)))

@@ -3,7 +3,7 @@ requests over device-resident buffers into graph-replayed lanes.
 
 Checks:
   every result of a request served in a mix equals the same request run alone
-  through a device plan -- bitwise (each centre is computed on its own and
+  through a device plan (ldsf_plan_forward_gradient) -- bitwise (each centre is computed on its own and
   each row's dE/dr records are summed in ascending edge order in both paths),
   in both precision forms;
   a request with an out-of-range neighbour index is rejected (LDSF_ERR_NEIGHBOR)
@@ -54,8 +54,8 @@ def alone(ds, prec, D):
     n = D["numneigh"].numel()
     d = torch.zeros((n, ds.width), dtype=torch.float64, device="cuda")
     g = torch.zeros_like(D["x"])
-    plan.compute(D["x"], d, prec)
-    plan.gradient(D["x"], D["dE"], g, prec)
+    # descriptors and dE/dr from one pass, as the server computes them
+    plan.forward_gradient(D["x"], D["dE"], d, g, prec)
     torch.cuda.synchronize()
     return d, g
 
