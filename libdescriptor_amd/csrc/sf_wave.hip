@@ -86,18 +86,6 @@ __device__ __forceinline__ void xswap(double &a, double &b)
     b = __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
 }
 
-template <int M>
-__device__ __forceinline__ void xswapf(float &a, float &b)
-{
-    static_assert(M == 32 || M == 16, "permlane swaps exchange 32- or 16-lane halves");
-    const auto r = M == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
-                                                              false, false)
-                           : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
-                                                              false, false);
-    a = __uint_as_float(r[0]);
-    b = __uint_as_float(r[1]);
-}
-
 // M <= 8: the partner lane's value through DPP: lane ^ 1 and lane ^ 2 (quad
 // permutations), the mirror lane in its 8-lane half row (differs in bit 2)
 // and in its 16-lane row (differs in bit 3).
@@ -823,7 +811,7 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 template <int NPB, bool FWD>
 __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, const Grp &L, int ns,
                                               float rc, double rc2, float rcs, int a, int b,
-                                              const float *w, const float *wv, float (&acc)[16])
+                                              const float *w, const float *wv, double (&acc)[16])
 {
     const double2 *D = reinterpret_cast<const double2 *>(L.px);
     const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
@@ -849,11 +837,11 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     E[0] = ea.x * eb.x * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2);
     E[1] = ea.y * eb.y * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2);
     float U[2] = {0.0f, 0.0f}, Vs[2] = {0.0f, 0.0f};
-    float te[2] = {0.0f, 0.0f};
+    double te[2] = {0.0, 0.0};  // FWD: FP32 terms, summed in FP64
     if (FWD) {
         const float fc3f = fa * fb * fjk;
-        te[0] = E[0] * fc3f;
-        te[1] = E[1] * fc3f;
+        te[0] = (double)(E[0] * fc3f);
+        te[1] = (double)(E[1] * fc3f);
     }
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
@@ -867,7 +855,7 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
             for (int e = 0; e < 2; ++e) {
                 U[e] = fmaf(w[(z * 2 + l) * 2 + e], pw, U[e]);
                 Vs[e] = fmaf(wv[(z * 2 + l) * 2 + e], pm, Vs[e]);
-                if (FWD) acc[(z * 2 + l) * 2 + e] = fmaf(pw, te[e], acc[(z * 2 + l) * 2 + e]);
+                if (FWD) acc[(z * 2 + l) * 2 + e] = fma((double)pw, te[e], acc[(z * 2 + l) * 2 + e]);
             }
         }
     }
@@ -1039,9 +1027,10 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                     const bool rc_pair = NPB == 1 && ns > 1;  // cutoff per species pair
                     const double2 *D = reinterpret_cast<const double2 *>(L.px);
                     const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
-                    float acc[16];
+                    // FP32 terms, FP64 sums (per lane, then over lanes and columns)
+                    double acc[16];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) acc[k] = 0.0f;
+                    for (int k = 0; k < 16; ++k) acc[k] = 0.0;
                     for (int i = 0; i < passes; ++i) {
                         const int t = u + i * stride;
                         if (mine && t < nb) {
@@ -1063,39 +1052,38 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                             const float inv = va.x * vb.x;  // 1 / (2 r_ij r_ik)
                             const double sp = (a1.y + b1.y) * (a1.y + b1.y);
                             const double sm = (a1.y - b1.y) * (a1.y - b1.y);
-                            float te[2];
-                            te[0] = va.y * vb.y * (fjk * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2));
-                            te[1] = va.z * vb.z * (fjk * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2));
+                            double te[2];
+                            te[0] = (double)(va.y * vb.y * (fjk * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2)));
+                            te[1] = (double)(va.z * vb.z * (fjk * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2)));
 #pragma unroll
                             for (int l = 0; l < 2; ++l) {
                                 const float lb = __builtin_amdgcn_logf(base_f(F, l, sp, sm, g.rjk2d, inv));
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
-                                    const float pw = __builtin_amdgcn_exp2f(F.zeta[z] * lb);
+                                    const double pw = (double)__builtin_amdgcn_exp2f(F.zeta[z] * lb);
 #pragma unroll
                                     for (int e = 0; e < 2; ++e)
-                                        acc[(z * 2 + l) * 2 + e] = fmaf(pw, te[e], acc[(z * 2 + l) * 2 + e]);
+                                        acc[(z * 2 + l) * 2 + e] = fma(pw, te[e], acc[(z * 2 + l) * 2 + e]);
                                 }
                             }
                         }
                     }
-                    // a lane's few terms and its column's 4 lanes (value halves
-                    // swapped between lane halves, then rows) are summed in FP32,
-                    // the columns in FP64: lane s keeps values 4 (s >> 4) .. +3
+                    // a column's 4 lanes (value halves swapped between lane
+                    // halves, then rows), in FP64: lane s keeps values 4 (s >> 4) .. +3
 #pragma unroll
                     for (int q = 0; q < 8; ++q) {
-                        float a = acc[q], b = acc[q + 8];
-                        xswapf<32>(a, b);
+                        double a = acc[q], b = acc[q + 8];
+                        xswap<32>(a, b);
                         acc[q] = a + b;
                     }
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        float a = acc[q], b = acc[q + 4];
-                        xswapf<16>(a, b);
+                        double a = acc[q], b = acc[q + 4];
+                        xswap<16>(a, b);
                         acc[q] = a + b;
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) v4[q] = (double)acc[q];
+                    for (int q = 0; q < 4; ++q) v4[q] = acc[q];
                 } else {
                     double acc3[16];
 #pragma unroll
@@ -1392,21 +1380,17 @@ __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(Wav
             const float rcl = P == 1 ? blk3(pl, F.rc0, F.rc1, F.rc2) : 0.0f;
             const double rc2l = P == 1 ? blk3(pl, F.rq0, F.rq1, F.rq2) : 0.0;
             const float rcsl = P == 1 ? blk3(pl, F.rs0, F.rs1, F.rs2) : 0.0f;
-            // FWD: the lane's row sums (FP32-bulk: FP32 per lane, as the forward kernel)
-            float accf[16];
+            // FWD: the lane's row sums, FP64 in both forms
             double accd[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                accf[k] = 0.0f;
-                accd[k] = 0.0;
-            }
+            for (int k = 0; k < 16; ++k) accd[k] = 0.0;
             for (int i = 0; i < passes; ++i) {
                 const int t = u + i * stride;
                 if (mine && t < nb) {
                     const int code = lst[t];
                     const int a = code & 63, b = (code >> 6) & 63;
                     if constexpr (P == 1) {
-                        adj_triplet_f<NPB, FWD>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accf);
+                        adj_triplet_f<NPB, FWD>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
                     } else {
                         Trip<P> q;
                         triplet<P, true>(S, wp, wt, L, ns, a, b, q);
@@ -1475,42 +1459,25 @@ __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(Wav
             }
             __syncthreads();
             if (FWD) {
-                // the row sums: a column's 4 lanes with two register swaps (FP32-bulk:
-                // in FP32), then the columns of a block in FP64 through LDS, in
-                // order -- the forward kernel's reduction; lane s keeps values
+                // the row sums: a column's 4 lanes with two register swaps, then
+                // the columns of a block through LDS, in order, all in FP64 --
+                // the forward kernel's reduction; lane s keeps values
                 // 4 (s >> 4) .. +3.  The neighbour arrays are dead here.
                 double v4[4];
-                if constexpr (P == 1) {
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        float a = accf[q], b = accf[q + 8];
-                        xswapf<32>(a, b);
-                        accf[q] = a + b;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        float a = accf[q], b = accf[q + 4];
-                        xswapf<16>(a, b);
-                        accf[q] = a + b;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) v4[q] = (double)accf[q];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        double a = accd[q], b = accd[q + 8];
-                        xswap<32>(a, b);
-                        accd[q] = a + b;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        double a = accd[q], b = accd[q + 4];
-                        xswap<16>(a, b);
-                        accd[q] = a + b;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) v4[q] = accd[q];
+                for (int q = 0; q < 8; ++q) {
+                    double a = accd[q], b = accd[q + 8];
+                    xswap<32>(a, b);
+                    accd[q] = a + b;
                 }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double a = accd[q], b = accd[q + 4];
+                    xswap<16>(a, b);
+                    accd[q] = a + b;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v4[q] = accd[q];
                 double *buf = L.px;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) buf[20 * (4 * rowq + q) + col] = v4[q];
