@@ -65,7 +65,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--family", type=int, default=-1, help="force kernel family (0 general, 1 wave)")
