@@ -74,6 +74,9 @@ LDSF_API void ldsf_sf_destroy(ldsf_sf *sf);
 
 /* DescriptorKind::width (include/Descriptors.hpp:184) */
 LDSF_API int ldsf_sf_width(const ldsf_sf *sf);
+/* number of parameters: sum of rows x cols over the blocks (the length of
+ * initDescriptor's flat params vector) */
+LDSF_API int ldsf_sf_num_params(const ldsf_sf *sf);
 LDSF_API int ldsf_sf_num_species(const ldsf_sf *sf);
 /* species name k (sorted order for file init); NULL if out of range */
 LDSF_API const char *ldsf_sf_species_name(const ldsf_sf *sf, int k);
@@ -127,6 +130,18 @@ LDSF_API int ldsf_gradient_batch(ldsf_sf *sf, int n_configurations, const int *n
 LDSF_API int ldsf_jacobian(ldsf_sf *sf, int n_atoms, int n_total_atoms, const int *species,
                            const int *neighbor_list, const int *number_of_neighs,
                            const double *coordinates, double *J_coordinates);
+
+/* Hyperparameter gradients: what the reference's gradient computes into the
+ * Enzyme shadow of the descriptor object and then deletes unread
+ * (src/Descriptors.cpp:176-188, enzyme_dup desc_kind, d_desc_kind).
+ * dparams[ldsf_sf_num_params] += dE/d(parameters) in initDescriptor's flat
+ * layout (per block rows x cols, row-major: g2 (eta, Rs), g3 (kappa), g4/g5
+ * (zeta, lambda, eta); g1 has none); drcut[n_species^2] += dE/d(cutoff matrix)
+ * (rcut_2D_, row-major; NULL: not returned).  FP64; at most 8 species. */
+LDSF_API int ldsf_param_gradient(ldsf_sf *sf, int n_atoms, const int *species,
+                                 const int *neighbor_list, const int *number_of_neighs,
+                                 const double *coordinates, const double *dE_dzeta,
+                                 double *dparams, double *drcut);
 
 /* Descriptor::num_gradient_single_atom (Descriptors.hpp:101-103,
  * Descriptors.cpp:488-518, include/finite_difference.hpp:46-143):
@@ -186,6 +201,13 @@ LDSF_API int ldsf_plan_forward_gradient(ldsf_plan *plan, const double *d_coordin
 LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
                                              const double *d_dE_dzeta, double *d_desc,
                                              int precision, void *stream);
+
+/* ldsf_param_gradient on the device: d_dparams[ldsf_sf_num_params] and
+ * d_drcut[n_species^2] (may be NULL) +=.  Allocates its partial sums and
+ * synchronises `stream`. */
+LDSF_API int ldsf_plan_param_gradient(ldsf_plan *plan, const double *d_coordinates,
+                                      const double *d_dE_dzeta, double *d_dparams,
+                                      double *d_drcut, void *stream);
 
 /* The two halves of ldsf_plan_gradient.  ldsf_plan_adjoint_edges runs the
  * adjoint kernel and keeps one dE/dr vector per neighbour-list edge and per

@@ -25,7 +25,7 @@ try:
                                 compute_single_atom, create, create_paddings,
                                 get_neigh_kim, gradient, gradient_batch,
                                 gradient_single_atom, jacobian,
-                                num_gradient_single_atom)
+                                num_gradient_single_atom, param_gradient)
 except ImportError as e:  # pragma: no cover - exercised only without a build
     raise ImportError(
         "libdescriptor_amd: the native module is not built; run "
@@ -37,5 +37,5 @@ __all__ = [
     "AvailableDescriptors", "DescriptorKind", "NeighList", "FP64", "FP32_BULK", "compute",
     "compute_batch", "compute_single_atom", "create", "create_paddings", "get_neigh_kim",
     "gradient", "gradient_batch", "gradient_single_atom", "jacobian",
-    "num_gradient_single_atom", "NeighborList", "NeighborListError", "assemble_forces",
+    "num_gradient_single_atom", "param_gradient", "NeighborList", "NeighborListError", "assemble_forces",
 ]

@@ -25,6 +25,9 @@ _PROTOS = {
     "ldsf_sf_create_from_file": (_i, [_vp, ctypes.c_char_p]),
     "ldsf_sf_destroy": (None, [_vp]),
     "ldsf_sf_width": (_i, [_vp]),
+    "ldsf_sf_num_params": (_i, [_vp]),
+    "ldsf_param_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_plan_param_gradient": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_sf_num_species": (_i, [_vp]),
     "ldsf_sf_species_name": (ctypes.c_char_p, [_vp, _i]),
     "ldsf_sf_set_precision": (_i, [_vp, _i]),
@@ -210,6 +213,12 @@ class Plan:
         check(lib().ldsf_plan_forward_adjoint_edges(self.handle, _ptr(coords), _ptr(dE_dzeta),
                                                     _ptr(desc), int(precision),
                                                     _stream_ptr(stream)))
+
+    def param_gradient(self, coords, dE_dzeta, dparams, drcut=None, stream=None):
+        """dparams += dE/d(parameters) (initDescriptor's flat layout), drcut +=
+        dE/d(cutoff matrix); FP64, device tensors."""
+        check(lib().ldsf_plan_param_gradient(self.handle, _ptr(coords), _ptr(dE_dzeta),
+                                             _ptr(dparams), _ptr(drcut), _stream_ptr(stream)))
 
     def gather(self, dcoords, stream=None):
         """Second half of gradient(): dcoords += the per-row sums of the records."""

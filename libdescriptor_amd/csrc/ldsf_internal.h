@@ -47,6 +47,12 @@ struct SFTab {
     const int *izeta3;   // zeta as an integer in [0, kMaxIntZeta], or -1
     const int *slot3;
     const int *str3;
+    // flat index, in initDescriptor's parameter layout (per block rows x cols,
+    // row-major), of each row's first parameter
+    const int *pidx2;
+    const int *pidx3;
+    int n_params;  // sum of rows x cols over the blocks
+    int pad2_;
 };
 
 }  // namespace ldsf
@@ -62,7 +68,8 @@ struct ldsf_sf {
     int has_three_body = 0;
     int precision = 0;
     // flattened row tables (host copies of SFTab's arrays)
-    std::vector<int> k2, slot2, str2, k3, izeta3, slot3, str3;
+    std::vector<int> k2, slot2, str2, k3, izeta3, slot3, str3, pidx2, pidx3;
+    int n_params = 0;
     std::vector<double> a2, b2, zeta3, lam3, eta3, coef3;
     // per-device uploads
     struct Dev {

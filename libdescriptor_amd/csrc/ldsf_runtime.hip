@@ -91,7 +91,7 @@ int device_table(const ldsf_sf *csf, SFTab *out)
     const int ns = (int)sf->species.size();
     const size_t n2 = sf->k2.size(), n3 = sf->k3.size();
     const size_t nd = ns * ns + 2 * n2 + 4 * n3;
-    const size_t ni = 3 * n2 + 4 * n3;
+    const size_t ni = 4 * n2 + 5 * n3;
     std::vector<double> hd;
     hd.reserve(nd);
     std::vector<int> hi;
@@ -101,7 +101,8 @@ int device_table(const ldsf_sf *csf, SFTab *out)
     size_t o_rc = put_d(sf->rcut), o_a2 = put_d(sf->a2), o_b2 = put_d(sf->b2), o_z3 = put_d(sf->zeta3),
            o_l3 = put_d(sf->lam3), o_e3 = put_d(sf->eta3), o_c3 = put_d(sf->coef3);
     size_t o_k2 = put_i(sf->k2), o_s2 = put_i(sf->slot2), o_t2 = put_i(sf->str2), o_k3 = put_i(sf->k3),
-           o_iz = put_i(sf->izeta3), o_s3 = put_i(sf->slot3), o_t3 = put_i(sf->str3);
+           o_iz = put_i(sf->izeta3), o_s3 = put_i(sf->slot3), o_t3 = put_i(sf->str3),
+           o_p2 = put_i(sf->pidx2), o_p3 = put_i(sf->pidx3);
     size_t bytes = sizeof(double) * (hd.size() + 1) + sizeof(int) * (hi.size() + 2);
     void *mem = nullptr;
     HIP_TRY(hipMalloc(&mem, bytes), "hipMalloc(parameter table)");
@@ -134,6 +135,9 @@ int device_table(const ldsf_sf *csf, SFTab *out)
     t.izeta3 = di + o_iz;
     t.slot3 = di + o_s3;
     t.str3 = di + o_t3;
+    t.pidx2 = di + o_p2;
+    t.pidx3 = di + o_p3;
+    t.n_params = sf->n_params;
     ldsf_sf::Dev d;
     d.device = device;
     d.mem = mem;
@@ -595,6 +599,64 @@ LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_co
         return ldsf::fail(LDSF_ERR_ARG, "plan_forward_adjoint_edges: unknown precision");
     return plan_forward_adjoint_edges(plan, d_coordinates, d_dE_dzeta, d_desc, precision,
                                       (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_plan_param_gradient(ldsf_plan *plan, const double *d_coordinates,
+                                      const double *d_dE_dzeta, double *d_dparams,
+                                      double *d_drcut, void *stream)
+{
+    if (!plan || !d_dparams || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_param_gradient: bad arguments");
+    ldsf::SFTab tab;
+    if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
+    if (tab.n_species > 8)
+        return ldsf::fail(LDSF_ERR_UNSUPPORTED, "plan_param_gradient: more than 8 species");
+    HIP_TRY(ldsf::launch_param_gradient(tab, plan->n_centers, plan->centers, plan->offs, plan->nl,
+                                        plan->species, d_coordinates, d_dE_dzeta, d_dparams,
+                                        d_drcut, (hipStream_t)stream),
+            "parameter gradient");
+    return LDSF_OK;
+}
+
+LDSF_API int ldsf_param_gradient(ldsf_sf *sf, int n_atoms, const int *species,
+                                 const int *neighbor_list, const int *number_of_neighs,
+                                 const double *coordinates, const double *dE_dzeta,
+                                 double *dparams, double *drcut)
+{
+    if (!sf || !dparams || (n_atoms > 0 && !dE_dzeta))
+        return ldsf::fail(LDSF_ERR_ARG, "param_gradient: bad arguments");
+    HostCall c{sf, n_atoms, {}, species, neighbor_list, number_of_neighs, coordinates};
+    if (int rc = host_prepare(c)) return rc;
+    if (n_atoms == 0) return LDSF_OK;
+    const int W = sf->width, ns = (int)sf->species.size();
+    DevBuf<int> d_species, d_nl, d_nn;
+    DevBuf<double> d_x, d_dE, d_dp, d_drc;
+    HIP_TRY(d_species.alloc(c.n_rows), "hipMalloc");
+    HIP_TRY(d_x.alloc(3 * (size_t)c.n_rows), "hipMalloc");
+    HIP_TRY(d_nl.alloc(c.n_edges), "hipMalloc");
+    HIP_TRY(d_nn.alloc(n_atoms), "hipMalloc");
+    HIP_TRY(d_dE.alloc((size_t)n_atoms * W), "hipMalloc");
+    HIP_TRY(d_dp.alloc(std::max(sf->n_params, 1)), "hipMalloc");
+    HIP_TRY(d_drc.alloc(ns * ns), "hipMalloc");
+    HIP_TRY(hipMemcpy(d_species.p, species, sizeof(int) * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_x.p, coordinates, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+    if (c.n_edges)
+        HIP_TRY(hipMemcpy(d_nl.p, neighbor_list, sizeof(int) * c.n_edges, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_nn.p, number_of_neighs, sizeof(int) * n_atoms, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMemcpy(d_dE.p, dE_dzeta, sizeof(double) * n_atoms * W, hipMemcpyHostToDevice), "hipMemcpy");
+    if (sf->n_params)
+        HIP_TRY(hipMemcpy(d_dp.p, dparams, sizeof(double) * sf->n_params, hipMemcpyHostToDevice), "hipMemcpy");
+    if (drcut) HIP_TRY(hipMemcpy(d_drc.p, drcut, sizeof(double) * ns * ns, hipMemcpyHostToDevice), "hipMemcpy");
+    ldsf_plan *plan = nullptr;
+    int rc = plan_build(&plan, sf, n_atoms, nullptr, c.n_rows, d_species.p, d_nl.p, d_nn.p, nullptr);
+    if (rc) return rc;
+    std::unique_ptr<ldsf_plan> guard(plan);
+    if ((rc = ldsf_plan_param_gradient(plan, d_x.p, d_dE.p, d_dp.p, drcut ? d_drc.p : nullptr, nullptr)))
+        return rc;
+    if (sf->n_params)
+        HIP_TRY(hipMemcpy(dparams, d_dp.p, sizeof(double) * sf->n_params, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (drcut) HIP_TRY(hipMemcpy(drcut, d_drc.p, sizeof(double) * ns * ns, hipMemcpyDeviceToHost), "hipMemcpy");
+    return LDSF_OK;
 }
 
 LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream)

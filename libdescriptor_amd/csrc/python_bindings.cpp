@@ -312,6 +312,30 @@ PYBIND11_MODULE(libdescriptor, m)
         "Compute gradient of descriptor for complete configuration.");
 
     m.def(
+        "param_gradient",
+        [](DescriptorKind &ds, int n_atoms, IArr &species, IArr &neighbors, IArr &number_of_neighbors,
+           DArr &coordinates, DArr &dE_ddesc) {
+            auto *sf = sf_of(ds);
+            if (!sf) throw std::invalid_argument("param_gradient: symmetry functions only");
+            check_layout(species, neighbors, sum_counts(number_of_neighbors, n_atoms), coordinates,
+                         n_atoms, nullptr);
+            check_width(dE_ddesc, n_atoms, ds.width, "dE_ddesc");
+            const int ns = ldsf_sf_num_species(sf->handle());
+            auto dpar = zeros({(py::ssize_t)ldsf_sf_num_params(sf->handle())});
+            auto drc = zeros({(py::ssize_t)ns, (py::ssize_t)ns});
+            int rc = 0;
+            run(ds, [&] {
+                rc = ldsf_param_gradient(sf->handle(), n_atoms, ip(species), ip(neighbors),
+                                         ip(number_of_neighbors), dp(coordinates), dp(dE_ddesc),
+                                         dpar.mutable_data(), drc.mutable_data());
+            });
+            if (rc) throw std::runtime_error(ldsf_last_error());
+            return py::make_tuple(dpar, drc);
+        },
+        "dE/d(parameters) (initDescriptor's flat layout) and dE/d(cutoff matrix): what the "
+        "reference's Enzyme gradient computes into the descriptor's shadow and discards.");
+
+    m.def(
         "gradient_batch",
         [](DescriptorKind &ds, IArr &n_atoms, IArr &config_ptr, IArr &species, IArr &neighbors,
            IArr &number_of_neighbors, DArr &coordinates, DArr &desc, DArr &dE_ddesc) {

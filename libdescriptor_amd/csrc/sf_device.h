@@ -23,6 +23,7 @@ namespace ldsf {
 
 // same literal as src/SymmetryFunctions.cpp:18
 constexpr double kPi = 3.1415926535897932;
+constexpr double kLn2 = 0.69314718055994531;  // ln 2: d 2^(1 - zeta) / d zeta = -ln 2 2^(1 - zeta)
 
 template <int P>
 struct Real;

@@ -10,6 +10,8 @@
 // kernels in sf_wave.hip take over where their preconditions hold.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sf_device.h"
 #include "sf_kernels.h"
 
@@ -267,6 +269,172 @@ __global__ __launch_bounds__(256) void k_adj_general(SFTab tab, int n_centers,
     vs[2] = s2;
 }
 
+
+// ---- hyperparameter gradients (FP64) ----
+//
+// The reference's gradient duplicates the descriptor object itself for
+// Enzyme (src/Descriptors.cpp:176-188, enzyme_dup desc_kind, d_desc_kind): the
+// shadow receives dE/d(parameters) and dE/d(cutoff matrix), which the
+// reference then deletes unread.  These kernels compute them: thread (row r,
+// centre t) walks the centre's neighbours and triplets in the reference's
+// order for its one parameter row and writes the row's parameter derivatives
+// and its cutoff-matrix derivatives into column-major partials
+// part[col][centre]; the columns are then summed over the centres in a fixed
+// order (a block per (column, 4096 centres), each thread 16 consecutive
+// centres then an LDS tree; the blocks in order), so the result is the same on
+// every run.
+constexpr int kParamMaxSpecies = 8;
+
+// d cut_cos(r; rc) / d rc for r < rc
+__device__ __forceinline__ double dcut_drc(double r, double rc)
+{
+    return 0.5 * sin(ref_arg(r, rc)) * kPi * r / (rc * rc);
+}
+
+__global__ __launch_bounds__(256) void k_param_rows(SFTab tab, int n_rows, int c0, int nc,
+                                                    const int *__restrict__ centers,
+                                                    const int *__restrict__ offs,
+                                                    const int *__restrict__ nl,
+                                                    const int *__restrict__ species,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ dE,
+                                                    double *__restrict__ part)
+{
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)n_rows * nc) return;
+    const int r = (int)(id / nc), tl = (int)(id - (long)r * nc), t = c0 + tl;
+    const int i = centers ? centers[t] : t;
+    const int ns = tab.n_species;
+    const double *g = dE + (size_t)t * tab.width;
+    double dp[3] = {0.0, 0.0, 0.0};
+    double drc[kParamMaxSpecies * kParamMaxSpecies];
+    for (int k = 0; k < ns * ns; ++k) drc[k] = 0.0;
+    const int beg = offs[t], end = offs[t + 1];
+    const int si = species[i];
+    const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+    int kd, ncols;
+    if (r < tab.n2) {
+        const int q = r;
+        kd = tab.k2[q];
+        ncols = kd == G2 ? 2 : (kd == G3 ? 1 : 0);
+        const double a = tab.a2[q], b = tab.b2[q];
+        for (int jj = beg; jj < end; ++jj) {
+            const int j = nl[jj];
+            const int sj = species[j];
+            const double rcij = tab.rc[si * ns + sj];
+            const double rij = sqrt(dist2(x[3 * j] - xi0, x[3 * j + 1] - xi1, x[3 * j + 2] - xi2));
+            if (!(rij < rcij)) continue;  // r >= rc: the term and its derivatives vanish
+            const double w = g[tab.slot2[q] + sj * tab.str2[q]];
+            const double fc = fcut_value(rij, rcij, Real<0>());
+            double val = 1.0;  // the term without its cutoff factor
+            if (kd == G2) {
+                const double dr = rij - b;
+                val = exp(-a * dr * dr);
+                dp[0] += w * (-dr * dr) * val * fc;
+                dp[1] += w * (2.0 * a * dr) * val * fc;
+            } else if (kd == G3) {
+                val = cos(a * rij);
+                dp[0] += w * (-rij * sin(a * rij)) * fc;
+            }
+            drc[si * ns + sj] += w * val * dcut_drc(rij, rcij);
+        }
+    } else {
+        const int q = r - tab.n2;
+        kd = tab.k3[q];
+        ncols = 3;
+        const double zeta = tab.zeta3[q], lam = tab.lam3[q], eta = tab.eta3[q];
+        const double coef = tab.coef3[q];
+        const int iz = tab.izeta3[q];
+        for (int jj = beg; jj < end; ++jj) {
+            const int j = nl[jj];
+            const int sj = species[j];
+            const double rcij = tab.rc[si * ns + sj];
+            const double rij = sqrt(dist2(x[3 * j] - xi0, x[3 * j + 1] - xi1, x[3 * j + 2] - xi2));
+            if (!(rij < rcij)) continue;
+            const double fij = fcut_value(rij, rcij, Real<0>()), dfij = dcut_drc(rij, rcij);
+            for (int kk = jj + 1; kk < end; ++kk) {
+                const int k = nl[kk];
+                const int sk = species[k];
+                const double rcik = tab.rc[si * ns + sk];
+                const double rcjk = tab.rc[sj * ns + sk];
+                const double rik = sqrt(dist2(x[3 * k] - xi0, x[3 * k + 1] - xi1, x[3 * k + 2] - xi2));
+                const double rjk = sqrt(dist2(x[3 * k] - x[3 * j], x[3 * k + 1] - x[3 * j + 1],
+                                              x[3 * k + 2] - x[3 * j + 2]));
+                if (!(rik < rcik)) continue;
+                const bool jk_in = rjk < rcjk;
+                if (kd == G4 && !jk_in) continue;
+                const RefTrip rt = ref_trip(rij, rik, rjk);
+                const double base = ref_base(lam, rt.cosv);
+                if (base <= 0.0) continue;
+                const double w = g[tab.slot3[q] + pair_index(ns, sj, sk) * tab.str3[q]];
+                const double fik = fcut_value(rik, rcik, Real<0>()), dfik = dcut_drc(rik, rcik);
+                const double fjk = kd == G4 ? fcut_value(rjk, rcjk, Real<0>()) : 1.0;
+                const double dfjk = kd == G4 ? dcut_drc(rjk, rcjk) : 0.0;
+                const double cost = dpow(base, zeta, iz, Real<0>());
+                const double S = kd == G4 ? rt.s2 : rt.rijsq + rt.riksq;
+                const double e = exp(-eta * S);
+                const double F = fij * fik * fjk;
+                const double term = coef * cost * e * F;
+                dp[0] += w * term * (log(base) - kLn2);
+                dp[1] += w * coef * zeta * dpowm1(base, zeta, iz, Real<0>()) * rt.cosv * e * F;
+                dp[2] += w * (-S) * term;
+                const double A = w * coef * cost * e;
+                drc[si * ns + sj] += A * dfij * fik * fjk;
+                drc[si * ns + sk] += A * fij * dfik * fjk;
+                if (kd == G4) drc[sj * ns + sk] += A * fij * fik * dfjk;
+            }
+        }
+    }
+    const int pc = r < tab.n2 ? tab.pidx2[r] : tab.pidx3[r - tab.n2];
+    for (int c = 0; c < ncols; ++c) part[(size_t)(pc + c) * nc + tl] = dp[c];
+    const int rcbase = tab.n_params + r * ns * ns;
+    for (int k = 0; k < ns * ns; ++k) part[(size_t)(rcbase + k) * nc + tl] = drc[k];
+}
+
+constexpr int kColChunk = 4096;  // centres per column-sum block: 256 threads x 16
+
+__global__ __launch_bounds__(256) void k_colsum(int nc, const double *__restrict__ part,
+                                                double *__restrict__ part2, int nb)
+{
+    __shared__ double sh[256];
+    const int c = blockIdx.x, b = blockIdx.y;
+    const size_t base = (size_t)c * nc;
+    const int lo = b * kColChunk + threadIdx.x * 16;
+    double s = 0.0;
+    for (int k = 0; k < 16; ++k)
+        if (lo + k < nc) s += part[base + lo + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part2[(size_t)c * nb + b] = sh[0];
+}
+
+__global__ void k_colsum_final(int ncol, int nb, const double *__restrict__ part2,
+                               double *__restrict__ acc)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part2[(size_t)c * nb + b];
+    acc[c] += s;
+}
+
+// dparams += the parameter columns; drcut[k] += the rows' cutoff columns, rows ascending
+__global__ void k_param_finish(int n_params, int n_rows, int ns2, const double *__restrict__ acc,
+                               double *__restrict__ dparams, double *__restrict__ drcut)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_params) dparams[c] += acc[c];
+    if (c < ns2 && drcut) {
+        double s = 0.0;
+        for (int r = 0; r < n_rows; ++r) s += acc[n_params + r * ns2 + c];
+        drcut[c] += s;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_fwd_general(const SFTab &tab, int precision, int n_centers, const int *centers,
@@ -298,6 +466,48 @@ hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, co
         hipLaunchKernelGGL(k_adj_general<1>, grid, block, 0, s, tab, n_centers, centers, offs, nl,
                            species, x, dE, inv_pos, edge, n_edges, dc);
     return hipGetLastError();
+}
+
+hipError_t launch_param_gradient(const SFTab &tab, int n_centers, const int *centers,
+                                 const int *offs, const int *nl, const int *species,
+                                 const double *x, const double *dE, double *dparams,
+                                 double *drcut, hipStream_t s)
+{
+    const int n_rows = tab.n2 + tab.n3, ns2 = tab.n_species * tab.n_species;
+    if (n_centers <= 0 || n_rows == 0) return hipSuccess;
+    if (tab.n_species > kParamMaxSpecies) return hipErrorInvalidValue;
+    const int ncol = tab.n_params + n_rows * ns2;
+    // centres per chunk: partials of at most 32 M doubles (256 MB)
+    const int chunk = (int)std::max<long>(1, std::min<long>(n_centers, (32L << 20) / ncol));
+    const int nbmax = (chunk + kColChunk - 1) / kColChunk;
+    double *part = nullptr, *part2 = nullptr, *acc = nullptr;
+    hipError_t e = hipMalloc((void **)&part, sizeof(double) * (size_t)ncol * chunk);
+    if (e == hipSuccess) e = hipMalloc((void **)&part2, sizeof(double) * (size_t)ncol * nbmax);
+    if (e == hipSuccess) e = hipMalloc((void **)&acc, sizeof(double) * ncol);
+    if (e == hipSuccess) e = hipMemsetAsync(acc, 0, sizeof(double) * ncol, s);
+    for (int c0 = 0; e == hipSuccess && c0 < n_centers; c0 += chunk) {
+        const int nc = std::min(chunk, n_centers - c0);
+        const int nb = (nc + kColChunk - 1) / kColChunk;
+        e = hipMemsetAsync(part, 0, sizeof(double) * (size_t)ncol * nc, s);
+        if (e != hipSuccess) break;
+        const long threads = (long)n_rows * nc;
+        hipLaunchKernelGGL(k_param_rows, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                           tab, n_rows, c0, nc, centers, offs, nl, species, x, dE, part);
+        hipLaunchKernelGGL(k_colsum, dim3(ncol, nb), dim3(256), 0, s, nc, part, part2, nb);
+        hipLaunchKernelGGL(k_colsum_final, dim3((ncol + 255) / 256), dim3(256), 0, s, ncol, nb,
+                           part2, acc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        const int m = std::max(tab.n_params, ns2);
+        hipLaunchKernelGGL(k_param_finish, dim3((m + 255) / 256), dim3(256), 0, s, tab.n_params,
+                           n_rows, ns2, acc, dparams, drcut);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (double *p : {part, part2, acc})
+        if (p) (void)hipFree(p);
+    return e;
 }
 
 }  // namespace ldsf

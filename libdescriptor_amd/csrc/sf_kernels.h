@@ -35,6 +35,15 @@ hipError_t launch_adj_general(const SFTab &tab, int precision, int n_centers, co
                               const double *dE, const int *inv_pos, double *edge, int n_edges,
                               hipStream_t s, DevCounts dc = DevCounts());
 
+// Hyperparameter gradients (FP64, general kernels): dparams[n_params] +=
+// dE/d(parameters) in initDescriptor's flat layout, drcut[ns*ns] += dE/d(rcut
+// matrix) (NULL: not returned), for the centres' dE/dzeta rows.  Allocates its
+// partials and synchronises `s`.  At most 8 species.
+hipError_t launch_param_gradient(const SFTab &tab, int n_centers, const int *centers,
+                                 const int *offs, const int *nl, const int *species,
+                                 const double *x, const double *dE, double *dparams,
+                                 double *drcut, hipStream_t s);
+
 // sf_wave.hip: one wavefront per centre, neighbours staged in LDS.  They
 // cover parameter sets whose two-body rows are g2 (at most 8) and whose
 // three-body rows are g4 forming a grid zeta x lambda x eta (at most 4 x 2 x 2

@@ -124,8 +124,9 @@ int finalize_params(ldsf_sf *sf)
     sf->k2.clear(); sf->slot2.clear(); sf->str2.clear(); sf->a2.clear(); sf->b2.clear();
     sf->k3.clear(); sf->slot3.clear(); sf->str3.clear(); sf->zeta3.clear(); sf->lam3.clear();
     sf->eta3.clear(); sf->coef3.clear(); sf->izeta3.clear();
+    sf->pidx2.clear(); sf->pidx3.clear();
     sf->has_three_body = 0;
-    int w = 0;
+    int w = 0, np = 0;
     for (size_t p = 0; p < sf->kind.size(); ++p) {
         int k = sf->kind[p], R = sf->rows[p], C = sf->cols[p];
         if (R < 0 || C < 0) return fail(LDSF_ERR_ARG, "negative parameter block size");
@@ -140,6 +141,7 @@ int finalize_params(ldsf_sf *sf)
                 sf->b2.push_back(C > 1 ? P[q * C + 1] : 0.0);
                 sf->slot2.push_back(w + q);
                 sf->str2.push_back(sf->species_resolved ? R : 0);
+                sf->pidx2.push_back(np + q * C);
             }
             w += R * (sf->species_resolved ? ns : 1);
         } else if (three_body(k)) {
@@ -157,13 +159,16 @@ int finalize_params(ldsf_sf *sf)
                 sf->izeta3.push_back(iz);
                 sf->slot3.push_back(w + q);
                 sf->str3.push_back(sf->species_resolved ? R : 0);
+                sf->pidx3.push_back(np + q * C);
             }
             w += R * (sf->species_resolved ? npairs : 1);
         } else {
             return fail(LDSF_ERR_ARG, "unknown symmetry function type");
         }
+        np += R * C;
     }
     sf->width = w;
+    sf->n_params = np;
     return LDSF_OK;
 }
 
@@ -319,6 +324,8 @@ LDSF_API int ldsf_sf_create_from_file(ldsf_sf **out, const char *path)
 }
 
 LDSF_API int ldsf_sf_width(const ldsf_sf *sf) { return sf ? sf->width : -1; }
+
+LDSF_API int ldsf_sf_num_params(const ldsf_sf *sf) { return sf ? sf->n_params : -1; }
 
 LDSF_API int ldsf_sf_num_species(const ldsf_sf *sf) { return sf ? (int)sf->species.size() : 0; }
 

@@ -51,6 +51,10 @@ def lib():
                                             ctypes.c_int]
         L.orc_num_gradient_single_atom.argtypes = [vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
                                                    ctypes.c_int, _f64p, _f64p, _f64p]
+        L.orc_num_params.argtypes = [vp]
+        L.orc_num_params.restype = ctypes.c_int
+        L.orc_param_gradient.argtypes = [vp, ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p,
+                                         _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -115,6 +119,17 @@ class OracleSF:
         lib().orc_gradient(self._h, n_atoms, _i32(species), _i32(neighlist), _i32(numneigh), c,
                            _f64(dE_dzeta), out)
         return out
+
+    # the shadow of the descriptor object in Descriptor::gradient (src/Descriptors.cpp:176-188):
+    # dE/d(parameters) in initDescriptor's flat layout and dE/d(rcut matrix)
+    def param_gradient(self, n_atoms, species, neighlist, numneigh, coords, dE_dzeta):
+        L = lib()
+        dp = np.zeros(L.orc_num_params(self._h))
+        ns = self.n_species
+        drc = np.zeros(ns * ns)
+        L.orc_param_gradient(self._h, n_atoms, _i32(species), _i32(neighlist), _i32(numneigh),
+                             _f64(coords), _f64(dE_dzeta), dp, drc)
+        return dp, drc.reshape(ns, ns)
 
     # Descriptor::gradient_single_atom (src/Descriptors.cpp:368-395)
     def gradient_single_atom(self, index, species, neighbors, coords, dE_dzeta):

@@ -603,3 +603,134 @@ void orc_num_gradient_single_atom(const orc_sf *sf, int index, int n_total, cons
     }
     free(x); free(y); free(dy);
 }
+
+/* ---- hyperparameter gradients ----
+ *
+ * The reference's gradient calls Enzyme with the descriptor object itself
+ * duplicated (src/Descriptors.cpp:176-188, enzyme_dup desc_kind, d_desc_kind):
+ * the shadow receives dE/d(parameters) and dE/d(cutoff matrix) and is then
+ * deleted unread.  This restates what that shadow holds, term by term in the
+ * forward's loop order (src/SymmetryFunctions.cpp:141-259):
+ *   dparams[n_params]  flat in the parameter layout of initDescriptor (per
+ *                      block, rows x cols row-major): g2 (eta, Rs), g3 (kappa),
+ *                      g4/g5 (zeta, lambda, eta); g1 has none
+ *   drcut[ns * ns]     rcut_2D_ entries, row-major (helper.hpp:2056-2067)
+ * both +=, for the centres 0..n_atoms-1 weighted by g = dE/dzeta.
+ */
+
+/* d cut_cos(r; rcut) / d rcut */
+static double dcut_drc(double r, double rcut)
+{
+    return (r < rcut) ? 0.5 * sin(ORC_PI * r / rcut) * ORC_PI * r / (rcut * rcut) : 0.0;
+}
+
+static void param_gradient_atom(const orc_sf *sf, int index, const int *species,
+                                const int *neigh_list, int number_of_neigh, const double *coords,
+                                const double *g, double *dparams, double *drcut)
+{
+    const int ns = sf->n_species;
+    const int iS = species[index];
+    const double *xi = coords + 3 * index;
+    int *poff = (int *)malloc(sizeof(int) * (sf->n_blocks + 1));
+    poff[0] = 0;
+    for (int p = 0; p < sf->n_blocks; ++p) poff[p + 1] = poff[p] + sf->rows[p] * sf->cols[p];
+    for (int jj = 0; jj < number_of_neigh; ++jj) {
+        int j = neigh_list[jj];
+        int jS = species[j];
+        double rcutij = sf->rcut[iS * ns + jS];
+        double rij[3];
+        for (int d = 0; d < 3; ++d) rij[d] = coords[3 * j + d] - xi[d];
+        double rijmag = sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
+        if (rijmag > rcutij) continue;
+        double fij = cut_cos(rijmag, rcutij), dfij = dcut_drc(rijmag, rcutij);
+        for (int p = 0; p < sf->n_blocks; ++p) {
+            int kd = sf->kind[p];
+            if (!is_two_body(kd)) continue;
+            int idx = slot2(sf, p, jS);
+            for (int q = 0; q < sf->rows[p]; ++q, ++idx) {
+                const double *prm = sf->par[p] + q * sf->cols[p];
+                double *dp = dparams + poff[p] + q * sf->cols[p];
+                double w = g[idx];
+                double val = 1.0; /* the term without its cutoff factor */
+                if (kd == 2) {
+                    double eta = prm[0], Rs = prm[1], dr = rijmag - Rs;
+                    val = exp(-eta * dr * dr);
+                    dp[0] += w * (-dr * dr) * val * fij;
+                    dp[1] += w * (2.0 * eta * dr) * val * fij;
+                } else if (kd == 3) {
+                    double kappa = prm[0];
+                    val = cos(kappa * rijmag);
+                    dp[0] += w * (-rijmag * sin(kappa * rijmag)) * fij;
+                }
+                drcut[iS * ns + jS] += w * val * dfij;
+            }
+        }
+        if (!sf->has_three_body) continue;
+        for (int kk = jj + 1; kk < number_of_neigh; ++kk) {
+            int k = neigh_list[kk];
+            int kS = species[k];
+            double rcutik = sf->rcut[iS * ns + kS];
+            double rcutjk = sf->rcut[jS * ns + kS];
+            double rik[3], rjk[3];
+            for (int d = 0; d < 3; ++d) {
+                rik[d] = coords[3 * k + d] - xi[d];
+                rjk[d] = coords[3 * k + d] - coords[3 * j + d];
+            }
+            double rikmag = sqrt(rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2]);
+            double rjkmag = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+            if (rikmag > rcutik) continue;
+            double rijsq = rijmag * rijmag, riksq = rikmag * rikmag, rjksq = rjkmag * rjkmag;
+            double cos_ijk = (rijsq + riksq - rjksq) / (2 * rijmag * rikmag);
+            double fik = cut_cos(rikmag, rcutik), dfik = dcut_drc(rikmag, rcutik);
+            double fjk = cut_cos(rjkmag, rcutjk), dfjk = dcut_drc(rjkmag, rcutjk);
+            for (int p = 0; p < sf->n_blocks; ++p) {
+                int kd = sf->kind[p];
+                if (!is_three_body(kd)) continue;
+                if (kd == 4 && rjkmag > rcutjk) continue;
+                int idx = slot3(sf, p, jS, kS);
+                for (int q = 0; q < sf->rows[p]; ++q, ++idx) {
+                    const double *prm = sf->par[p] + q * sf->cols[p];
+                    double *dp = dparams + poff[p] + q * sf->cols[p];
+                    double zeta = prm[0], lambda = prm[1], eta = prm[2];
+                    double base = 1 + lambda * cos_ijk;
+                    if (base <= 0) continue;
+                    double w = g[idx];
+                    double p2 = pow(2, 1 - zeta);
+                    double costerm = pow(base, zeta);
+                    double S = kd == 4 ? rijsq + riksq + rjksq : rijsq + riksq;
+                    double e = exp(-eta * S);
+                    double F = fij * fik * (kd == 4 ? fjk : 1.0);
+                    double term = p2 * costerm * e * F;
+                    dp[0] += w * term * (log(base) - log(2.0));
+                    dp[1] += w * p2 * zeta * pow(base, zeta - 1) * cos_ijk * e * F;
+                    dp[2] += w * (-S) * term;
+                    double a = w * p2 * costerm * e;
+                    drcut[iS * ns + jS] += a * dfij * fik * (kd == 4 ? fjk : 1.0);
+                    drcut[iS * ns + kS] += a * fij * dfik * (kd == 4 ? fjk : 1.0);
+                    if (kd == 4) drcut[jS * ns + kS] += a * fij * fik * dfjk;
+                }
+            }
+        }
+    }
+    free(poff);
+}
+
+int orc_num_params(const orc_sf *sf)
+{
+    int n = 0;
+    for (int p = 0; p < sf->n_blocks; ++p) n += sf->rows[p] * sf->cols[p];
+    return n;
+}
+
+void orc_param_gradient(const orc_sf *sf, int n_atoms, const int *species,
+                        const int *neighbor_list, const int *number_of_neighs,
+                        const double *coords, const double *dE_dzeta, double *dparams,
+                        double *drcut)
+{
+    const int *nl = neighbor_list;
+    for (int i = 0; i < n_atoms; ++i) {
+        param_gradient_atom(sf, i, species, nl, number_of_neighs[i], coords,
+                            dE_dzeta + (size_t)i * sf->width, dparams, drcut);
+        nl += number_of_neighs[i];
+    }
+}

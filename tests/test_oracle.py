@@ -312,3 +312,49 @@ def test_batch_omp_equals_serial_batch_walk():
     d1, G1 = O.fwd_adj_batch_omp(*args, g, 4)
     np.testing.assert_array_equal(d1, d0)
     assert np.abs(G1 - G0).max() <= 1e-13 * np.abs(G0).max()
+
+
+# ---------------- hyperparameter gradients ----------------
+
+@pytest.mark.parametrize("case", ["one_species", "two_species_resolved"])
+def test_param_gradient_matches_central_fd(case):
+    """dE/d(parameters) and dE/d(cutoff matrix) (what the reference's Enzyme shadow of the
+    descriptor object holds) against central differences of L = sum(g * desc) over every
+    parameter and every cutoff entry, each moved by +-1e-6."""
+    pos = cluster(13, 7)
+    types, sizes, params = ALL_KINDS
+    if case == "one_species":
+        species, rc, resolved = ["Si"], np.array([[4.0]]), False
+        sp = np.zeros(len(pos), np.int32)
+    else:
+        species, rc, resolved = ["A", "B"], np.array([[4.0, 4.2], [3.9, 3.8]]), True
+        sp = (np.arange(len(pos)) % 3 == 0).astype(np.int32)
+    nn, nl = full_list(pos, 4.2)
+    n = len(pos)
+    O = orc.OracleSF(species, rc, types, sizes, params, species_resolved=resolved)
+    g = np.random.default_rng(3).normal(size=(n, O.width))
+    dp, drc = O.param_gradient(n, sp, nl, nn, pos, g)
+
+    def loss(prm, rcm):
+        return float((orc.OracleSF(species, rcm, types, sizes, prm, species_resolved=resolved)
+                      .compute(n, sp, nl, nn, pos) * g).sum())
+
+    h = 1e-6
+    fd = np.zeros(len(params))
+    for k in range(len(params)):
+        a, b = np.array(params, float), np.array(params, float)
+        a[k] += h
+        b[k] -= h
+        fd[k] = (loss(a, rc) - loss(b, rc)) / (2 * h)
+    fdr = np.zeros_like(rc)
+    for a_ in range(rc.shape[0]):
+        for b_ in range(rc.shape[1]):
+            p, m = rc.copy(), rc.copy()
+            p[a_, b_] += h
+            m[a_, b_] -= h
+            fdr[a_, b_] = (loss(params, p) - loss(params, m)) / (2 * h)
+    assert np.abs(dp).max() > 0 and np.abs(drc).max() > 0
+    assert rel_err(dp, fd) <= 1e-7
+    assert rel_err(drc, fdr) <= 1e-7
+    # every parameter column that the kinds use has a non-trivial derivative
+    assert np.count_nonzero(np.abs(fd) > 1e-9) == np.count_nonzero(np.abs(dp) > 1e-9)
