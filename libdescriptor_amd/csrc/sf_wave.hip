@@ -543,6 +543,44 @@ struct Fast {
     double rq0, rq1, rq2;
 };
 
+// The zeta grid of the FP32-bulk kernels: ZG = 0 any grid (B^zeta =
+// exp2(zeta log2 B)); ZG = 1 and 2 the power-of-two grids (1, 2, 4, 16) and
+// (1, 2, 4, 8), in that order, whose powers come from a squaring chain with
+// the exponents known at compile time (no transcendental per power).
+template <int ZG>
+struct ZGrid {
+    static constexpr int e[4] = {0, 0, 0, 0};
+    static constexpr int nsq = 0;
+};
+template <>
+struct ZGrid<1> {
+    static constexpr int e[4] = {0, 1, 2, 4};
+    static constexpr int nsq = 4;
+};
+template <>
+struct ZGrid<2> {
+    static constexpr int e[4] = {0, 1, 2, 3};
+    static constexpr int nsq = 3;
+};
+
+// B^(2^k), k = 0..nsq
+template <int ZG>
+__device__ __forceinline__ void squares(float b, float (&sq)[5])
+{
+    sq[0] = b;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) sq[k] = k <= ZGrid<ZG>::nsq ? sq[k - 1] * sq[k - 1] : 0.0f;
+}
+
+// B^(2^k - 1), k = 0..nsq: 1, B, B^3, B^7, B^15
+template <int ZG>
+__device__ __forceinline__ void squares_m1(const float (&sq)[5], float (&mk)[5])
+{
+    mk[0] = 1.0f;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) mk[k] = k <= ZGrid<ZG>::nsq ? mk[k - 1] * sq[k - 1] : 0.0f;
+}
+
 // the value of block p (0, 1 or 2)
 template <typename V>
 __device__ __forceinline__ V blk3(int p, V v0, V v1, V v2)
@@ -808,7 +846,7 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 // weights dE/dzeta 2^(1-zeta) and those times zeta lambda.  FWD: the triplet's
 // descriptor terms B^zeta exp(..) fc fc fc (without 2^(1-zeta)) are also added
 // into the lane's row accumulators `acc`, from the same powers.
-template <int NPB, bool FWD>
+template <int NPB, bool FWD, int ZG>
 __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, const Grp &L, int ns,
                                               float rc, double rc2, float rcs, int a, int b,
                                               const float *w, const float *wv, double (&acc)[16])
@@ -846,11 +884,20 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
-        const float lb = __builtin_amdgcn_logf(bl);
+        // power-of-two grids: B^(2^k) and B^(2^k - 1) by squaring; else
+        // B^(zeta - 1) = exp2((zeta - 1) log2 B)
+        float sq[5], mk[5], lb = 0.0f;
+        if constexpr (ZG != 0) {
+            squares<ZG>(bl, sq);
+            squares_m1<ZG>(sq, mk);
+        } else {
+            lb = __builtin_amdgcn_logf(bl);
+        }
 #pragma unroll
         for (int z = 0; z < 4; ++z) {
-            const float pm = __builtin_amdgcn_exp2f((F.zeta[z] - 1.0f) * lb);  // B^(zeta - 1)
-            const float pw = pm * bl;
+            const float pm = ZG != 0 ? mk[ZGrid<ZG>::e[z]]  // B^(zeta - 1)
+                                     : __builtin_amdgcn_exp2f((F.zeta[z] - 1.0f) * lb);
+            const float pw = ZG != 0 ? sq[ZGrid<ZG>::e[z]] : pm * bl;
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 U[e] = fmaf(w[(z * 2 + l) * 2 + e], pw, U[e]);
@@ -881,7 +928,8 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     atomicAdd(&L.fz[b], fma(a2c, b1.x, a3c * g.ez));
 }
 
-template <int P, int NSB, int NPB>
+// ZG: the zeta grid of the FP32-bulk form (ZGrid)
+template <int P, int NSB, int NPB, int ZG>
 __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
@@ -1057,10 +1105,14 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                             te[1] = (double)(va.z * vb.z * (fjk * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2)));
 #pragma unroll
                             for (int l = 0; l < 2; ++l) {
-                                const float lb = __builtin_amdgcn_logf(base_f(F, l, sp, sm, g.rjk2d, inv));
+                                const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
+                                float sq[5], lb = 0.0f;
+                                if constexpr (ZG != 0) squares<ZG>(bl, sq);
+                                else lb = __builtin_amdgcn_logf(bl);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
-                                    const double pw = (double)__builtin_amdgcn_exp2f(F.zeta[z] * lb);
+                                    const double pw = ZG != 0 ? (double)sq[ZGrid<ZG>::e[z]]
+                                                              : (double)__builtin_amdgcn_exp2f(F.zeta[z] * lb);
 #pragma unroll
                                     for (int e = 0; e < 2; ++e)
                                         acc[(z * 2 + l) * 2 + e] = fma(pw, te[e], acc[(z * 2 + l) * 2 + e]);
@@ -1170,7 +1222,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
 // reference's gradient adds the forward into desc as well,
 // src/Descriptors.cpp:165-190), its triplet lanes dealt the forward kernel's
 // 4-lane columns so that the row sums reduce the same way.
-template <int P, int NSB, int NPB, bool FWD>
+template <int P, int NSB, int NPB, bool FWD, int ZG>
 __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
@@ -1390,7 +1442,7 @@ __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(Wav
                     const int code = lst[t];
                     const int a = code & 63, b = (code >> 6) & 63;
                     if constexpr (P == 1) {
-                        adj_triplet_f<NPB, FWD>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
+                        adj_triplet_f<NPB, FWD, ZG>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
                     } else {
                         Trip<P> q;
                         triplet<P, true>(S, wp, wt, L, ns, a, b, q);
@@ -1603,6 +1655,9 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
     wt->nz = (int)zs.size();
     wt->nl = (int)ls.size();
     wt->ne = (int)es.size();
+    // the power-of-two grids with compile-time squaring chains (ZGrid)
+    if (zs.size() == 4 && zs[0] == 1.0 && zs[1] == 2.0 && zs[2] == 4.0)
+        wt->zgrid = zs[3] == 16.0 ? 1 : (zs[3] == 8.0 ? 2 : 0);
     for (int z = 0; z < wt->nz; ++z) {
         wp->zeta[z] = zs[z];
         wp->coef[z] = std::pow(2, 1 - zs[z]);
@@ -1641,14 +1696,18 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
     const int maxn = std::max(max_nn, 1);
     const size_t lds = lds_for(max_nn, kFwd, wt.n_species);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
-#define LDSF_FWD(P, NSB, NPB)                                                                  \
-    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB>), grid, block, lds, s, wt, n_centers, centers, \
-                       offs, nl, species, x, desc, width, maxn, dc)
-    if (wt.nsb == 1) {
-        if (precision == 0) LDSF_FWD(0, 1, 1); else LDSF_FWD(1, 1, 1);
-    } else {
-        if (precision == 0) LDSF_FWD(0, 2, 3); else LDSF_FWD(1, 2, 3);
-    }
+#define LDSF_FWD(P, NSB, NPB, ZG)                                                            \
+    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, ZG>), grid, block, lds, s, wt, n_centers,      \
+                       centers, offs, nl, species, x, desc, width, maxn, dc)
+#define LDSF_FWD_P(NSB, NPB)                                                        \
+    do {                                                                            \
+        if (precision == 0) LDSF_FWD(0, NSB, NPB, 0);                              \
+        else if (wt.zgrid == 1) LDSF_FWD(1, NSB, NPB, 1);                          \
+        else if (wt.zgrid == 2) LDSF_FWD(1, NSB, NPB, 2);                          \
+        else LDSF_FWD(1, NSB, NPB, 0);                                             \
+    } while (0)
+    if (wt.nsb == 1) LDSF_FWD_P(1, 1); else LDSF_FWD_P(2, 3);
+#undef LDSF_FWD_P
 #undef LDSF_FWD
     return hipGetLastError();
 }
@@ -1665,23 +1724,23 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     const bool fused = desc != nullptr || dc.out_row != nullptr;
     const size_t lds = lds_for(max_nn, fused ? kFused : kAdj, wt.n_species);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
-#define LDSF_ADJ(P, NSB, NPB, FWD)                                                              \
-    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD>), grid, block, lds, s, wt, n_centers,      \
+#define LDSF_ADJ(P, NSB, NPB, FWD, ZG)                                                         \
+    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD, ZG>), grid, block, lds, s, wt, n_centers,  \
                        centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, width, \
                        maxn, dc)
+#define LDSF_ADJ_P(NSB, NPB, FWD)                                \
+    do {                                                         \
+        if (precision == 0) LDSF_ADJ(0, NSB, NPB, FWD, 0);       \
+        else if (wt.zgrid == 1) LDSF_ADJ(1, NSB, NPB, FWD, 1);   \
+        else if (wt.zgrid == 2) LDSF_ADJ(1, NSB, NPB, FWD, 2);   \
+        else LDSF_ADJ(1, NSB, NPB, FWD, 0);                      \
+    } while (0)
     if (fused) {
-        if (wt.nsb == 1) {
-            if (precision == 0) LDSF_ADJ(0, 1, 1, true); else LDSF_ADJ(1, 1, 1, true);
-        } else {
-            if (precision == 0) LDSF_ADJ(0, 2, 3, true); else LDSF_ADJ(1, 2, 3, true);
-        }
+        if (wt.nsb == 1) LDSF_ADJ_P(1, 1, true); else LDSF_ADJ_P(2, 3, true);
     } else {
-        if (wt.nsb == 1) {
-            if (precision == 0) LDSF_ADJ(0, 1, 1, false); else LDSF_ADJ(1, 1, 1, false);
-        } else {
-            if (precision == 0) LDSF_ADJ(0, 2, 3, false); else LDSF_ADJ(1, 2, 3, false);
-        }
+        if (wt.nsb == 1) LDSF_ADJ_P(1, 1, false); else LDSF_ADJ_P(2, 3, false);
     }
+#undef LDSF_ADJ_P
 #undef LDSF_ADJ
     return hipGetLastError();
 }

@@ -361,6 +361,11 @@ GRIDS = {
     # g4 only, lambda not +/-1, a non-integer zeta
     "g4_general": (["g4"], [4, 3], [1.0, 0.5, 0.002, 2.5, 0.5, 0.002, 1.0, 0.5, 0.02,
                                     2.5, 0.5, 0.02]),
+    # 2 g2 rows + the power-of-two grid zeta (1, 2, 4, 8) x lambda +-1 x 2 eta (the
+    # FP32-bulk kernels' compile-time squaring chain for this grid)
+    "zeta_1248": (["g2", "g4"], [2, 2, 16, 3],
+                  [0.01, 0.0, 0.2, 0.0] + [v for z in (1.0, 2.0, 4.0, 8.0) for lam in (1.0, -1.0)
+                                           for e in (0.002, 0.02) for v in (z, lam, e)]),
 }
 
 
@@ -384,19 +389,22 @@ def test_wave_kernels_on_partial_grids(grid, precision):
     x = t(L["coords"], torch.float64)
     g = t(np.random.default_rng(21).normal(size=(n, ds.width)), torch.float64)
     out = {}
-    for fam in (0, 1):
-        plan.kernel_family = fam
+    for fam in (0, 1, "fused"):
+        plan.kernel_family = 1 if fam == "fused" else fam
         d = torch.full((n, ds.width), 0.0, dtype=torch.float64, device="cuda")
         dx = torch.zeros_like(x)
-        plan.compute(x, d, precision)
-        plan.gradient(x, g, dx, precision)
+        if fam == "fused":
+            plan.forward_gradient(x, g, d, dx, precision)
+        else:
+            plan.compute(x, d, precision)
+            plan.gradient(x, g, dx, precision)
         torch.cuda.synchronize()
         out[fam] = (d.cpu().numpy(), dx.cpu().numpy())
     tol = TOL64 if precision == 0 else TOL32
     O = oracle_of(species, 5.0, kinds=kinds, resolved=True)
     d0 = O.compute(n, L["species"], L["neighlist"], L["numneigh"], L["coords"])
     G0 = O.gradient(n, L["species"], L["neighlist"], L["numneigh"], L["coords"], g.cpu().numpy())
-    for fam in (0, 1):
+    for fam in (0, 1, "fused"):
         sig = np.abs(d0) >= 1e-6 * np.abs(d0).max(axis=0, keepdims=True)
         rel = np.abs(out[fam][0] - d0)[sig] / np.abs(d0)[sig]
         assert rel.max() <= tol, (fam, rel.max())
