@@ -72,6 +72,33 @@ __device__ __forceinline__ double ref_base(double lam, double cosv)
     return 1 + lam * cosv;
 }
 
+// ---- the gather: a row's dE/dr records summed in ascending order ----
+
+// s += records b .. e-1 of `edge` (3 doubles each), added one record after the
+// other; four records' loads are issued before their adds, so a thread keeps
+// 12 loads in flight (the order of the adds, and so the sum, is unchanged)
+__device__ __forceinline__ void row_sum3(const double *__restrict__ edge, long b, long e,
+                                         double &s0, double &s1, double &s2)
+{
+    long p = b;
+    for (; p + 4 <= e; p += 4) {
+        double v[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = edge[3 * p + k];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            s0 += v[3 * r];
+            s1 += v[3 * r + 1];
+            s2 += v[3 * r + 2];
+        }
+    }
+    for (; p < e; ++p) {
+        s0 += edge[3 * p];
+        s1 += edge[3 * p + 1];
+        s2 += edge[3 * p + 2];
+    }
+}
+
 // ---- transposed neighbour lists ----
 
 // Sorts ids[b .. b+m) ascending with one wavefront (all 64 lanes call it with

@@ -102,11 +102,7 @@ __global__ void k_gather(int n_total, const int *__restrict__ in_start,
     int b = in_start[a], e = in_start[a + 1];
     if (b == e) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (long p = b; p < e; ++p) {
-        s0 += edge[3 * p];
-        s1 += edge[3 * p + 1];
-        s2 += edge[3 * p + 2];
-    }
+    row_sum3(edge, b, e, s0, s1, s2);
     dcoords[3 * (long)a] += s0;
     dcoords[3 * (long)a + 1] += s1;
     dcoords[3 * (long)a + 2] += s2;

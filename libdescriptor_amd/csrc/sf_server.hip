@@ -298,11 +298,7 @@ __global__ void k_gather_out(const int *__restrict__ hdr, const Seg *__restrict_
         const Seg &S = seg[row_req[a]];
         if (!S.dcoords) continue;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (long p = in_start[a]; p < in_start[a + 1]; ++p) {
-            s0 += edge[3 * p];
-            s1 += edge[3 * p + 1];
-            s2 += edge[3 * p + 2];
-        }
+        row_sum3(edge, in_start[a], in_start[a + 1], s0, s1, s2);
         double *o = S.dcoords + 3 * (a - S.row_base);
         o[0] = s0;
         o[1] = s1;
