@@ -58,3 +58,64 @@ class SymmetryFunctionsLayer(torch.nn.Module):
 
     def forward(self, coords):
         return SymmetryFunctionsFn.apply(coords, self.plan, self.width, self.precision)
+
+
+class _FromPositionsFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, positions, module):
+        pos = positions.detach().contiguous()
+        lay = _capi.DeviceLayout(pos, module.species, module.cell, module.pbc, module.cutoff)
+        plan = _capi.Plan(module.descriptor._handle, lay.n_contrib + lay.n_padding, lay.species,
+                          lay.neighlist, lay.numneigh)
+        desc = torch.zeros((lay.n_contrib, module.width), dtype=torch.float64,
+                           device=positions.device)
+        plan.compute(lay.coords, desc, module.precision)
+        ctx.lay, ctx.plan, ctx.precision = lay, plan, module.precision
+        module.last_layout = lay
+        return desc
+
+    @staticmethod
+    def backward(ctx, grad_desc):
+        g = grad_desc.to(torch.float64).contiguous()
+        lay = ctx.lay
+        drows = torch.zeros_like(lay.coords)
+        ctx.plan.gradient(lay.coords, g, drows, ctx.precision)
+        # padding rows are their atoms shifted by lattice vectors: fold them back
+        return lay.assemble_forces(drows), None
+
+
+class SymmetryFunctionsFromPositions(torch.nn.Module):
+    """Descriptors of the atoms of one configuration straight from their positions
+    on the device, differentiable with respect to those positions.
+
+    Each call builds the padding rows and the KLIFF-order neighbour list on the
+    GPU (``ldsf_layout_create``, the device twin of NeighborList,
+    python_package/libdescriptor/neighbor.py:35-154), a plan over it and the
+    forward kernel; the backward runs the adjoint kernel, the gather, and the
+    fold of the padding rows onto their atoms (assemble_forces,
+    neighbor.py:227-263).  This is the reference's NeighborList -> compute ->
+    gradient -> assemble_forces pipeline as one autograd node, with nothing
+    leaving the device but the layout's sizes.
+
+    ``species`` are int codes per atom (device or host), ``cell`` a 3x3 host
+    array (rows are the cell vectors), ``pbc`` three flags, ``cutoff`` the
+    neighbour-list cutoff (the influence distance)."""
+
+    def __init__(self, descriptor, species, cell, pbc, cutoff, precision=_capi.FP64,
+                 device="cuda"):
+        super().__init__()
+        import numpy as np
+        self.descriptor = descriptor
+        self.width = descriptor.width
+        self.precision = precision
+        self.species = torch.as_tensor(species, dtype=torch.int32).to(torch.device(device))
+        self.species = self.species.contiguous()
+        self.cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+        self.pbc = np.ascontiguousarray(pbc, dtype=np.int32).reshape(3)
+        self.cutoff = float(cutoff)
+        self.last_layout = None
+
+    def forward(self, positions):
+        if positions.dtype != torch.float64 or not positions.is_cuda:
+            raise TypeError("positions must be a float64 tensor on the GPU")
+        return _FromPositionsFn.apply(positions, self)

@@ -203,3 +203,58 @@ def test_device_layout_feeds_the_plan(c1):
     G = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"], c1["dE_dzeta"])
     F0 = nbl.assemble_forces(G, 64, c1["image"])
     assert float(np.abs(forces - F0).max() / np.abs(F0).max()) <= 1e-12
+
+
+def test_positions_module_c1(c1):
+    """SymmetryFunctionsFromPositions: positions -> device layout -> descriptors, and
+    the backward -> adjoint -> gather -> fold, against the golden descriptors and the
+    oracle's folded gradient."""
+    import libdescriptor_amd as ld
+    from libdescriptor_amd.autograd import SymmetryFunctionsFromPositions
+    from libdescriptor_amd.workloads import behler_si_params
+    from oracle import nbl, orc
+    print("case: C1 positions module", flush=True)
+    d = _dev()
+    types, sizes, params = behler_si_params()
+    ds = ld.DescriptorKind.init_descriptor(ld.AvailableDescriptors.SymmetryFunctions, ["Si"], "cos",
+                                           4.5, types, sizes, params)
+    mod = SymmetryFunctionsFromPositions(ds, np.zeros(64, np.int32), c1["cell"], [1, 1, 1], 4.5)
+    pos = torch.tensor(c1["positions"], device=d, requires_grad=True)
+    desc = mod(pos)
+    (desc * torch.tensor(c1["dE_dzeta"], device=d)).sum().backward()
+    assert float(np.abs(desc.detach().cpu().numpy() - c1["desc"]).max()
+                 / np.abs(c1["desc"]).max()) <= 1e-12
+    O = orc.OracleSF(["Si"], 4.5, types, sizes, params)
+    G = O.gradient(64, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"], c1["dE_dzeta"])
+    F0 = nbl.assemble_forces(G, 64, c1["image"])
+    F = pos.grad.cpu().numpy()
+    assert float(np.abs(F - F0).max() / np.abs(F0).max()) <= 1e-12
+
+
+def test_positions_module_central_fd():
+    """d(sum w * desc)/d(positions) by autograd against central differences of the whole
+    pipeline (layout rebuilt on the GPU at every displaced position), two species, FP64."""
+    import libdescriptor_amd as ld
+    from libdescriptor_amd import workloads
+    from libdescriptor_amd.autograd import SymmetryFunctionsFromPositions
+    print("case: positions module central FD", flush=True)
+    d = _dev()
+    pos0, cell, pbc, sp = workloads.random_system(150, 0.05, 1.8, seed=8)
+    types, sizes, params = workloads.behler_si_params()
+    ds = ld.DescriptorKind.init_symmetry_functions(["A", "B"], 5.0, types, sizes, params,
+                                                   species_resolved=True)
+    mod = SymmetryFunctionsFromPositions(ds, sp, cell, pbc, 5.0)
+    w = torch.tensor(np.random.default_rng(2).normal(size=(150, ds.width)), device=d)
+    pos = torch.tensor(pos0, device=d, requires_grad=True)
+    (mod(pos) * w).sum().backward()
+    G = pos.grad.cpu().numpy()
+    h = 1e-5
+    for a, c in [(0, 0), (17, 1), (88, 2), (149, 0)]:
+        vals = []
+        for sgn in (1, -1):
+            p = pos0.copy()
+            p[a, c] += sgn * h
+            with torch.no_grad():
+                vals.append(float((mod(torch.tensor(p, device=d)) * w).sum()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - G[a, c]) <= 1e-6 * max(1.0, abs(G[a, c])), (a, c, fd, G[a, c])
