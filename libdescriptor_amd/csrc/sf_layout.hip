@@ -538,13 +538,14 @@ struct ldsf_layout {
 
 namespace {
 
-long read_box(const Box *partial, int nb, Box &out, Box *dev_out, hipStream_t s)
+// the extent over the blocks' partial boxes, read back to the host
+hipError_t read_box(const Box *partial, int nb, Box &out, Box *dev_out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, s, nb, partial, dev_out);
-    if (hipGetLastError() != hipSuccess) return -1;
-    if (hipMemcpyAsync(&out, dev_out, sizeof(Box), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    return 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&out, dev_out, sizeof(Box), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
 }
 
 // padding rows appended to the contributing rows (nbl_create_paddings)
@@ -570,7 +571,7 @@ int make_padding(ldsf_layout *L, int n, double cutoff, const double *cell, const
     if (n > 0) {
         hipLaunchKernelGGL(k_frac, dim3(nb), dim3(kBlock), 0, s, n, fm, d_coords, frac.p, partial.p);
         HIP_TRY(hipGetLastError(), "frac");
-        if (read_box(partial.p, nb, box, boxd.p, s)) return hip_fail(hipGetLastError(), "extent");
+        HIP_TRY(read_box(partial.p, nb, box, boxd.p, s), "extent");
     } else {
         for (int d = 0; d < 3; ++d) {
             box.lo[d] = 1e10;
@@ -669,7 +670,7 @@ int make_neighbors(ldsf_layout *L, int n_total, int n_query, double infl, hipStr
     hipLaunchKernelGGL(k_extent, dim3(nb), dim3(kBlock), 0, s, n_total, x, partial.p);
     HIP_TRY(hipGetLastError(), "extent");
     Box box;
-    if (read_box(partial.p, nb, box, boxd.p, s)) return hip_fail(hipGetLastError(), "extent");
+    HIP_TRY(read_box(partial.p, nb, box, boxd.p, s), "extent");
     double x0[3];
     HIP_TRY(hipMemcpy(x0, x, sizeof x0, hipMemcpyDeviceToHost), "copy coords");
     Grid g;
