@@ -61,6 +61,8 @@ struct WaveParams {
     int nsq;                     // squarings needed for the largest integer zeta
     int slots[64];               // [0,16): g2 slot of k = s*8+q; [16,64): g4 slot of
                                  // p*16 + (z*2+l)*2+e; -1 unused
+    float e2l[8], m2e[8], rs2f[8];  // FP32-bulk g2 rows: eta log2(e), -2 eta, Rs (0 past
+                                    // the set's rows)
 };
 struct WaveTab {
     int n_species;

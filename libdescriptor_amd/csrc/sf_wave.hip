@@ -683,6 +683,47 @@ __device__ __forceinline__ float base_f(const Fast &F, int l, double sp, double 
     return fmaxf((float)num * inv, 1e-30f);
 }
 
+// The g2 rows of one neighbour at distance r in the FP32-bulk form, with
+// phi_q = exp(-eta_q (r - Rs_q)^2) fc(r).  ADJ: returns sum_q dE/dzeta_q
+// d(phi_q)/dr (weights gw[8 sj + q], FP32, in LDS).  FWD: sets the terms
+// phi_q into the neighbour's species block of acc (the other block 0).  The
+// exponent is formed in FP32: relative error <= |eta (r - Rs)^2| * 1e-7 per
+// term, the largest terms having the smallest exponents.  Rows past the set's
+// g2 rows have eta 0, weight 0 and no slot.
+template <int NSB, bool ADJ, bool FWD>
+__device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *gw, float r,
+                                             float fc, float dfc, int sj,
+                                             double (&acc)[NSB * 8])
+{
+    float gq[8];
+    if (ADJ) {
+        const float2 *G = reinterpret_cast<const float2 *>(gw + (NSB == 1 ? 0 : 8 * sj));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float2 v = G[k];
+            gq[2 * k] = v.x;
+            gq[2 * k + 1] = v.y;
+        }
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float dr = r - wp.rs2f[q];
+        const float ex = __builtin_amdgcn_exp2f(-wp.e2l[q] * (dr * dr));
+        if (ADJ) sum = fmaf(gq[q], ex * fmaf(wp.m2e[q] * dr, fc, dfc), sum);
+        if (FWD) {
+            const float v = ex * fc;
+            if (NSB == 1) {
+                acc[q] = (double)v;
+            } else {
+                acc[q] = (double)(sj ? 0.0f : v);
+                acc[8 + q] = (double)(sj ? v : 0.0f);
+            }
+        }
+    }
+    return sum;
+}
+
 // Phase 2: the active pairs (a < b, r_jk < rc_jk) of the M compacted
 // neighbours, walked in diagonal order, compacted into one region of
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
@@ -1001,9 +1042,12 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 float4 *V = reinterpret_cast<float4 *>(D + 2 * kW);
                 D[pos] = make_double2(d0, d1);
                 D[kW + pos] = make_double2(d2, r);
-                V[pos] = make_float4((float)(0.70710678118654752 / r),
-                                     (float)fc * dexp(-wp.eta3[0] * r2, R),
-                                     (float)fc * dexp(-wp.eta3[1] * r2, R), __int_as_float(sj));
+                const float r2f = (float)r2;
+                V[pos] = make_float4(0.70710678f * rcp_refined((float)r),
+                                     (float)fc * __builtin_amdgcn_exp2f(-F.etal[0] * r2f),
+                                     (float)fc * __builtin_amdgcn_exp2f(-F.etal[1] * r2f),
+                                     __int_as_float(sj));
+                pair_rows_f<NSB, false, true>(wp, nullptr, (float)r, fc, 0.0f, sj, acc2);
             } else {
                 L.px[pos] = B.p0;
                 L.py[pos] = B.p1;
@@ -1016,7 +1060,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                if (P == 1 || q < wt.nr2) {  // rows past nr2: eta 0, no slot
+                if (P == 0 && q < wt.nr2) {
                     const double dr = r - wp.rs2[q];
                     const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
                     if (NSB == 1) {
@@ -1222,18 +1266,16 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
 // reference's gradient adds the forward into desc as well,
 // src/Descriptors.cpp:165-190), its triplet lanes dealt the forward kernel's
 // 4-lane columns so that the row sums reduce the same way.
+// Occupancy floors (waves per SIMD): FP64 fused 2, FP32-bulk fused 3,
+// FP32-bulk adjoint 4 (128 VGPRs).
 template <int P, int NSB, int NPB, bool FWD, int ZG>
-__global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(WaveTab wt, int n_centers,
-                                                 const int *__restrict__ centers,
-                                                 const int *__restrict__ offs,
-                                                 const int *__restrict__ nl,
-                                                 const int *__restrict__ species,
-                                                 const double *__restrict__ x,
-                                                 const double *__restrict__ dE,
-                                                 const int *__restrict__ inv_pos,
-                                                 double *__restrict__ edge, int n_edges,
-                                                 double *__restrict__ desc, int width,
-                                                 int maxn, DevCounts dc)
+__global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : (P == 0 ? 1 : 4))
+void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
+                const int *__restrict__ offs, const int *__restrict__ nl,
+                const int *__restrict__ species, const double *__restrict__ x,
+                const double *__restrict__ dE, const int *__restrict__ inv_pos,
+                double *__restrict__ edge, int n_edges, double *__restrict__ desc, int width,
+                int maxn, DevCounts dc)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -1280,6 +1322,14 @@ __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(Wav
         // a skipped centre (flagged request) writes only its zero self record
         if (C.work && s < width) L.g[s] = gs;
         if (FWD && s < kWaveSlots) L.d[s] = rin;
+        // FP32-bulk: the g2 rows' weights per species block, from the lanes
+        // that hold them (past the triplet weights in L.w)
+        float *gw = reinterpret_cast<float *>(L.w) + 48;
+        if constexpr (P == 1) {
+            const int slot = wp.slots[s & 15];
+            const double gv = __shfl(gs, slot < 0 ? 0 : slot);
+            if (s < 8 * NSB) gw[s] = slot >= 0 ? (float)gv : 0.0f;
+        }
         __syncthreads();
 
         // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
@@ -1314,9 +1364,11 @@ __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(Wav
             T fc, dfc;
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
             double sum = 0.0;
+            if constexpr (P == 1)
+                sum = (double)pair_rows_f<NSB, true, FWD>(wp, gw, (float)r, fc, dfc, sj, acc2);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                if (q < wt.nr2) {  // L.g[slot] needs a row of the set
+                if (P == 0 && q < wt.nr2) {  // L.g[slot] needs a row of the set
                     const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
                     const double dr = r - wp.rs2[q];
                     const T ex = dexp(-wp.eta2[q] * dr * dr, R);
@@ -1333,15 +1385,18 @@ __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : 1) void k_adj_wave(Wav
                     }
                 }
             }
-            const double ir = 1.0 / r;
+            // 1/r: FP32 (one Newton step) in the FP32-bulk form
+            const double ir = P == 1 ? (double)rcp_refined((float)r) : 1.0 / r;
             if constexpr (P == 1) {
                 double2 *D = reinterpret_cast<double2 *>(L.px);
                 float4 *V = reinterpret_cast<float4 *>(D + 2 * kW);
                 float2 *W = reinterpret_cast<float2 *>(V + kW);
+                const float r2f = (float)r2;
                 D[pos] = make_double2(d0, d1);
                 D[kW + pos] = make_double2(d2, r);
                 V[pos] = make_float4((float)ir, fc, dfc, __int_as_float(sj));
-                W[pos] = make_float2(dexp(-wp.eta3[0] * r2, R), dexp(-wp.eta3[1] * r2, R));
+                W[pos] = make_float2(__builtin_amdgcn_exp2f(-F.etal[0] * r2f),
+                                     __builtin_amdgcn_exp2f(-F.etal[1] * r2f));
             } else {
                 L.px[pos] = B.p0;
                 L.py[pos] = B.p1;
@@ -1622,6 +1677,9 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
         if (sf->k2[q] != G2) return false;
         wp->eta2[q] = sf->a2[q];
         wp->rs2[q] = sf->b2[q];
+        wp->e2l[q] = (float)(sf->a2[q] * 1.4426950408889634);
+        wp->m2e[q] = (float)(-2.0 * sf->a2[q]);
+        wp->rs2f[q] = (float)sf->b2[q];
         for (int s = 0; s < wt->nsb; ++s) slots[s * 8 + q] = sf->slot2[q] + s * sf->str2[q];
     }
     wt->nr2 = n2;
