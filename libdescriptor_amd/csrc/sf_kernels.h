@@ -70,6 +70,7 @@ struct WaveTab {
     int nz, nl, ne;         // distinct zeta, lambda, eta of the g4 grid
     int nsb, npb;           // species blocks (1 or 2) and pair blocks (1 or 3)
     int has3;               // any g4 row
+    int rc_uniform;         // every species pair has the same cutoff
     int zgrid;              // zeta grid (1, 2, 4, 16) = 1, (1, 2, 4, 8) = 2, else 0: the FP32-bulk
                             // kernels' compile-time squaring chains
     const double *rc;       // device, n_species^2

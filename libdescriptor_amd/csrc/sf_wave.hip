@@ -729,8 +729,9 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
-// n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.
-template <int P, int NPB>
+// n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.  UNI: every species pair has the cutoff
+// of pair 0 (no per-pair choice).
+template <int P, int NPB, bool UNI>
 __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, const Fast &F,
                                                 int ns, int M, int n1, int s, int code0,
                                                 int (&cnt)[3], int (&base)[3])
@@ -757,7 +758,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                     dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
                 sa = L.sp[a];
                 sb = L.sp[b];
-                act = sqrt(rjk2) < S.rc[pidx(sa, sb, ns)];
+                act = sqrt(rjk2) < S.rc[UNI ? 0 : pidx(sa, sb, ns)];
             } else {
                 // the triplet phase forms the same r_jk^2 from the same operands:
                 // an active pair has r_jk^2 < rc^2 there too.  (The reference
@@ -769,8 +770,9 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                 const double rjk2 = rjk2_of(D[a], D[kW + a], D[b], D[kW + b], ex, ey, ez);
                 sa = __float_as_int(V[a].w);
                 sb = __float_as_int(V[b].w);
-                const double rc2 = NPB > 1 ? blk3(sa + sb, F.rq0, F.rq1, F.rq2)
-                                           : (ns == 1 ? F.rq0 : S.rc2[pidx(sa, sb, ns)]);
+                const double rc2 = (UNI || ns == 1) ? F.rq0
+                                   : (NPB > 1 ? blk3(sa + sb, F.rq0, F.rq1, F.rq2)
+                                              : S.rc2[pidx(sa, sb, ns)]);
                 act = rjk2 < rc2;
             }
             if (NPB > 1) blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
@@ -1080,7 +1082,10 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, code0, cnt, base);
+            if (wt.rc_uniform)
+                enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base);
+            else
+                enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base);
             Bn = load_nbr(jn, species, x);
             __syncthreads();
             // the blocks are dealt columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48;
@@ -1422,7 +1427,10 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         Nbr Bn;
         if (wt.has3) {
             int cnt[3], base[3];
-            enumerate_pairs<P, NPB>(S, L, F, ns, M, n1, s, code0, cnt, base);
+            if (wt.rc_uniform)
+                enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base);
+            else
+                enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base);
             Bn = load_nbr(jn, species, x);
             // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda,
             // kept in the compute type (a loop-invariant copy costs half the registers)
@@ -1668,6 +1676,8 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
     wt->n_species = ns;
     if (ns > kMaxWaveSpecies) return false;
     if (sf->species_resolved && ns > 2) return false;
+    wt->rc_uniform = 1;
+    for (double v : sf->rcut) wt->rc_uniform &= v == sf->rcut[0] ? 1 : 0;
     wt->nsb = (sf->species_resolved && ns == 2) ? 2 : 1;
     wt->npb = (sf->species_resolved && ns == 2) ? 3 : 1;
     // two-body rows: g2 only, at most 8

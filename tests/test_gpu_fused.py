@@ -113,6 +113,29 @@ def test_fused_two_species_against_oracle(family):
     np.testing.assert_array_equal(G2, G32)
 
 
+@pytest.mark.parametrize("resolved,rc", [
+    (True, [[5.0, 4.6], [4.6, 4.2]]),    # three pair blocks, one cutoff each
+    (False, [[5.0, 4.4], [4.7, 4.1]]),   # one block, the cutoff looked up per species pair
+])
+def test_wave_kernels_species_pair_cutoffs(resolved, rc):
+    """The wave kernels with a cutoff matrix that differs per species pair (the
+    reference's rcut_2D_, src/SymmetryFunctions.cpp:150-157,203-207), fused and
+    separate, both precisions, against the oracle."""
+    print(f"case: 400-atom two-species, cutoff matrix {rc}, resolved {resolved}", flush=True)
+    rcm = np.array(rc)
+    S = Setup(two_species(400, 12), rc=rcm.reshape(-1), resolved=resolved)
+    assert S.plan.kernel_family == 1
+    d0, G0 = S.oracle()
+    blocks = BLOCKS if resolved else [(0, 8), (8, 24)]
+    for fused in (True, False):
+        d, G = S.fused(0) if fused else S.split(0)
+        assert elem_rel_err(d, d0, 1e-300) <= TOL64
+        assert rel_err(G, G0) <= TOL64
+        d32, G32 = S.fused(1) if fused else S.split(1)
+        assert signal_rel(d32, d0, blocks) <= TOL32
+        assert rel_err(G32, G0) <= TOL32
+
+
 def test_fused_accumulates_into_desc(c1):
     """desc += descriptors (the reference's += semantics), single species."""
     print("case: C1 fused +=", flush=True)
