@@ -345,7 +345,7 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 // (stride 65, not 64: arrays read or updated together fall in different banks)
 constexpr int kStr = kW + 1;
 constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
-constexpr int kAdjDoubles = 11 * kStr + 48 + 48 + kWaveSlots;  // px..fz, w, wv, g
+constexpr int kAdjDoubles = 11 * kStr + 1 + 48 + 48 + kWaveSlots;  // px..fz, pad, w, wv, g
 constexpr int kFusedDoubles = kAdjDoubles + kWaveSlots;        // ... and d
 
 // kernel kinds: the forward, the adjoint, and the fused forward + adjoint
@@ -392,9 +392,9 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, int kind)
         L.fx = p + 8 * kStr;
         L.fy = p + 9 * kStr;
         L.fz = p + 10 * kStr;
-        L.w = p + 11 * kStr;
-        L.wv = p + 11 * kStr + 48;
-        L.g = p + 11 * kStr + 96;
+        L.w = p + 11 * kStr + 1;  // 16-byte aligned: read as float4
+        L.wv = L.w + 48;
+        L.g = L.w + 96;
         L.d = kind == kFused ? p + kAdjDoubles : nullptr;
     } else {
         L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
@@ -914,6 +914,25 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     const float fa = va.y, fb = vb.y, dfa = va.z, dfb = vb.z;
     const float inv = 0.5f * ria * rib;  // 1 / (2 r_ij r_ik)
     const double sp = (a1.y + b1.y) * (a1.y + b1.y), sm = (a1.y - b1.y) * (a1.y - b1.y);
+    // the block's row weights: the adjoint reads all 32 up front, four to an
+    // LDS read; the fused kernel, short of registers, two at their use
+    float wl[16], wvl[16];
+    if constexpr (!FWD) {
+        const float4 *w4 = reinterpret_cast<const float4 *>(__builtin_assume_aligned(w, 16));
+        const float4 *v4 = reinterpret_cast<const float4 *>(__builtin_assume_aligned(wv, 16));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 a = w4[k], b = v4[k];
+            wl[4 * k] = a.x;
+            wl[4 * k + 1] = a.y;
+            wl[4 * k + 2] = a.z;
+            wl[4 * k + 3] = a.w;
+            wvl[4 * k] = b.x;
+            wvl[4 * k + 1] = b.y;
+            wvl[4 * k + 2] = b.z;
+            wvl[4 * k + 3] = b.w;
+        }
+    }
     float E[2];
     E[0] = ea.x * eb.x * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2);
     E[1] = ea.y * eb.y * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2);
@@ -943,8 +962,9 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
             const float pw = ZG != 0 ? sq[ZGrid<ZG>::e[z]] : pm * bl;
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                U[e] = fmaf(w[(z * 2 + l) * 2 + e], pw, U[e]);
-                Vs[e] = fmaf(wv[(z * 2 + l) * 2 + e], pm, Vs[e]);
+                const int k = (z * 2 + l) * 2 + e;
+                U[e] = fmaf(FWD ? w[k] : wl[k], pw, U[e]);
+                Vs[e] = fmaf(FWD ? wv[k] : wvl[k], pm, Vs[e]);
                 if (FWD) acc[(z * 2 + l) * 2 + e] = fma((double)pw, te[e], acc[(z * 2 + l) * 2 + e]);
             }
         }
