@@ -1699,6 +1699,9 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
     wt->rc_uniform = 1;
     for (double v : sf->rcut) wt->rc_uniform &= v == sf->rcut[0] ? 1 : 0;
     wt->nsb = (sf->species_resolved && ns == 2) ? 2 : 1;
+    // the FP32-bulk kernels give each two-species block one cutoff, rc(0,1)
+    // for the mixed block whichever of its neighbours comes first in the list
+    if (wt->nsb == 2 && sf->rcut[1] != sf->rcut[2]) return false;
     wt->npb = (sf->species_resolved && ns == 2) ? 3 : 1;
     // two-body rows: g2 only, at most 8
     const int n2 = (int)sf->k2.size();

@@ -113,18 +113,21 @@ def test_fused_two_species_against_oracle(family):
     np.testing.assert_array_equal(G2, G32)
 
 
-@pytest.mark.parametrize("resolved,rc", [
-    (True, [[5.0, 4.6], [4.6, 4.2]]),    # three pair blocks, one cutoff each
-    (False, [[5.0, 4.4], [4.7, 4.1]]),   # one block, the cutoff looked up per species pair
+@pytest.mark.parametrize("resolved,rc,family", [
+    (True, [[5.0, 4.6], [4.6, 4.2]], 1),    # three pair blocks, one cutoff each
+    (False, [[5.0, 4.4], [4.7, 4.1]], 1),   # one block, the cutoff looked up per species pair
+    (True, [[5.0, 4.4], [4.7, 4.1]], 0),    # resolved, rc(A, B) != rc(B, A): general kernels
 ])
-def test_wave_kernels_species_pair_cutoffs(resolved, rc):
-    """The wave kernels with a cutoff matrix that differs per species pair (the
-    reference's rcut_2D_, src/SymmetryFunctions.cpp:150-157,203-207), fused and
-    separate, both precisions, against the oracle."""
+def test_species_pair_cutoffs(resolved, rc, family):
+    """Cutoff matrices that differ per species pair (the reference's rcut_2D_,
+    src/SymmetryFunctions.cpp:150-157,203-207; asymmetric ones only through
+    the vector init), fused and separate, both precisions, against the oracle.
+    The wave kernels take the species-resolved blocks only with a symmetric
+    matrix."""
     print(f"case: 400-atom two-species, cutoff matrix {rc}, resolved {resolved}", flush=True)
     rcm = np.array(rc)
     S = Setup(two_species(400, 12), rc=rcm.reshape(-1), resolved=resolved)
-    assert S.plan.kernel_family == 1
+    assert S.plan.kernel_family == family
     d0, G0 = S.oracle()
     blocks = BLOCKS if resolved else [(0, 8), (8, 24)]
     for fused in (True, False):
