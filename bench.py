@@ -122,7 +122,7 @@ def main():
     x = torch.tensor(L["coords"], dtype=torch.float64, device=dev)
     rng = np.random.default_rng(1)
     g = torch.tensor(rng.normal(size=(n, W)), dtype=torch.float64, device=dev)
-    plan = _capi.Plan(ds._handle, n_total, sp, nl, nn)
+    plan = _capi.Plan(ds, n_total, sp, nl, nn)
     if args.family >= 0:
         plan.kernel_family = args.family
     log(f"plan ready: family {plan.kernel_family}, max neighbours {plan.max_neighbors}")
@@ -345,7 +345,7 @@ def bench_batched(args, ds):
     n = int(B["n_atoms"].sum())
     x = t(B["coords"], torch.float64)
     g = t(np.random.default_rng(2).normal(size=(n, ds.width)), torch.float64)
-    plan = _capi.Plan(ds._handle, len(B["coords"]), t(B["species"]), t(B["neighlist"]),
+    plan = _capi.Plan(ds, len(B["coords"]), t(B["species"]), t(B["neighlist"]),
                       t(B["numneigh"]), n_atoms=B["n_atoms"], config_ptr=B["config_ptr"])
     desc = torch.zeros((n, ds.width), dtype=torch.float64, device=dev)
     dx = torch.zeros_like(x)
@@ -409,7 +409,7 @@ def bench_serving(args):
     atoms = int(sum(P[k]["n"] for k in pick))
 
     def run(lanes, lc, min_centers, window_us, load, dump=None):
-        srv = _capi.Server(ds._handle, _capi.FP32_BULK, hbm_budget_bytes=8 << 30,
+        srv = _capi.Server(ds, _capi.FP32_BULK, hbm_budget_bytes=8 << 30,
                            lane_centers=lc, lane_rows=max(3 * lc, 65536), lane_edges=lc * 48,
                            max_requests=256, max_lanes=lanes, dispatch_min_centers=min_centers,
                            batch_window_us=window_us)

@@ -161,9 +161,10 @@ LDSF_API int ldsf_num_gradient_single_atom(ldsf_sf *sf, int index, int n_atoms,
  * `stream` (a hipStream_t, NULL = default stream).
  * ------------------------------------------------------------------------ */
 
-/* Plan over centres centers[0..n_centers) (device int32; NULL = rows
+/* Plan over centres centers[0..n_centers) (device int32, copied; NULL = rows
  * 0..n_centers-1), with n_total coordinate rows.  Synchronises `stream` once
- * (to size the launch from the largest neighbour count). */
+ * (to size the launch from the largest neighbour count).  The plan keeps sf,
+ * d_species and d_neighbor_list without copying them: they must outlive it. */
 LDSF_API int ldsf_plan_create(ldsf_plan **out, const ldsf_sf *sf, int n_centers,
                               const int *d_centers, int n_total, const int *d_species,
                               const int *d_neighbor_list, const int *d_number_of_neighs,
@@ -265,7 +266,7 @@ typedef struct {
 } ldsf_request;
 
 /* Create on the current HIP device; starts one worker and one completion
- * thread per lane. */
+ * thread per lane.  The server uses sf's device tables: sf must outlive it. */
 LDSF_API int ldsf_server_create(ldsf_server **out, const ldsf_sf *sf,
                                 const ldsf_server_config *config);
 LDSF_API int ldsf_server_num_lanes(const ldsf_server *server);

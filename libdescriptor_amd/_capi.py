@@ -136,13 +136,15 @@ def _ptr(t):
 
 class Plan:
     """A device plan (ldsf_plan) over device-resident int32 species,
-    neighbour list and neighbour counts.  Keeps references to the tensors it
-    borrows so they outlive the plan."""
+    neighbour list and neighbour counts.  `sf` is a DescriptorKind (kept
+    alive with the plan) or a raw ldsf_sf handle (the caller keeps it alive).
+    Keeps references to the tensors it borrows so they outlive the plan."""
 
-    def __init__(self, sf_handle, n_total, species, neighlist, numneigh, centers=None,
+    def __init__(self, sf, n_total, species, neighlist, numneigh, centers=None,
                  n_atoms=None, config_ptr=None, stream=None):
         import numpy as np
-        self._keep = (species, neighlist, numneigh, centers)
+        sf_handle = sf._handle if hasattr(sf, "_handle") else sf
+        self._keep = (sf, species, neighlist, numneigh, centers)
         out = ctypes.c_void_p()
         st = _stream_ptr(stream)
         n_centers = int(numneigh.numel()) if hasattr(numneigh, "numel") else len(numneigh)
@@ -329,9 +331,12 @@ class Server:
     and returns its latency in seconds (submission to completion).  The
     tensors of a request are kept alive until its wait."""
 
-    def __init__(self, sf_handle, precision=FP32_BULK, hbm_budget_bytes=2 << 30,
+    def __init__(self, sf, precision=FP32_BULK, hbm_budget_bytes=2 << 30,
                  lane_centers=65536, lane_rows=196608, lane_edges=65536 * 48, max_requests=256,
                  max_lanes=4, dispatch_min_centers=0, batch_window_us=0):
+        # sf: a DescriptorKind (kept alive with the server) or a raw ldsf_sf handle
+        self._sf = sf
+        sf_handle = sf._handle if hasattr(sf, "_handle") else sf
         cfg = ServerConfig(int(precision), int(hbm_budget_bytes), int(lane_centers),
                            int(lane_rows), int(lane_edges), int(max_requests), int(max_lanes),
                            int(dispatch_min_centers), int(batch_window_us))

@@ -52,7 +52,7 @@ class SymmetryFunctionsLayer(torch.nn.Module):
         self.species = torch.as_tensor(species, dtype=torch.int32).to(dev).contiguous()
         self.neighlist = torch.as_tensor(neighlist, dtype=torch.int32).to(dev).contiguous()
         self.numneigh = torch.as_tensor(numneigh, dtype=torch.int32).to(dev).contiguous()
-        self.plan = _capi.Plan(descriptor._handle, int(self.species.numel()), self.species,
+        self.plan = _capi.Plan(descriptor, int(self.species.numel()), self.species,
                                self.neighlist, self.numneigh, n_atoms=n_atoms,
                                config_ptr=config_ptr)
 
@@ -65,7 +65,7 @@ class _FromPositionsFn(torch.autograd.Function):
     def forward(ctx, positions, module):
         pos = positions.detach().contiguous()
         lay = _capi.DeviceLayout(pos, module.species, module.cell, module.pbc, module.cutoff)
-        plan = _capi.Plan(module.descriptor._handle, lay.n_contrib + lay.n_padding, lay.species,
+        plan = _capi.Plan(module.descriptor, lay.n_contrib + lay.n_padding, lay.species,
                           lay.neighlist, lay.numneigh)
         desc = torch.zeros((lay.n_contrib, module.width), dtype=torch.float64,
                            device=positions.device)

@@ -666,7 +666,9 @@ __device__ __forceinline__ void geom_f(const double2 &a0, const double2 &a1, con
 {
     t.rjk2d = rjk2_of(a0, a1, b0, b1, t.ex, t.ey, t.ez);
     t.rjk2 = (float)t.rjk2d;
-    t.rjkinv = __builtin_amdgcn_rsqf(t.rjk2);
+    // (coincident neighbours, r_jk = 0: a finite 1 / r_jk and r_jk = 0, as the
+    // reference's forward, whose fc_jk(0) = 1)
+    t.rjkinv = __builtin_amdgcn_rsqf(fmaxf(t.rjk2, 1e-30f));
     t.rjk = t.rjk2 * t.rjkinv;
     // rc - r_jk = (rc^2 - r_jk^2) / (rc + r_jk) keeps its relative accuracy near rc
     t.y = (float)(rc2 - t.rjk2d) * __builtin_amdgcn_rcpf(rc + t.rjk) * rcs;
@@ -674,13 +676,14 @@ __device__ __forceinline__ void geom_f(const double2 &a0, const double2 &a1, con
 }
 
 // the base 1 + lambda_l cos(theta) = (lp (r_ij + r_ik)^2 - lm (r_ij - r_ik)^2 -
-// lambda r_jk^2) / (2 r_ij r_ik), numerator in FP64; a base <= 0 (rounding of
-// an exactly collinear triplet) becomes 1e-30, whose powers vanish
+// lambda r_jk^2) / (2 r_ij r_ik), numerator in FP64; a base <= 0 becomes 0,
+// and its powers and their derivatives 0 as the reference's base <= 0 branch
+// (src/SymmetryFunctions.cpp:118-119)
 __device__ __forceinline__ float base_f(const Fast &F, int l, double sp, double sm, double rjk2,
                                         float inv)
 {
     const double num = fma(-F.lam[l], rjk2, fma(F.lp[l], sp, -F.lm[l] * sm));
-    return fmaxf((float)num * inv, 1e-30f);
+    return fmaxf((float)num * inv, 0.0f);
 }
 
 // The g2 rows of one neighbour at distance r in the FP32-bulk form, with
@@ -948,17 +951,20 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
         const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
         // power-of-two grids: B^(2^k) and B^(2^k - 1) by squaring; else
         // B^(zeta - 1) = exp2((zeta - 1) log2 B)
+        // (B = 0: B^0 = 0 as well, the derivative of the reference's 0 branch)
         float sq[5], mk[5], lb = 0.0f;
         if constexpr (ZG != 0) {
             squares<ZG>(bl, sq);
             squares_m1<ZG>(sq, mk);
+            mk[0] = bl > 0.0f ? 1.0f : 0.0f;
         } else {
-            lb = __builtin_amdgcn_logf(bl);
+            lb = __builtin_amdgcn_logf(fmaxf(bl, 1e-30f));
         }
 #pragma unroll
         for (int z = 0; z < 4; ++z) {
             const float pm = ZG != 0 ? mk[ZGrid<ZG>::e[z]]  // B^(zeta - 1)
-                                     : __builtin_amdgcn_exp2f((F.zeta[z] - 1.0f) * lb);
+                                     : (bl > 0.0f ? __builtin_amdgcn_exp2f((F.zeta[z] - 1.0f) * lb)
+                                                  : 0.0f);
             const float pw = ZG != 0 ? sq[ZGrid<ZG>::e[z]] : pm * bl;
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
@@ -1177,11 +1183,13 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                                 const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
                                 float sq[5], lb = 0.0f;
                                 if constexpr (ZG != 0) squares<ZG>(bl, sq);
-                                else lb = __builtin_amdgcn_logf(bl);
+                                else lb = __builtin_amdgcn_logf(fmaxf(bl, 1e-30f));
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
-                                    const double pw = ZG != 0 ? (double)sq[ZGrid<ZG>::e[z]]
-                                                              : (double)__builtin_amdgcn_exp2f(F.zeta[z] * lb);
+                                    const double pw =
+                                        ZG != 0 ? (double)sq[ZGrid<ZG>::e[z]]
+                                                : (bl > 0.0f ? (double)__builtin_amdgcn_exp2f(F.zeta[z] * lb)
+                                                             : 0.0);
 #pragma unroll
                                     for (int e = 0; e < 2; ++e)
                                         acc[(z * 2 + l) * 2 + e] = fma(pw, te[e], acc[(z * 2 + l) * 2 + e]);

@@ -59,7 +59,7 @@ class Setup:
         self.L = L
         self.n = L["n_contrib"]
         self.x = t(L["coords"], torch.float64)
-        self.plan = _capi.Plan(self.ds._handle, len(L["coords"]), t(L["species"]),
+        self.plan = _capi.Plan(self.ds, len(L["coords"]), t(L["species"]),
                                t(L["neighlist"]), t(L["numneigh"]))
         self.W = self.ds.width
         self.g = t(np.random.default_rng(5).normal(size=(self.n, self.W)), torch.float64)

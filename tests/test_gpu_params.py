@@ -54,7 +54,7 @@ def test_param_gradient_two_species_system_on_device():
                                                    species_resolved=True)
     d = torch.device("cuda:0")
     t = lambda a, dt=torch.int32: torch.tensor(np.asarray(a), dtype=dt, device=d)  # noqa: E731
-    plan = _capi.Plan(ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
+    plan = _capi.Plan(ds, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
                       t(L["numneigh"]))
     g = np.random.default_rng(4).normal(size=(n, ds.width))
     x, gd = t(L["coords"], torch.float64), t(g, torch.float64)

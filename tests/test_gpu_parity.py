@@ -237,11 +237,11 @@ def test_out_of_range_neighbour_is_rejected(c1):
     bad[5] = len(c1["coords"]) + 7
     t = lambda a: torch.tensor(a, dtype=torch.int32, device="cuda")  # noqa: E731
     with pytest.raises(_capi.LdsfError):
-        _capi.Plan(ds._handle, len(c1["coords"]), t(c1["species"]), t(bad), t(c1["numneigh"]))
+        _capi.Plan(ds, len(c1["coords"]), t(c1["species"]), t(bad), t(c1["numneigh"]))
     sp = c1["species"].copy()
     sp[100] = 4  # species code beyond the parameter set
     with pytest.raises(_capi.LdsfError):
-        _capi.Plan(ds._handle, len(c1["coords"]), t(sp), t(c1["neighlist"]), t(c1["numneigh"]))
+        _capi.Plan(ds, len(c1["coords"]), t(sp), t(c1["neighlist"]), t(c1["numneigh"]))
 
 
 def test_c2_full_size_properties():
@@ -274,7 +274,7 @@ def test_wave_and_general_kernels_agree(precision, resolved):
     species = ("A", "B")
     ds = sf_desc(species, 5.0, resolved=resolved)
     t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device="cuda")  # noqa: E731
-    plan = _capi.Plan(ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
+    plan = _capi.Plan(ds, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
                       t(L["numneigh"]))
     assert plan.kernel_family == 1
     n = L["n_contrib"]
@@ -305,7 +305,7 @@ def test_adjoint_edges_then_gather_equals_gradient():
     L = small_two_species(300, seed=12)
     ds = sf_desc(("A", "B"), 5.0, resolved=True)
     t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device="cuda")  # noqa: E731
-    plan = _capi.Plan(ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
+    plan = _capi.Plan(ds, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
                       t(L["numneigh"]))
     n = L["n_contrib"]
     x = t(L["coords"], torch.float64)
@@ -382,7 +382,7 @@ def test_wave_kernels_on_partial_grids(grid, precision):
     kinds = GRIDS[grid]
     ds = sf_desc(species, 5.0, kinds=kinds, resolved=True)
     t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device="cuda")  # noqa: E731
-    plan = _capi.Plan(ds._handle, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
+    plan = _capi.Plan(ds, len(L["coords"]), t(L["species"]), t(L["neighlist"]),
                       t(L["numneigh"]))
     assert plan.kernel_family == 1
     n = L["n_contrib"]

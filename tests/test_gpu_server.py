@@ -50,7 +50,7 @@ def to_dev(L, width, seed):
 def alone(ds, prec, D):
     import torch
     from libdescriptor_amd import _capi
-    plan = _capi.Plan(ds._handle, D["x"].shape[0], D["species"], D["nl"], D["numneigh"])
+    plan = _capi.Plan(ds, D["x"].shape[0], D["species"], D["nl"], D["numneigh"])
     n = D["numneigh"].numel()
     d = torch.zeros((n, ds.width), dtype=torch.float64, device="cuda")
     g = torch.zeros_like(D["x"])
@@ -84,7 +84,7 @@ def test_mix_equals_alone(prec):
     ds = sf(prec)
     reqs = [to_dev(L, ds.width, 100 + k) for k, L in enumerate(make_requests(40, seed=prec))]
     # small lanes: several batches, several lanes busy at once
-    srv = _capi.Server(ds._handle, prec, hbm_budget_bytes=1 << 30, lane_centers=4000,
+    srv = _capi.Server(ds, prec, hbm_budget_bytes=1 << 30, lane_centers=4000,
                        lane_rows=16000, lane_edges=4000 * 40, max_requests=16, max_lanes=3)
     assert srv.num_lanes == 3
     tickets = [submit(srv, ds, D) for D in reqs]
@@ -115,7 +115,7 @@ def test_bad_request_is_rejected_alone():
     nl = bad["nl"].clone()
     nl[3] = bad["x"].shape[0] + 5  # one past the request's rows
     bad["nl"] = nl
-    srv = _capi.Server(ds._handle, 1, hbm_budget_bytes=1 << 30, lane_centers=4000,
+    srv = _capi.Server(ds, 1, hbm_budget_bytes=1 << 30, lane_centers=4000,
                        lane_rows=16000, lane_edges=4000 * 40, max_requests=16, max_lanes=1)
     tickets = [submit(srv, ds, D) for D in reqs]
     for k, t in enumerate(tickets):
@@ -147,7 +147,7 @@ def test_threads_submit_while_batches_run():
     from libdescriptor_amd import _capi
     ds = sf(1)
     reqs = [to_dev(L, ds.width, 300 + k) for k, L in enumerate(make_requests(24, seed=9))]
-    srv = _capi.Server(ds._handle, 1, hbm_budget_bytes=1 << 30, lane_centers=3000,
+    srv = _capi.Server(ds, 1, hbm_budget_bytes=1 << 30, lane_centers=3000,
                        lane_rows=12000, lane_edges=3000 * 40, max_requests=8, max_lanes=2)
     errors = []
 
@@ -178,8 +178,8 @@ def test_admission_limits():
     from libdescriptor_amd import workloads as w
     ds = sf(1)
     with pytest.raises(_capi.LdsfError):
-        _capi.Server(ds._handle, 1, hbm_budget_bytes=1 << 20)  # below one lane
-    srv = _capi.Server(ds._handle, 1, hbm_budget_bytes=1 << 30, lane_centers=100, lane_rows=400,
+        _capi.Server(ds, 1, hbm_budget_bytes=1 << 20)  # below one lane
+    srv = _capi.Server(ds, 1, hbm_budget_bytes=1 << 30, lane_centers=100, lane_rows=400,
                        lane_edges=4000, max_requests=4, max_lanes=1)
     pos, cell, pbc, sp = w.cluster(150, 0.05, 1.8, seed=4)
     D = to_dev(w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"]), ds.width, 1)
