@@ -537,10 +537,7 @@ struct Fast {
     double lp[2], lm[2], lam[2];  // the base's numerator is FP64
     float eta[2];                 // eta_e
     float etal[2];                // eta_e log2(e)
-    // rc, rc^2, (pi/2)/rc per species-pair block (two species), else block 0;
-    // separate scalars, so that a per-lane choice stays a register select
-    float rc0, rc1, rc2, rs0, rs1, rs2;
-    double rq0, rq1, rq2;
+    double rq0;  // rc^2 of species pair 0 (the pair enumeration's test for one cutoff)
 };
 
 // The zeta grid of the FP32-bulk kernels: ZG = 0 any grid (B^zeta =
@@ -581,12 +578,8 @@ __device__ __forceinline__ void squares_m1(const float (&sq)[5], float (&mk)[5])
     for (int k = 1; k < 5; ++k) mk[k] = k <= ZGrid<ZG>::nsq ? mk[k - 1] * sq[k - 1] : 0.0f;
 }
 
-// the value of block p (0, 1 or 2)
-template <typename V>
-__device__ __forceinline__ V blk3(int p, V v0, V v1, V v2)
-{
-    return p == 0 ? v0 : (p == 1 ? v1 : v2);
-}
+// the species-pair index of two-species block p (0-0, 0-1, 1-1: 0, 1, 3)
+__device__ __forceinline__ int blk_pair(int p) { return p + (p >> 1); }
 
 template <int NPB>
 __device__ __forceinline__ Fast load_fast(const Shared &S, const WaveParams &wp)
@@ -605,17 +598,7 @@ __device__ __forceinline__ Fast load_fast(const Shared &S, const WaveParams &wp)
         F.eta[e] = uniform_f(wp.eta3[e]);
         F.etal[e] = uniform_f(wp.eta3[e] * 1.4426950408889634);
     }
-    // blocks 0-0, 0-1, 1-1 of two species: pair indices 0, 1, 3
-    const int k1 = NPB > 1 ? 1 : 0, k2 = NPB > 1 ? 3 : 0;
-    F.rc0 = uniform_f(S.rc[0]);
-    F.rc1 = uniform_f(S.rc[k1]);
-    F.rc2 = uniform_f(S.rc[k2]);
     F.rq0 = uniform_d(S.rc2[0]);
-    F.rq1 = uniform_d(S.rc2[k1]);
-    F.rq2 = uniform_d(S.rc2[k2]);
-    F.rs0 = uniform_f(S.rcs[0]);
-    F.rs1 = uniform_f(S.rcs[k1]);
-    F.rs2 = uniform_f(S.rcs[k2]);
     return F;
 }
 
@@ -774,7 +757,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                 sa = __float_as_int(V[a].w);
                 sb = __float_as_int(V[b].w);
                 const double rc2 = (UNI || ns == 1) ? F.rq0
-                                   : (NPB > 1 ? blk3(sa + sb, F.rq0, F.rq1, F.rq2)
+                                   : (NPB > 1 ? S.rc2[blk_pair(sa + sb)]
                                               : S.rc2[pidx(sa, sb, ns)]);
                 act = rjk2 < rc2;
             }
@@ -1144,9 +1127,11 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 double v4[4];
                 if constexpr (P == 1) {
                     // this lane's species-pair cutoff
-                    const float rcl = blk3(pl, F.rc0, F.rc1, F.rc2);
-                    const double rc2l = blk3(pl, F.rq0, F.rq1, F.rq2);
-                    const float rcsl = blk3(pl, F.rs0, F.rs1, F.rs2);
+                    // the lane's block cutoff, from LDS (one read per centre)
+                    const int kp = blk_pair(pl);
+                    const float rcl = (float)S.rc[kp];
+                    const double rc2l = S.rc2[kp];
+                    const float rcsl = (float)S.rcs[kp];
                     const bool rc_pair = NPB == 1 && ns > 1;  // cutoff per species pair
                     const double2 *D = reinterpret_cast<const double2 *>(L.px);
                     const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
@@ -1520,9 +1505,10 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
             const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
             // this lane's species-pair cutoff (FP32-bulk)
-            const float rcl = P == 1 ? blk3(pl, F.rc0, F.rc1, F.rc2) : 0.0f;
-            const double rc2l = P == 1 ? blk3(pl, F.rq0, F.rq1, F.rq2) : 0.0;
-            const float rcsl = P == 1 ? blk3(pl, F.rs0, F.rs1, F.rs2) : 0.0f;
+            const int kp = blk_pair(pl);
+            const float rcl = P == 1 ? (float)S.rc[kp] : 0.0f;
+            const double rc2l = P == 1 ? S.rc2[kp] : 0.0;
+            const float rcsl = P == 1 ? (float)S.rcs[kp] : 0.0f;
             // FWD: the lane's row sums, FP64 in both forms
             double accd[16];
 #pragma unroll
