@@ -250,10 +250,10 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
     HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, in_start.p,
                                            count.p, in_edges.p, s), "fill");
-    HIP_TRY(ldsf::launch_sort_buckets(n_total, in_start.p, in_edges.p, s), "sort");
     DevBuf<int> inv_pos;
     HIP_TRY(inv_pos.alloc(plan->n_edges + n_centers), "hipMalloc(inv_pos)");
-    HIP_TRY(ldsf::launch_inverse(plan->n_edges + n_centers, in_edges.p, inv_pos.p, s), "inverse");
+    HIP_TRY(ldsf::launch_sort_inverse(n_total, in_start.p, in_edges.p, inv_pos.p, s),
+            "sort and inverse");
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     plan->offs = offs.release_ownership();
     plan->in_start = in_start.release_ownership();

@@ -133,6 +133,49 @@ __device__ inline void sort_bucket_wave(int *ids, long b, int m, int lane)
     }
 }
 
+// One wavefront's share of the transposed list's last step: rows w, w + nw,
+// ... of [0, n_rows) (w: this wave's index, nw: the waves of the grid), two
+// rows at a time.  Each row's ids[in_start[a] .. in_start[a+1]) are sorted
+// ascending and inv[id] = position written for every id.  Two rows of at
+// most 32 ids each take a 32-lane bitonic network in each half-wave; longer
+// ones take sort_bucket_wave one row after the other.
+__device__ inline void sort_rows_inverse(int n_rows, const int *in_start, int *ids, int *inv,
+                                         long w, long nw, int lane)
+{
+    const int half = lane >> 5, hl = lane & 31;
+    for (long a0 = 2 * w; a0 < n_rows; a0 += 2 * nw) {
+        const long a = a0 + half;
+        const int b = a < n_rows ? in_start[a] : 0;
+        const int m = a < n_rows ? in_start[a + 1] - b : 0;
+        const int mmax = max(m, __shfl_xor(m, 32));  // the same in both halves
+        if (mmax <= 32) {
+            int v = hl < m ? ids[b + hl] : 0x7fffffff;
+#pragma unroll
+            for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const int u = __shfl_xor(v, j);
+                    const bool up = (hl & k) == 0, lower = (hl & j) == 0;
+                    v = (lower == up) ? min(v, u) : max(v, u);
+                }
+            }
+            if (hl < m) {
+                ids[b + hl] = v;
+                inv[v] = b + hl;
+            }
+        } else {
+            for (int r = 0; r < 2; ++r) {
+                const int br = __shfl(b, 32 * r), mr = __shfl(m, 32 * r);
+                sort_bucket_wave(ids, br, mr, lane);
+                // the sorted ids, re-read after the sort's own stores (m > 64
+                // is sorted by lane 0 alone)
+                __threadfence_block();
+                for (int k = lane; k < mr; k += 64) inv[ids[br + k]] = br + k;
+            }
+        }
+    }
+}
+
 // ---- cutoff function fc(r) = 0.5 (cos(pi r / rc) + 1) for r < rc, and dfc/dr ----
 
 __device__ __forceinline__ void fcut(double r, double rc, double &fc, double &dfc, Real<0>)

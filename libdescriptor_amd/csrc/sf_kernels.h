@@ -109,8 +109,9 @@ hipError_t launch_edge_targets_count(long n_edges, int n_centers, const int *nl,
                                      int n_total, int *count, int *flags, hipStream_t s);
 hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, const int *centers,
                                     const int *in_start, int *cursor, int *in_edges, hipStream_t s);
-hipError_t launch_sort_buckets(int n_total, const int *in_start, int *in_edges, hipStream_t s);
-hipError_t launch_inverse(long n, const int *in_edges, int *inv_pos, hipStream_t s);
+// sorts every row's bucket of in_edges and writes inv_pos[edge id] = its position
+hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, int *inv_pos,
+                               hipStream_t s);
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
                          hipStream_t s);
 hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s);

@@ -78,18 +78,13 @@ __global__ void k_fill(long n_edges, int n_centers, const int *__restrict__ nl,
 }
 
 // each row's bucket (a few tens of ids) sorted by one wavefront
-__global__ void k_sort(int n_total, const int *__restrict__ in_start, int *in_edges)
+// every row's bucket sorted, and inv_pos[edge id] = its position
+__global__ void k_sort_inverse(int n_total, const int *__restrict__ in_start, int *in_edges,
+                               int *inv_pos)
 {
-    const int lane = threadIdx.x & 63;
     const long nw = ((long)gridDim.x * blockDim.x) >> 6;
-    for (long a = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; a < n_total; a += nw)
-        sort_bucket_wave(in_edges, in_start[a], in_start[a + 1] - in_start[a], lane);
-}
-
-__global__ void k_inverse(long n, const int *__restrict__ in_edges, int *__restrict__ inv_pos)
-{
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) inv_pos[in_edges[p]] = (int)p;
+    sort_rows_inverse(n_total, in_start, in_edges, inv_pos,
+                      ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw, threadIdx.x & 63);
 }
 
 // records of row a are contiguous: [in_start[a], in_start[a+1]), in ascending
@@ -150,19 +145,14 @@ hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, 
     return hipGetLastError();
 }
 
-hipError_t launch_sort_buckets(int n_total, const int *in_start, int *in_edges, hipStream_t s)
+hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, int *inv_pos,
+                               hipStream_t s)
 {
     if (n_total <= 0) return hipSuccess;
-    const long blocks = std::min<long>(((long)n_total * 64 + kBlock - 1) / kBlock, 8192);
-    hipLaunchKernelGGL(k_sort, dim3((unsigned)blocks), dim3(kBlock), 0, s, n_total, in_start,
-                       in_edges);
-    return hipGetLastError();
-}
-
-hipError_t launch_inverse(long n, const int *in_edges, int *inv_pos, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_inverse, grid_for(n), dim3(kBlock), 0, s, n, in_edges, inv_pos);
+    // a wave per two rows, at most 8192 blocks of 4 waves
+    const long blocks = std::min<long>(((long)n_total * 32 + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_sort_inverse, dim3((unsigned)blocks), dim3(kBlock), 0, s, n_total,
+                       in_start, in_edges, inv_pos);
     return hipGetLastError();
 }
 

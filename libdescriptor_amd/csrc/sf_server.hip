@@ -272,20 +272,12 @@ __global__ void k_fill_t(const int *__restrict__ hdr, const int *__restrict__ nl
     }
 }
 
-__global__ void k_sort_t(const int *__restrict__ hdr, const int *__restrict__ in_start,
-                         int *in_edges)
+// every row's bucket sorted, and inv_pos[edge id] = its position
+__global__ void k_sort_inverse_t(const int *__restrict__ hdr, const int *__restrict__ in_start,
+                                 int *in_edges, int *inv_pos)
 {
-    const int n = hdr[H_ROWS];
-    const int lane = threadIdx.x & 63;
-    for (long a = gtid() >> 6; a < n; a += gstride() >> 6)
-        sort_bucket_wave(in_edges, in_start[a], in_start[a + 1] - in_start[a], lane);
-}
-
-__global__ void k_inverse_t(const int *__restrict__ hdr, const int *__restrict__ in_edges,
-                            int *inv_pos)
-{
-    const long n = (long)hdr[H_EDGES] + hdr[H_CENTERS];
-    for (long p = gtid(); p < n; p += gstride()) inv_pos[in_edges[p]] = (int)p;
+    sort_rows_inverse(hdr[H_ROWS], in_start, in_edges, inv_pos, gtid() >> 6, gstride() >> 6,
+                      threadIdx.x & 63);
 }
 
 // per-row sums of the records, written (=) into the owning request's output
@@ -491,8 +483,7 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
                        L.part, L.in_start);
     hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.cursor,
                        L.in_edges);
-    hipLaunchKernelGGL(k_sort_t, gs, tb, 0, s, hdr, L.in_start, L.in_edges);
-    hipLaunchKernelGGL(k_inverse_t, gs, tb, 0, s, hdr, L.in_edges, L.inv_pos);
+    hipLaunchKernelGGL(k_sort_inverse_t, gs, tb, 0, s, hdr, L.in_start, L.in_edges, L.inv_pos);
     if ((e = hipGetLastError())) return e;
     DevCounts dc;
     dc.hdr = hdr;
