@@ -186,3 +186,13 @@ def test_fused_full_size_262144():
     d32, G32 = S.fused(1)
     assert signal_rel(d32, d64, BLOCKS) <= TOL32
     assert rel_err(G32, G64) <= TOL32
+    # size-independent properties at full size, both forms: translation
+    # invariance (every row, padding included: sum of dE/dr = 0), and
+    # linearity in dE/dzeta, exact for a power-of-two scale
+    for prec, G in ((0, Gf), (1, G32)):
+        assert np.abs(G.sum(axis=0)).max() <= 1e-12 * np.abs(G).sum(axis=0).max()
+        g1 = S.g
+        S.g = 2.0 * g1
+        _, G2 = S.fused(prec)
+        S.g = g1
+        np.testing.assert_array_equal(G2, 2.0 * G)
