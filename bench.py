@@ -231,6 +231,10 @@ def main():
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
     dom_kernel = "k_adj_wave" if plan.kernel_family == 1 else "k_fwd_general+k_adj_general"
     traffic = measured_traffic(dom_kernel, n) if args.n == 262144 else None
+    # the same kernel in the FP64 form, against the FP64 vector peak
+    dom64_ms = res["fp64"]["fused_ms"]
+    achieved64 = dom_flop / (dom64_ms * 1e-3) / 1e12
+    traffic64 = measured_traffic(dom_kernel, n, "_fp64") if args.n == 262144 else None
 
     batched = serving = layout = None
     if not args.no_extra:
@@ -277,6 +281,10 @@ def main():
                      "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
                      "traffic": traffic,
                      "hbm_GBps": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None},
+        "roofline_fp64": {"bound": "valu", "kernel": dom_kernel + " (fused, FP64 form)",
+                          "achieved": achieved64, "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                          "frac": achieved64 / PEAK["fp64"], "traffic": traffic64,
+                          "hbm_GBps": (traffic64 / (dom64_ms * 1e-3) / 1e9) if traffic64 else None},
         "cpu_baseline": cpu,
         "layout": layout,
         "batched": batched,
@@ -475,11 +483,12 @@ def bench_serving(args):
     return out
 
 
-def measured_traffic(kernel, n):
+def measured_traffic(kernel, n, form=""):
     """HBM bytes per launch of `kernel` on this workload, from the PMC summary
     kept beside the kernel traces (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration recorded in the same file); None if absent."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_c3_{n}_pmc.txt")
+    copy-kernel calibration recorded in the same file; form "_fp64" for the
+    FP64 instance); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}{form}_c3_{n}_pmc.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
