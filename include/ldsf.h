@@ -102,6 +102,19 @@ LDSF_API int ldsf_compute_batch(ldsf_sf *sf, int n_configurations, const int *n_
                                 const int *neighbor_list, const int *number_of_neighs,
                                 const double *coordinates, double *desc);
 
+/* ldsf_compute / ldsf_compute_batch with desc overwritten (=) instead of
+ * accumulated (+=), so desc's old contents are not uploaded: for callers whose
+ * desc starts at zero, such as the Python module, which returns a new zeroed
+ * array (the reference's binding zero-fills it, python_bindings.cpp:146-147).
+ * No reference counterpart. */
+LDSF_API int ldsf_compute_overwrite(ldsf_sf *sf, int n_atoms, const int *species,
+                                    const int *neighbor_list, const int *number_of_neighs,
+                                    const double *coordinates, double *desc);
+LDSF_API int ldsf_compute_batch_overwrite(ldsf_sf *sf, int n_configurations, const int *n_atoms,
+                                          const int *config_ptr, const int *species,
+                                          const int *neighbor_list, const int *number_of_neighs,
+                                          const double *coordinates, double *desc);
+
 /* Descriptor::gradient (Descriptors.hpp:58-60, Descriptors.cpp:165-190).
  * d_coordinates += J^T dE_dzeta, and desc += the descriptors: the reference's
  * Enzyme call runs the forward too (enzyme_dup desc), so the same holds here,

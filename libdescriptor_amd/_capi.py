@@ -34,6 +34,8 @@ _PROTOS = {
     "ldsf_compute": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_compute_single_atom": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp]),
     "ldsf_compute_batch": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_compute_overwrite": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "ldsf_compute_batch_overwrite": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_gradient_single_atom": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "ldsf_gradient_batch": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

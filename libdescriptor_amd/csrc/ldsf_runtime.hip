@@ -409,8 +409,10 @@ int host_prepare(HostCall &c)
     return LDSF_OK;
 }
 
-// desc_io: [n_centers][width] (+=) or NULL; dE + dcoords_io for the adjoint
-int host_run(HostCall &c, double *desc_io, const double *dE, double *dcoords_io, int precision)
+// desc_io: [n_centers][width] (+=, or = with assign_desc) or NULL; dE +
+// dcoords_io for the adjoint
+int host_run(HostCall &c, double *desc_io, const double *dE, double *dcoords_io, int precision,
+             bool assign_desc = false)
 {
     if (int rc = host_prepare(c)) return rc;
     if (c.n_centers == 0) return LDSF_OK;
@@ -439,7 +441,10 @@ int host_run(HostCall &c, double *desc_io, const double *dE, double *dcoords_io,
     std::unique_ptr<ldsf_plan> guard(plan);
     if (desc_io) {
         HIP_TRY(d_desc.alloc((size_t)c.n_centers * W), "hipMalloc");
-        HIP_TRY(hipMemcpy(d_desc.p, desc_io, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
+        if (assign_desc)  // starts from zero on the device: no upload
+            HIP_TRY(hipMemsetAsync(d_desc.p, 0, sizeof(double) * c.n_centers * W, s), "hipMemsetAsync");
+        else
+            HIP_TRY(hipMemcpy(d_desc.p, desc_io, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
     }
     if (dE) {
         HIP_TRY(d_dE.alloc((size_t)c.n_centers * W), "hipMalloc");
@@ -674,6 +679,15 @@ LDSF_API int ldsf_compute(ldsf_sf *sf, int n_atoms, const int *species, const in
     return host_run(c, desc, nullptr, nullptr, sf->precision);
 }
 
+LDSF_API int ldsf_compute_overwrite(ldsf_sf *sf, int n_atoms, const int *species,
+                                    const int *neighbor_list, const int *number_of_neighs,
+                                    const double *coordinates, double *desc)
+{
+    if (!sf || (n_atoms > 0 && !desc)) return ldsf::fail(LDSF_ERR_ARG, "compute: bad arguments");
+    HostCall c{sf, n_atoms, {}, species, neighbor_list, number_of_neighs, coordinates};
+    return host_run(c, desc, nullptr, nullptr, sf->precision, true);
+}
+
 LDSF_API int ldsf_compute_single_atom(ldsf_sf *sf, int index, int n_atoms, const int *species,
                                       const int *neighbor_list, int number_of_neighs,
                                       const double *coordinates, double *desc)
@@ -697,6 +711,21 @@ LDSF_API int ldsf_compute_batch(ldsf_sf *sf, int n_configurations, const int *n_
     if (n > 0 && !desc) return ldsf::fail(LDSF_ERR_ARG, "compute_batch: null desc");
     HostCall c{sf, n, std::move(ctr), species, neighbor_list, number_of_neighs, coordinates};
     return host_run(c, desc, nullptr, nullptr, sf->precision);
+}
+
+LDSF_API int ldsf_compute_batch_overwrite(ldsf_sf *sf, int n_configurations, const int *n_atoms,
+                                          const int *config_ptr, const int *species,
+                                          const int *neighbor_list, const int *number_of_neighs,
+                                          const double *coordinates, double *desc)
+{
+    if (!sf || n_configurations < 0 || (n_configurations > 0 && (!n_atoms || !config_ptr)))
+        return ldsf::fail(LDSF_ERR_ARG, "compute_batch: bad arguments");
+    int n = 0, rc = 0;
+    std::vector<int> ctr = batch_centers(n_configurations, n_atoms, config_ptr, &n, &rc);
+    if (rc) return rc;
+    if (n > 0 && !desc) return ldsf::fail(LDSF_ERR_ARG, "compute_batch: null desc");
+    HostCall c{sf, n, std::move(ctr), species, neighbor_list, number_of_neighs, coordinates};
+    return host_run(c, desc, nullptr, nullptr, sf->precision, true);
 }
 
 LDSF_API int ldsf_gradient(ldsf_sf *sf, int n_atoms, const int *species, const int *neighbor_list,
