@@ -373,7 +373,7 @@ PYBIND11_MODULE(libdescriptor, m)
                                          ip(number_of_neighbors), dp(coordinates), dp(dE_ddesc),
                                          dpar.mutable_data(), drc.mutable_data());
             });
-            if (rc) throw std::runtime_error(ldsf_last_error());
+            raise_status(rc);
             return py::make_tuple(dpar, drc);
         },
         "dE/d(parameters) (initDescriptor's flat layout) and dE/d(cutoff matrix): what the "
