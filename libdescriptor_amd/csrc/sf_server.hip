@@ -19,8 +19,9 @@
 //   table H2D -> prepare (copy and validate every request's inputs into the
 //   lane: rows, centres, neighbour entries offset to batch rows; per centre
 //   the addresses of its descriptor and dE/dzeta rows in the request's own
-//   buffers) -> neighbour sums check -> CSR offsets per request -> transposed
-//   CSR (count, scan, fill, per-row sort, inverse) -> the fused forward +
+//   buffers; the transposed list's bucket counts) -> neighbour sums check ->
+//   CSR offsets per request -> transposed CSR (scan, fill, per-row sort with
+//   the inverse) -> the fused forward +
 //   adjoint kernel (descriptors straight into each request's desc; the
 //   general kernels of the largest bucket run as forward, then adjoint) ->
 //   gather straight into each request's d_coordinates -> flags D2H.
@@ -86,10 +87,13 @@ __global__ void k_zero_int(int *p, long n)
 
 // flags: 1 index/species/count out of range, 2 neighbour counts do not sum
 // to n_neigh
+// count[row] += 1 for every edge and centre self-term pointing at the row (the
+// transposed list's bucket sizes; count starts at zero)
 __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ seg, int width,
                           int n_species, int *species, int *numneigh, int *nl, int *centres,
                           int *req_of, int *row_req, double *x, double **out_row,
-                          const double **in_row, const double *zero_row, int *req_sum, int *flags)
+                          const double **in_row, const double *zero_row, int *req_sum, int *flags,
+                          int *count)
 {
     const int r = blockIdx.y;
     if (r >= hdr[H_REQ]) return;
@@ -120,6 +124,7 @@ __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ s
         const long c = S.centre_base + i;
         numneigh[c] = nn;
         centres[c] = S.row_base + (int)i;
+        atomicAdd(count + S.row_base + i, 1);
         req_of[c] = r;
         out_row[c] = S.desc + i * width;
         in_row[c] = S.dE ? S.dE + i * width : zero_row;
@@ -132,6 +137,7 @@ __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ s
             v = 0;
         }
         nl[S.edge_base + e] = S.row_base + v;
+        atomicAdd(count + S.row_base + v, 1);
     }
     // one atomic per wave for the neighbour-count sum
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
@@ -181,31 +187,43 @@ __device__ int block_scan(int v, int *total)
     return v + before;
 }
 
+// Block sums of the chunks into part[0..nb); the block that finishes last
+// (a ticket counter, which it puts back to zero) then scans them: part[b] =
+// exclusive prefix, part[nb] = total.  The partials are read with agent-scope
+// atomic loads, past this CU's cache.
 __global__ void k_scan_partial(const int *__restrict__ hdr, int which, const int *__restrict__ in,
-                               int *part)
+                               int *part, unsigned *ticket)
 {
     const long n = hdr[which];
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int nb = gridDim.x;
+    const long chunk = (n + nb - 1) / nb;
     const long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     int s = 0;
     for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += in[i];
     s = block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-__global__ void k_scan_top(int *part, int nb)
-{
-    // one block: exclusive scan of part[0..nb), total into part[nb]
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == (unsigned)nb - 1u;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
     int run = 0;
     for (int base = 0; base < nb; base += blockDim.x) {
         const int i = base + threadIdx.x;
-        const int v = i < nb ? part[i] : 0;
+        const int v = i < nb ? __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : 0;
         int tot;
         const int inc = block_scan(v, &tot);
         if (i < nb) part[i] = run + inc - v;
         run += tot;
     }
-    if (threadIdx.x == 0) part[nb] = run;
+    if (threadIdx.x == 0) {
+        part[nb] = run;
+        *ticket = 0u;
+    }
 }
 
 __global__ void k_scan_final(const int *__restrict__ hdr, int which, const int *__restrict__ in,
@@ -247,16 +265,6 @@ __global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *cursor
     for (long i = gtid(); i <= n; i += gstride()) {
         count[i] = 0;
         if (i < n) cursor[i] = 0;
-    }
-}
-
-__global__ void k_count_t(const int *__restrict__ hdr, const int *__restrict__ nl,
-                          const int *__restrict__ centres, int *count)
-{
-    const long ne = hdr[H_EDGES], n = ne + hdr[H_CENTERS];
-    for (long e = gtid(); e < n; e += gstride()) {
-        const int a = e < ne ? nl[e] : centres[e - ne];
-        atomicAdd(count + a, 1);
     }
 }
 
@@ -325,6 +333,7 @@ struct ldsf_lane {
     int *req_of = nullptr, *row_req = nullptr, *req_sum = nullptr, *flags = nullptr;
     int *scan = nullptr, *offs = nullptr, *count = nullptr, *in_start = nullptr;
     int *cursor = nullptr, *in_edges = nullptr, *inv_pos = nullptr, *part = nullptr;
+    unsigned *ticket = nullptr;  // the scans' last-block counter, back to zero after each scan
     double *x = nullptr, *edge = nullptr, *zero_row = nullptr;
     double **out_row = nullptr;
     const double **in_row = nullptr;
@@ -389,7 +398,7 @@ size_t lane_bytes(const ldsf_server_config &c, int width)
     b += sizeof(int) * (R /*species*/ + C /*numneigh*/ + E /*nl*/ + C /*centres*/ + C /*req_of*/ +
                         R /*row_req*/ + 2 * Q /*sums, flags*/ + (C + 1) * 2 /*scan, offs*/ +
                         (R + 1) * 2 /*count, in_start*/ + R /*cursor*/ + 2 * (E + C) +
-                        (ldsf::kScanBlocks + 1));
+                        (ldsf::kScanBlocks + 1) + 1 /*ticket*/);
     b += sizeof(double) * (3 * R + 2 * C + (size_t)width + 3 * (E + C));
     b += ldsf::H_INTS * sizeof(int) + Q * sizeof(Seg);
     return b;
@@ -418,6 +427,8 @@ int lane_alloc(ldsf_server *S, ldsf_lane &L)
     SRV_TRY(dalloc(&L.in_edges, E + C), "hipMalloc");
     SRV_TRY(dalloc(&L.inv_pos, E + C), "hipMalloc");
     SRV_TRY(dalloc(&L.part, ldsf::kScanBlocks + 1), "hipMalloc");
+    SRV_TRY(dalloc(&L.ticket, 1), "hipMalloc");
+    SRV_TRY(hipMemset(L.ticket, 0, sizeof(unsigned)), "hipMemset");
     SRV_TRY(dalloc(&L.x, 3 * R), "hipMalloc");
     SRV_TRY(dalloc(&L.out_row, C), "hipMalloc");
     SRV_TRY(dalloc(&L.in_row, C), "hipMalloc");
@@ -437,7 +448,8 @@ void lane_free(ldsf_lane &L)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {L.d_table, L.species, L.numneigh, L.nl, L.centres, L.req_of, L.row_req,
                    L.req_sum, L.flags, L.scan, L.offs, L.count, L.in_start, L.cursor,
-                   L.in_edges, L.inv_pos, L.part, L.x, L.edge, L.zero_row, L.out_row,
+                   L.in_edges, L.inv_pos, L.part, (void *)L.ticket, L.x, L.edge, L.zero_row,
+                   L.out_row,
                    (void *)L.in_row};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -462,23 +474,21 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
         return e;
     hipLaunchKernelGGL(k_zero_int, dim3(8), tb, 0, s, L.req_sum, (long)Q);
     hipLaunchKernelGGL(k_zero_int, dim3(8), tb, 0, s, L.flags, (long)Q);
+    // the transposed list's counts start at zero; k_prepare counts as it copies
+    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.cursor);
     hipLaunchKernelGGL(k_prepare, gseg, tb, 0, s, hdr, seg, S->width, S->n_species, L.species,
                        L.numneigh, L.nl, L.centres, L.req_of, L.row_req, L.x, L.out_row,
-                       L.in_row, L.zero_row, L.req_sum, L.flags);
+                       L.in_row, L.zero_row, L.req_sum, L.flags, L.count);
     hipLaunchKernelGGL(k_check_sums, dim3(1), tb, 0, s, hdr, seg, L.req_sum, L.flags);
     // CSR offsets: scan of the neighbour counts, rebased per request
     hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_CENTERS,
-                       L.numneigh, L.part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, L.part, kScanBlocks);
+                       L.numneigh, L.part, L.ticket);
     hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_CENTERS, L.numneigh,
                        L.part, L.scan);
     hipLaunchKernelGGL(k_offsets, gs, tb, 0, s, hdr, seg, L.req_of, L.scan, L.offs);
-    // transposed CSR
-    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.cursor);
-    hipLaunchKernelGGL(k_count_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.count);
+    // transposed CSR: scan of the counts, fill, per-row sort with the inverse
     hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_ROWS, L.count,
-                       L.part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, L.part, kScanBlocks);
+                       L.part, L.ticket);
     hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_ROWS, L.count,
                        L.part, L.in_start);
     hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.cursor,
