@@ -187,43 +187,31 @@ __device__ int block_scan(int v, int *total)
     return v + before;
 }
 
-// Block sums of the chunks into part[0..nb); the block that finishes last
-// (a ticket counter, which it puts back to zero) then scans them: part[b] =
-// exclusive prefix, part[nb] = total.  The partials are read with agent-scope
-// atomic loads, past this CU's cache.
 __global__ void k_scan_partial(const int *__restrict__ hdr, int which, const int *__restrict__ in,
-                               int *part, unsigned *ticket)
+                               int *part)
 {
     const long n = hdr[which];
-    const int nb = gridDim.x;
-    const long chunk = (n + nb - 1) / nb;
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
     const long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     int s = 0;
     for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += in[i];
     s = block_sum(s);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = s;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == (unsigned)nb - 1u;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_scan_top(int *part, int nb)
+{
+    // one block: exclusive scan of part[0..nb), total into part[nb]
     int run = 0;
     for (int base = 0; base < nb; base += blockDim.x) {
         const int i = base + threadIdx.x;
-        const int v = i < nb ? __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                             : 0;
+        const int v = i < nb ? part[i] : 0;
         int tot;
         const int inc = block_scan(v, &tot);
         if (i < nb) part[i] = run + inc - v;
         run += tot;
     }
-    if (threadIdx.x == 0) {
-        part[nb] = run;
-        *ticket = 0u;
-    }
+    if (threadIdx.x == 0) part[nb] = run;
 }
 
 __global__ void k_scan_final(const int *__restrict__ hdr, int which, const int *__restrict__ in,
@@ -333,7 +321,6 @@ struct ldsf_lane {
     int *req_of = nullptr, *row_req = nullptr, *req_sum = nullptr, *flags = nullptr;
     int *scan = nullptr, *offs = nullptr, *count = nullptr, *in_start = nullptr;
     int *cursor = nullptr, *in_edges = nullptr, *inv_pos = nullptr, *part = nullptr;
-    unsigned *ticket = nullptr;  // the scans' last-block counter, back to zero after each scan
     double *x = nullptr, *edge = nullptr, *zero_row = nullptr;
     double **out_row = nullptr;
     const double **in_row = nullptr;
@@ -398,7 +385,7 @@ size_t lane_bytes(const ldsf_server_config &c, int width)
     b += sizeof(int) * (R /*species*/ + C /*numneigh*/ + E /*nl*/ + C /*centres*/ + C /*req_of*/ +
                         R /*row_req*/ + 2 * Q /*sums, flags*/ + (C + 1) * 2 /*scan, offs*/ +
                         (R + 1) * 2 /*count, in_start*/ + R /*cursor*/ + 2 * (E + C) +
-                        (ldsf::kScanBlocks + 1) + 1 /*ticket*/);
+                        (ldsf::kScanBlocks + 1));
     b += sizeof(double) * (3 * R + 2 * C + (size_t)width + 3 * (E + C));
     b += ldsf::H_INTS * sizeof(int) + Q * sizeof(Seg);
     return b;
@@ -427,8 +414,6 @@ int lane_alloc(ldsf_server *S, ldsf_lane &L)
     SRV_TRY(dalloc(&L.in_edges, E + C), "hipMalloc");
     SRV_TRY(dalloc(&L.inv_pos, E + C), "hipMalloc");
     SRV_TRY(dalloc(&L.part, ldsf::kScanBlocks + 1), "hipMalloc");
-    SRV_TRY(dalloc(&L.ticket, 1), "hipMalloc");
-    SRV_TRY(hipMemset(L.ticket, 0, sizeof(unsigned)), "hipMemset");
     SRV_TRY(dalloc(&L.x, 3 * R), "hipMalloc");
     SRV_TRY(dalloc(&L.out_row, C), "hipMalloc");
     SRV_TRY(dalloc(&L.in_row, C), "hipMalloc");
@@ -448,8 +433,7 @@ void lane_free(ldsf_lane &L)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {L.d_table, L.species, L.numneigh, L.nl, L.centres, L.req_of, L.row_req,
                    L.req_sum, L.flags, L.scan, L.offs, L.count, L.in_start, L.cursor,
-                   L.in_edges, L.inv_pos, L.part, (void *)L.ticket, L.x, L.edge, L.zero_row,
-                   L.out_row,
+                   L.in_edges, L.inv_pos, L.part, L.x, L.edge, L.zero_row, L.out_row,
                    (void *)L.in_row};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -482,13 +466,15 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
     hipLaunchKernelGGL(k_check_sums, dim3(1), tb, 0, s, hdr, seg, L.req_sum, L.flags);
     // CSR offsets: scan of the neighbour counts, rebased per request
     hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_CENTERS,
-                       L.numneigh, L.part, L.ticket);
+                       L.numneigh, L.part);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, L.part, kScanBlocks);
     hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_CENTERS, L.numneigh,
                        L.part, L.scan);
     hipLaunchKernelGGL(k_offsets, gs, tb, 0, s, hdr, seg, L.req_of, L.scan, L.offs);
     // transposed CSR: scan of the counts, fill, per-row sort with the inverse
     hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_ROWS, L.count,
-                       L.part, L.ticket);
+                       L.part);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, L.part, kScanBlocks);
     hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_ROWS, L.count,
                        L.part, L.in_start);
     hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.cursor,
