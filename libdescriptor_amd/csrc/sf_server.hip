@@ -80,11 +80,6 @@ constexpr int kWaveBlocks = 4096;  // persistent grid of the wave kernels (~16 w
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ long gstride() { return (long)gridDim.x * blockDim.x; }
 
-__global__ void k_zero_int(int *p, long n)
-{
-    for (long i = gtid(); i < n; i += gstride()) p[i] = 0;
-}
-
 // flags: 1 index/species/count out of range, 2 neighbour counts do not sum
 // to n_neigh
 // count[row] += 1 for every edge and centre self-term pointing at the row (the
@@ -246,13 +241,24 @@ __global__ void k_offsets(const int *__restrict__ hdr, const Seg *__restrict__ s
 
 // ---- transposed CSR: the edges (and centre self-terms) pointing at each row ----
 
-// count[0..n_rows] and cursor[0..n_rows) to zero
-__global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *cursor)
+// count[0..n_rows] and cursor[0..n_rows) to zero,
+// with the per-request sums and flags (Q entries each): every counter a
+// batch starts from zero, in one launch
+__global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *cursor, int *req_sum,
+                            int *flags, int Q)
 {
     const int n = hdr[H_ROWS];
     for (long i = gtid(); i <= n; i += gstride()) {
         count[i] = 0;
         if (i < n) cursor[i] = 0;
+        if (i < Q) {
+            req_sum[i] = 0;
+            flags[i] = 0;
+        }
+    }
+    for (long i = gtid() + n + 1; i < Q; i += gstride()) {  // Q > n + 1: the rest
+        req_sum[i] = 0;
+        flags[i] = 0;
     }
 }
 
@@ -456,10 +462,9 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
     hipError_t e;
     if ((e = hipMemcpyAsync(L.d_table, L.h_table, S->table_bytes, hipMemcpyHostToDevice, s)))
         return e;
-    hipLaunchKernelGGL(k_zero_int, dim3(8), tb, 0, s, L.req_sum, (long)Q);
-    hipLaunchKernelGGL(k_zero_int, dim3(8), tb, 0, s, L.flags, (long)Q);
-    // the transposed list's counts start at zero; k_prepare counts as it copies
-    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.cursor);
+    // the transposed list's counts start at zero (k_prepare counts as it
+    // copies), and so do the per-request sums and flags
+    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.cursor, L.req_sum, L.flags, Q);
     hipLaunchKernelGGL(k_prepare, gseg, tb, 0, s, hdr, seg, S->width, S->n_species, L.species,
                        L.numneigh, L.nl, L.centres, L.req_of, L.row_req, L.x, L.out_row,
                        L.in_row, L.zero_row, L.req_sum, L.flags, L.count);
