@@ -691,6 +691,9 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
             gq[2 * k + 1] = v.y;
         }
     }
+    // two species: the term goes to the neighbour's block as v * 1, the other
+    // block gets v * 0 = +0 (v >= 0), both exact
+    const double m1 = NSB == 1 ? 1.0 : (double)sj, m0 = 1.0 - m1;
     float sum = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -698,12 +701,12 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
         const float ex = __builtin_amdgcn_exp2f(-wp.e2l[q] * (dr * dr));
         if (ADJ) sum = fmaf(gq[q], ex * fmaf(wp.m2e[q] * dr, fc, dfc), sum);
         if (FWD) {
-            const float v = ex * fc;
+            const double v = (double)(ex * fc);
             if (NSB == 1) {
-                acc[q] = (double)v;
+                acc[q] = v;
             } else {
-                acc[q] = (double)(sj ? 0.0f : v);
-                acc[8 + q] = (double)(sj ? v : 0.0f);
+                acc[q] = v * m0;
+                acc[8 + q] = v * m1;
             }
         }
     }
