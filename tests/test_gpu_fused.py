@@ -174,6 +174,37 @@ def test_host_gradient_readds_forward(c1):
     assert rel_err(dx, G0) <= TOL64
 
 
+def test_host_compute_overwrite_and_accumulate(c1):
+    """ldsf_compute adds into desc (the reference's +=); ldsf_compute_overwrite
+    (and the batch form) overwrites it, whatever it held; the Python module's
+    compute, which uses the overwrite form, equals both from zero."""
+    from libdescriptor_amd import _capi
+    print("case: C1 host compute += and overwrite", flush=True)
+    _dev()
+    ds = ld.DescriptorKind.init_descriptor(ld.AvailableDescriptors.SymmetryFunctions, ["Si"], "cos",
+                                           4.5, *behler())
+    sp = np.ascontiguousarray(c1["species"], np.int32)
+    nl = np.ascontiguousarray(c1["neighlist"], np.int32)
+    nn = np.ascontiguousarray(c1["numneigh"], np.int32)
+    x = np.ascontiguousarray(c1["coords"])
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    h = ctypes.c_void_p(ds._handle)
+    base = np.random.default_rng(3).normal(size=(64, 24))
+    acc = base.copy()
+    _capi.check(_capi.lib().ldsf_compute(h, 64, p(sp), p(nl), p(nn), p(x), p(acc)))
+    over = base.copy()
+    _capi.check(_capi.lib().ldsf_compute_overwrite(h, 64, p(sp), p(nl), p(nn), p(x), p(over)))
+    bover = base.copy()
+    na, cp = np.array([64], np.int32), np.array([0], np.int32)
+    _capi.check(_capi.lib().ldsf_compute_batch_overwrite(h, 1, p(na), p(cp), p(sp), p(nl), p(nn),
+                                                         p(x), p(bover)))
+    mod = ld.compute(ds, 64, sp, nl, nn, x)
+    assert elem_rel_err(over, c1["desc"], 1e-300) <= TOL64
+    np.testing.assert_array_equal(bover, over)
+    np.testing.assert_array_equal(mod, over)
+    assert elem_rel_err(acc - base, over, 1e-300) <= 1e-11
+
+
 def test_fused_full_size_262144():
     """The benchmark's system: fused against the separate kernels, both forms."""
     from libdescriptor_amd import workloads as w
