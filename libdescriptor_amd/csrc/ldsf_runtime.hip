@@ -240,21 +240,22 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     HIP_TRY(hipMemsetAsync(scratch_i.p, 0, sizeof(int), s), "hipMemsetAsync");
     HIP_TRY(ldsf::launch_edge_targets_count(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
                                             count.p, scratch_i.p, s), "count");
-    HIP_TRY(hipMemcpyAsync(host_flags, scratch_i.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flags");
-    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
-    if (host_flags[0] & 1) return ldsf::fail(LDSF_ERR_ARG, "neighbour index out of range [0, n_total)");
+    // out-of-range targets are skipped by the fill as by the count, and the
+    // flag is read once the list is built
     DevBuf<int> in_start, in_edges;
     HIP_TRY(in_start.alloc(n_total + 1), "hipMalloc(in_start)");
     HIP_TRY(in_edges.alloc(plan->n_edges + n_centers), "hipMalloc(in_edges)");
     HIP_TRY(ldsf::launch_scan_int(n_total, count.p, in_start.p, tmp.p, &tmp_bytes, s), "scan");
     HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
-    HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, in_start.p,
-                                           count.p, in_edges.p, s), "fill");
+    HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
+                                           in_start.p, count.p, in_edges.p, s), "fill");
     DevBuf<int> inv_pos;
     HIP_TRY(inv_pos.alloc(plan->n_edges + n_centers), "hipMalloc(inv_pos)");
     HIP_TRY(ldsf::launch_sort_inverse(n_total, in_start.p, in_edges.p, inv_pos.p, s),
             "sort and inverse");
+    HIP_TRY(hipMemcpyAsync(host_flags, scratch_i.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flags");
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (host_flags[0] & 1) return ldsf::fail(LDSF_ERR_ARG, "neighbour index out of range [0, n_total)");
     plan->offs = offs.release_ownership();
     plan->in_start = in_start.release_ownership();
     plan->in_edges = in_edges.release_ownership();

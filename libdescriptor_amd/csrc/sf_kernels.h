@@ -107,8 +107,10 @@ hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, 
                                 const int *species, int n_species, int *flags, hipStream_t s);
 hipError_t launch_edge_targets_count(long n_edges, int n_centers, const int *nl, const int *centers,
                                      int n_total, int *count, int *flags, hipStream_t s);
+// skips (as the count flags) targets outside [0, n_total)
 hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, const int *centers,
-                                    const int *in_start, int *cursor, int *in_edges, hipStream_t s);
+                                    int n_total, const int *in_start, int *cursor, int *in_edges,
+                                    hipStream_t s);
 // sorts every row's bucket of in_edges and writes inv_pos[edge id] = its position
 hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, int *inv_pos,
                                hipStream_t s);

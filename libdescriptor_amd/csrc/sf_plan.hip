@@ -67,12 +67,13 @@ __global__ void k_count(long n_edges, int n_centers, const int *__restrict__ nl,
 }
 
 __global__ void k_fill(long n_edges, int n_centers, const int *__restrict__ nl,
-                       const int *__restrict__ centers, const int *__restrict__ in_start,
-                       int *cursor, int *in_edges)
+                       const int *__restrict__ centers, int n_total,
+                       const int *__restrict__ in_start, int *cursor, int *in_edges)
 {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n_edges + n_centers) return;
     int a = edge_target(e, n_edges, nl, centers);
+    if (a < 0 || a >= n_total) return;  // flagged by k_count; the plan is refused
     int pos = atomicAdd(cursor + a, 1);
     in_edges[in_start[a] + pos] = (int)e;
 }
@@ -136,12 +137,13 @@ hipError_t launch_edge_targets_count(long n_edges, int n_centers, const int *nl,
 }
 
 hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, const int *centers,
-                                    const int *in_start, int *cursor, int *in_edges, hipStream_t s)
+                                    int n_total, const int *in_start, int *cursor, int *in_edges,
+                                    hipStream_t s)
 {
     long n = n_edges + n_centers;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, s, n_edges, n_centers, nl, centers,
-                       in_start, cursor, in_edges);
+                       n_total, in_start, cursor, in_edges);
     return hipGetLastError();
 }
 
