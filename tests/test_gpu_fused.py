@@ -214,6 +214,13 @@ def test_fused_full_size_262144():
     df, Gf = S.fused(0)
     assert elem_rel_err(df, d64, 1e-300) <= 1e-13
     assert rel_err(Gf, G64) <= 1e-13
+    # the FP64 form against the oracle at full size (its OpenMP forward + adjoint)
+    L = S.L
+    offs = np.r_[0, np.cumsum(L["numneigh"])].astype(np.int32)
+    d0, G0 = S.O.fwd_adj_omp(S.n, L["species"], L["neighlist"], offs, L["coords"],
+                             S.g.cpu().numpy(), 0)
+    assert elem_rel_err(df, d0, 1e-300) <= TOL64
+    assert rel_err(Gf, G0) <= TOL64
     d32, G32 = S.fused(1)
     assert signal_rel(d32, d64, BLOCKS) <= TOL32
     assert rel_err(G32, G64) <= TOL32
