@@ -233,6 +233,15 @@ LDSF_API int ldsf_plan_adjoint_edges(ldsf_plan *plan, const double *d_coordinate
                                      const double *d_dE_dzeta, int precision, void *stream);
 LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream);
 
+/* The adjoint's record buffer (no reference counterpart).  A plan allocates
+ * 3 * ldsf_plan_num_records(plan) doubles on its first adjoint call and frees
+ * them with the plan; ldsf_plan_set_records hands it a caller's device buffer
+ * of at least that size instead (kept, not freed, by the plan; it must
+ * outlive it), so that callers who build a plan per call can take the records
+ * from their own memory pool. */
+LDSF_API long ldsf_plan_num_records(const ldsf_plan *plan);
+LDSF_API int ldsf_plan_set_records(ldsf_plan *plan, double *d_records, long n_doubles);
+
 /* Which kernel family a plan uses: 0 = general (any parameter set), 1 = wave-
  * per-centre LDS kernels.  Setting 0 forces the general kernels. */
 LDSF_API int ldsf_plan_kernel_family(const ldsf_plan *plan);

@@ -55,6 +55,8 @@ _PROTOS = {
     "ldsf_plan_forward_adjoint_edges": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_kernel_family": (_i, [_vp]),
     "ldsf_plan_set_kernel_family": (_i, [_vp, _i]),
+    "ldsf_plan_num_records": (ctypes.c_long, [_vp]),
+    "ldsf_plan_set_records": (_i, [_vp, _vp, ctypes.c_long]),
     "ldsf_nbl_create_paddings": (_i, [_i, _d, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_padding_count": (_i, [_vp]),
     "ldsf_padding_copy": (_i, [_vp, _vp, _vp, _vp]),
@@ -191,6 +193,16 @@ class Plan:
     @kernel_family.setter
     def kernel_family(self, fam):
         check(lib().ldsf_plan_set_kernel_family(self.handle, int(fam)))
+
+    def set_records(self, records):
+        """Take the adjoint's records (3 * num_records float64 values) from
+        `records`, a device tensor kept alive with the plan."""
+        check(lib().ldsf_plan_set_records(self.handle, _ptr(records), int(records.numel())))
+        self._records = records
+
+    @property
+    def num_records(self):
+        return lib().ldsf_plan_num_records(self.handle)
 
     def compute(self, coords, desc, precision=FP64, stream=None):
         check(lib().ldsf_plan_compute(self.handle, _ptr(coords), _ptr(desc), int(precision),

@@ -67,6 +67,10 @@ class _FromPositionsFn(torch.autograd.Function):
         lay = _capi.DeviceLayout(pos, module.species, module.cell, module.pbc, module.cutoff)
         plan = _capi.Plan(module.descriptor, lay.n_contrib + lay.n_padding, lay.species,
                           lay.neighlist, lay.numneigh)
+        # the adjoint's records from torch's caching allocator: a plan per call
+        # then allocates no large buffer of its own
+        plan.set_records(torch.empty(3 * plan.num_records, dtype=torch.float64,
+                                     device=positions.device))
         desc = torch.zeros((lay.n_contrib, module.width), dtype=torch.float64,
                            device=positions.device)
         plan.compute(lay.coords, desc, module.precision)

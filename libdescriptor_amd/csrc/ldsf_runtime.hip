@@ -169,6 +169,7 @@ struct ldsf_plan {
     int *in_edges = nullptr;
     int *inv_pos = nullptr;
     double *edge = nullptr;
+    bool edge_owned = true;        // false: a caller's buffer (ldsf_plan_set_records)
     ldsf::WaveParams *wave_par = nullptr;
 
     ~ldsf_plan()
@@ -177,7 +178,7 @@ struct ldsf_plan {
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device);
         for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
-                        (void *)inv_pos, (void *)edge, (void *)wave_par})
+                        (void *)inv_pos, edge_owned ? (void *)edge : nullptr, (void *)wave_par})
             if (p) (void)hipFree(p);
         (void)hipSetDevice(cur);
     }
@@ -549,6 +550,27 @@ LDSF_API int ldsf_plan_set_kernel_family(ldsf_plan *plan, int family)
     if (family == 1 && !plan->wave_ok)
         return ldsf::fail(LDSF_ERR_UNSUPPORTED, "wave kernels do not cover this parameter set");
     plan->family = family;
+    return LDSF_OK;
+}
+
+LDSF_API long ldsf_plan_num_records(const ldsf_plan *plan)
+{
+    return plan ? plan->n_edges + plan->n_centers : 0;
+}
+
+LDSF_API int ldsf_plan_set_records(ldsf_plan *plan, double *d_records, long n_doubles)
+{
+    if (!plan) return ldsf::fail(LDSF_ERR_ARG, "plan_set_records: null plan");
+    const long need = 3 * (plan->n_edges + plan->n_centers);
+    if (need > 0 && (!d_records || n_doubles < need))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_set_records: buffer smaller than 3 * ldsf_plan_num_records");
+    if (plan->edge && plan->edge_owned) {
+        HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        (void)hipFree(plan->edge);
+    }
+    plan->edge = need > 0 ? d_records : nullptr;
+    plan->edge_owned = false;
+    plan->edges_ready = false;
     return LDSF_OK;
 }
 

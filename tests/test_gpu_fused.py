@@ -205,6 +205,24 @@ def test_host_compute_overwrite_and_accumulate(c1):
     assert elem_rel_err(acc - base, over, 1e-300) <= 1e-11
 
 
+def test_caller_record_buffer():
+    """ldsf_plan_set_records: the adjoint writes into the caller's buffer and the
+    results equal the plan's own; a buffer too small is refused."""
+    print("case: 400-atom two-species, caller record buffer", flush=True)
+    S = Setup(two_species(400, 13))
+    d0, G0 = S.fused(1)
+    with pytest.raises(Exception):
+        S.plan.set_records(torch.empty(3 * S.plan.num_records - 1, dtype=torch.float64,
+                                       device=S.x.device))
+    rec = torch.full((3 * S.plan.num_records,), float("nan"), dtype=torch.float64,
+                     device=S.x.device)
+    S.plan.set_records(rec)
+    d1, G1 = S.fused(1)
+    np.testing.assert_array_equal(d1, d0)
+    np.testing.assert_array_equal(G1, G0)
+    assert not torch.isnan(rec).any()
+
+
 def test_fused_full_size_262144():
     """The benchmark's system: fused against the separate kernels, both forms."""
     from libdescriptor_amd import workloads as w
