@@ -204,16 +204,18 @@ __global__ void k_extent(int n, const double *__restrict__ x, Box *partial)
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
+// one block: the threads take the partial boxes k = t, t + kBlock, ..., then
+// the block reduction (min and max are exact, so the order does not matter)
 __global__ void k_box_final(int nb, const Box *partial, Box *out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     Box b = partial[0];
-    for (int k = 1; k < nb; ++k)
+    for (int k = threadIdx.x; k < nb; k += blockDim.x)
         for (int d = 0; d < 3; ++d) {
             b.lo[d] = fmin(b.lo[d], partial[k].lo[d]);
             b.hi[d] = fmax(b.hi[d], partial[k].hi[d]);
         }
-    *out = b;
+    box_reduce_block(b);
+    if (threadIdx.x == 0) *out = b;
 }
 
 struct PadParams {
@@ -541,7 +543,7 @@ namespace {
 // the extent over the blocks' partial boxes, read back to the host
 hipError_t read_box(const Box *partial, int nb, Box &out, Box *dev_out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, s, nb, partial, dev_out);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(kBlock), 0, s, nb, partial, dev_out);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&out, dev_out, sizeof(Box), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
