@@ -109,9 +109,15 @@ __global__ void k_max(int n, const int *__restrict__ v, int *out)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int m = 0;
     for (; i < n; i += gridDim.x * blockDim.x) m = max(m, v[i]);
-    // wave reduction, then one atomic per wave
+    // wave reduction, then the block's waves through LDS: one atomic per block
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+    __shared__ int wm[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = max(m, wm[k]);
+        atomicMax(out, m);
+    }
 }
 
 }  // namespace
