@@ -55,6 +55,19 @@ struct SFTab {
     int pad2_;
 };
 
+// The device table of a parameter set packed on the host, as device_table
+// uploads it: every double array, then every int array, at the recorded
+// offsets (elements from the start of its block).
+struct PackedTable {
+    std::vector<double> d;
+    std::vector<int> i;
+    size_t o_rc = 0, o_a2 = 0, o_b2 = 0, o_z3 = 0, o_l3 = 0, o_e3 = 0, o_c3 = 0;
+    size_t o_k2 = 0, o_s2 = 0, o_t2 = 0, o_k3 = 0, o_iz = 0, o_s3 = 0, o_t3 = 0, o_p2 = 0,
+           o_p3 = 0;
+    // bytes of one upload: the doubles (plus one), then the ints (plus two)
+    size_t bytes() const { return sizeof(double) * (d.size() + 1) + sizeof(int) * (i.size() + 2); }
+};
+
 }  // namespace ldsf
 
 // Host-side parameter set (opaque in the C-ABI).
@@ -71,6 +84,7 @@ struct ldsf_sf {
     std::vector<int> k2, slot2, str2, k3, izeta3, slot3, str3, pidx2, pidx3;
     int n_params = 0;
     std::vector<double> a2, b2, zeta3, lam3, eta3, coef3;
+    ldsf::PackedTable packed;  // built by finalize_params
     // per-device uploads
     struct Dev {
         int device = -1;
@@ -86,6 +100,8 @@ namespace ldsf {
 int finalize_params(ldsf_sf *sf);
 // device table for the current device (uploads on first use)
 int device_table(const ldsf_sf *sf, SFTab *out);
+// the table's view of an upload of sf->packed at dd (doubles) and di (ints)
+SFTab table_view(const ldsf_sf *sf, const double *dd, const int *di);
 }  // namespace ldsf
 
 #endif  // LDSF_INTERNAL_H_

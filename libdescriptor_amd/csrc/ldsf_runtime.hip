@@ -77,6 +77,35 @@ ldsf_sf::~ldsf_sf()
 
 namespace ldsf {
 
+SFTab table_view(const ldsf_sf *sf, const double *dd, const int *di)
+{
+    const PackedTable &T = sf->packed;
+    SFTab t{};
+    t.n_species = (int)sf->species.size();
+    t.width = sf->width;
+    t.n2 = (int)sf->k2.size();
+    t.n3 = (int)sf->k3.size();
+    t.species_resolved = sf->species_resolved;
+    t.rc = dd + T.o_rc;
+    t.a2 = dd + T.o_a2;
+    t.b2 = dd + T.o_b2;
+    t.zeta3 = dd + T.o_z3;
+    t.lam3 = dd + T.o_l3;
+    t.eta3 = dd + T.o_e3;
+    t.coef3 = dd + T.o_c3;
+    t.k2 = di + T.o_k2;
+    t.slot2 = di + T.o_s2;
+    t.str2 = di + T.o_t2;
+    t.k3 = di + T.o_k3;
+    t.izeta3 = di + T.o_iz;
+    t.slot3 = di + T.o_s3;
+    t.str3 = di + T.o_t3;
+    t.pidx2 = di + T.o_p2;
+    t.pidx3 = di + T.o_p3;
+    t.n_params = sf->n_params;
+    return t;
+}
+
 int device_table(const ldsf_sf *csf, SFTab *out)
 {
     auto *sf = const_cast<ldsf_sf *>(csf);
@@ -87,63 +116,26 @@ int device_table(const ldsf_sf *csf, SFTab *out)
             *out = d.tab;
             return LDSF_OK;
         }
-    // one allocation: doubles first, then ints, each array 8-byte aligned
-    const int ns = (int)sf->species.size();
-    const size_t n2 = sf->k2.size(), n3 = sf->k3.size();
-    const size_t nd = ns * ns + 2 * n2 + 4 * n3;
-    const size_t ni = 4 * n2 + 5 * n3;
-    std::vector<double> hd;
-    hd.reserve(nd);
-    std::vector<int> hi;
-    hi.reserve(ni + 1);
-    auto put_d = [&](const std::vector<double> &v) { size_t o = hd.size(); hd.insert(hd.end(), v.begin(), v.end()); return o; };
-    auto put_i = [&](const std::vector<int> &v) { size_t o = hi.size(); hi.insert(hi.end(), v.begin(), v.end()); return o; };
-    size_t o_rc = put_d(sf->rcut), o_a2 = put_d(sf->a2), o_b2 = put_d(sf->b2), o_z3 = put_d(sf->zeta3),
-           o_l3 = put_d(sf->lam3), o_e3 = put_d(sf->eta3), o_c3 = put_d(sf->coef3);
-    size_t o_k2 = put_i(sf->k2), o_s2 = put_i(sf->slot2), o_t2 = put_i(sf->str2), o_k3 = put_i(sf->k3),
-           o_iz = put_i(sf->izeta3), o_s3 = put_i(sf->slot3), o_t3 = put_i(sf->str3),
-           o_p2 = put_i(sf->pidx2), o_p3 = put_i(sf->pidx3);
-    size_t bytes = sizeof(double) * (hd.size() + 1) + sizeof(int) * (hi.size() + 2);
+    // one allocation: the packed doubles, then the packed ints
+    const PackedTable &T = sf->packed;
     void *mem = nullptr;
-    HIP_TRY(hipMalloc(&mem, bytes), "hipMalloc(parameter table)");
+    HIP_TRY(hipMalloc(&mem, T.bytes()), "hipMalloc(parameter table)");
     double *dd = (double *)mem;
-    int *di = (int *)(dd + hd.size() + 1);
+    int *di = (int *)(dd + T.d.size() + 1);
     hipError_t e = hipSuccess;
-    if (!hd.empty()) e = hipMemcpy(dd, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !hi.empty()) e = hipMemcpy(di, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice);
+    if (!T.d.empty()) e = hipMemcpy(dd, T.d.data(), sizeof(double) * T.d.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !T.i.empty())
+        e = hipMemcpy(di, T.i.data(), sizeof(int) * T.i.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(mem);
         return hip_fail(e, "upload parameter table");
     }
-    SFTab t{};
-    t.n_species = ns;
-    t.width = sf->width;
-    t.n2 = (int)n2;
-    t.n3 = (int)n3;
-    t.species_resolved = sf->species_resolved;
-    t.rc = dd + o_rc;
-    t.a2 = dd + o_a2;
-    t.b2 = dd + o_b2;
-    t.zeta3 = dd + o_z3;
-    t.lam3 = dd + o_l3;
-    t.eta3 = dd + o_e3;
-    t.coef3 = dd + o_c3;
-    t.k2 = di + o_k2;
-    t.slot2 = di + o_s2;
-    t.str2 = di + o_t2;
-    t.k3 = di + o_k3;
-    t.izeta3 = di + o_iz;
-    t.slot3 = di + o_s3;
-    t.str3 = di + o_t3;
-    t.pidx2 = di + o_p2;
-    t.pidx3 = di + o_p3;
-    t.n_params = sf->n_params;
     ldsf_sf::Dev d;
     d.device = device;
     d.mem = mem;
-    d.tab = t;
+    d.tab = table_view(sf, dd, di);
     sf->dev.push_back(d);
-    *out = t;
+    *out = d.tab;
     return LDSF_OK;
 }
 
@@ -284,7 +276,7 @@ int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, 
     ldsf::SFTab tab;
     if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
     hipError_t e;
-    if (plan->family == 1 && plan->wave_ok)
+    if (plan->family == 1 && plan->wave_ok && ldsf::wave_runs(plan->wave, precision))
         e = ldsf::launch_fwd_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, desc, tab.width,
                                   plan->max_nn, plan->wave_group, s);
@@ -305,7 +297,7 @@ int plan_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, int p
     if (!plan->edge && n_rec > 0)
         HIP_TRY(hipMalloc((void **)&plan->edge, sizeof(double) * 3 * n_rec), "hipMalloc(edge)");
     hipError_t e;
-    if (plan->family == 1 && plan->wave_ok)
+    if (plan->family == 1 && plan->wave_ok && ldsf::wave_runs(plan->wave, precision))
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
                                   plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,
@@ -330,7 +322,7 @@ int plan_forward_adjoint_edges(ldsf_plan *plan, const double *x, const double *d
     if (!plan->edge && n_rec > 0)
         HIP_TRY(hipMalloc((void **)&plan->edge, sizeof(double) * 3 * n_rec), "hipMalloc(edge)");
     hipError_t e;
-    if (plan->family == 1 && plan->wave_ok) {
+    if (plan->family == 1 && plan->wave_ok && ldsf::wave_runs(plan->wave, precision)) {
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
                                   plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,

@@ -73,11 +73,22 @@ struct WaveTab {
     int rc_uniform;         // every species pair has the same cutoff
     int zgrid;              // zeta grid (1, 2, 4, 16) = 1, (1, 2, 4, 8) = 2, else 0: the FP32-bulk
                             // kernels' compile-time squaring chains
+    int fp32_exp_ok;        // every exponent the FP32-bulk form takes in FP32 is <= kFp32ExpMax
     const double *rc;       // device, n_species^2
     const WaveParams *par;  // device
 };
 constexpr int kWaveSlots = 64;
 constexpr int kMaxWaveSpecies = 8;
+// Largest exponent eta (r - Rs)^2 (g2) or eta (r_ij^2 + r_ik^2 + r_jk^2) (g4)
+// that the FP32-bulk wave kernels form in FP32; sets that reach beyond it
+// over the cutoff sphere run the FP32-bulk general kernels, which split the
+// exponent in FP64 (relative accuracy ~1e-7 for any exponent).
+constexpr double kFp32ExpMax = 32.0;
+// whether the wave kernels (when built for the set) run in this precision form
+inline bool wave_runs(const WaveTab &wt, int precision)
+{
+    return precision == 0 || wt.fp32_exp_ok != 0;
+}
 // host: true (and the table and parameters) if the wave kernels cover this
 // parameter set
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *par);
@@ -114,8 +125,9 @@ hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, 
 // sorts every row's bucket of in_edges and writes inv_pos[edge id] = its position
 hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, int *inv_pos,
                                hipStream_t s);
+// dcoords[a] += the sum of row a's records; nothing when *skip != 0 (skip may be NULL)
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
-                         hipStream_t s);
+                         hipStream_t s, const int *skip = nullptr);
 hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s);
 hipError_t launch_scan_int(int n, const int *in, int *out_exclusive_plus_total, void *tmp,
                            size_t *tmp_bytes, hipStream_t s);

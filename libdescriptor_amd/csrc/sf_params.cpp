@@ -169,6 +169,36 @@ int finalize_params(ldsf_sf *sf)
     }
     sf->width = w;
     sf->n_params = np;
+    // the device table's host image
+    PackedTable &T = sf->packed;
+    T.d.clear();
+    T.i.clear();
+    auto put_d = [&](const std::vector<double> &v) {
+        const size_t o = T.d.size();
+        T.d.insert(T.d.end(), v.begin(), v.end());
+        return o;
+    };
+    auto put_i = [&](const std::vector<int> &v) {
+        const size_t o = T.i.size();
+        T.i.insert(T.i.end(), v.begin(), v.end());
+        return o;
+    };
+    T.o_rc = put_d(sf->rcut);
+    T.o_a2 = put_d(sf->a2);
+    T.o_b2 = put_d(sf->b2);
+    T.o_z3 = put_d(sf->zeta3);
+    T.o_l3 = put_d(sf->lam3);
+    T.o_e3 = put_d(sf->eta3);
+    T.o_c3 = put_d(sf->coef3);
+    T.o_k2 = put_i(sf->k2);
+    T.o_s2 = put_i(sf->slot2);
+    T.o_t2 = put_i(sf->str2);
+    T.o_k3 = put_i(sf->k3);
+    T.o_iz = put_i(sf->izeta3);
+    T.o_s3 = put_i(sf->slot3);
+    T.o_t3 = put_i(sf->str3);
+    T.o_p2 = put_i(sf->pidx2);
+    T.o_p3 = put_i(sf->pidx3);
     return LDSF_OK;
 }
 

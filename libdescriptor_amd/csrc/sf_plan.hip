@@ -89,12 +89,14 @@ __global__ void k_sort_inverse(int n_total, const int *__restrict__ in_start, in
 }
 
 // records of row a are contiguous: [in_start[a], in_start[a+1]), in ascending
-// edge id, written there by the adjoint kernels
+// edge id, written there by the adjoint kernels; nothing is added when *skip
+// is non-zero (skip may be NULL)
 __global__ void k_gather(int n_total, const int *__restrict__ in_start,
-                         const double *__restrict__ edge, double *__restrict__ dcoords)
+                         const double *__restrict__ edge, double *__restrict__ dcoords,
+                         const int *__restrict__ skip)
 {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_total) return;
+    if (a >= n_total || (skip && *skip)) return;
     int b = in_start[a], e = in_start[a + 1];
     if (b == e) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -165,11 +167,11 @@ hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, 
 }
 
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
-                         hipStream_t s)
+                         hipStream_t s, const int *skip)
 {
     if (n_total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start, edge,
-                       dcoords);
+                       dcoords, skip);
     return hipGetLastError();
 }
 

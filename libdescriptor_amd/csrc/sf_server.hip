@@ -48,6 +48,7 @@
 #include "ldsf_internal.h"
 #include "sf_device.h"
 #include "sf_kernels.h"
+#include "sf_scan.h"
 
 namespace ldsf {
 
@@ -70,10 +71,10 @@ struct Seg {
 enum { H_CENTERS = 0, H_ROWS = 1, H_EDGES = 2, H_REQ = 3, H_INTS = 16 };
 
 constexpr int kThreads = 256;
-constexpr int kScanBlocks = 256;
 constexpr int kGridStride = 2048;
 constexpr int kSegBlocks = 32;     // blocks per request in the copy kernels
-constexpr int kBuckets = 9;        // wave kernels for max_nn <= 16, 24, ..., 64; general kernels
+constexpr int kBuckets = 9;        // wave kernels for max_nn <= 8, 16, ..., 64; general kernels
+                                   // (max_nn > 64, or a set the wave kernels do not cover)
 constexpr int kGeneralBucket = 8;
 constexpr int kWaveBlocks = 4096;  // persistent grid of the wave kernels (~16 waves per CU)
 
@@ -145,86 +146,6 @@ __global__ void k_check_sums(const int *__restrict__ hdr, const Seg *__restrict_
 {
     for (int r = threadIdx.x; r < hdr[H_REQ]; r += blockDim.x)
         if (req_sum[r] != seg[r].n_neigh) atomicOr(flags + r, 2);
-}
-
-// ---- exclusive scan of in[0..n) into out[0..n], n = hdr[which] ----
-
-__device__ int block_sum(int v)
-{
-    __shared__ int ws[kThreads / 64];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    int s = 0;
-    for (int k = 0; k < kThreads / 64; ++k) s += ws[k];
-    __syncthreads();
-    return s;
-}
-
-// inclusive scan over the block; *total gets the block's sum
-__device__ int block_scan(int v, int *total)
-{
-    __shared__ int ws[kThreads / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int u = __shfl_up(v, off);
-        if (lane >= off) v += u;
-    }
-    if (lane == 63) ws[w] = v;
-    __syncthreads();
-    int before = 0, all = 0;
-    for (int k = 0; k < kThreads / 64; ++k) {
-        if (k < w) before += ws[k];
-        all += ws[k];
-    }
-    __syncthreads();
-    *total = all;
-    return v + before;
-}
-
-__global__ void k_scan_partial(const int *__restrict__ hdr, int which, const int *__restrict__ in,
-                               int *part)
-{
-    const long n = hdr[which];
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
-    const long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-    int s = 0;
-    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += in[i];
-    s = block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-__global__ void k_scan_top(int *part, int nb)
-{
-    // one block: exclusive scan of part[0..nb), total into part[nb]
-    int run = 0;
-    for (int base = 0; base < nb; base += blockDim.x) {
-        const int i = base + threadIdx.x;
-        const int v = i < nb ? part[i] : 0;
-        int tot;
-        const int inc = block_scan(v, &tot);
-        if (i < nb) part[i] = run + inc - v;
-        run += tot;
-    }
-    if (threadIdx.x == 0) part[nb] = run;
-}
-
-__global__ void k_scan_final(const int *__restrict__ hdr, int which, const int *__restrict__ in,
-                             const int *__restrict__ part, int *out)
-{
-    const long n = hdr[which];
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
-    const long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-    int run = part[blockIdx.x];
-    for (long base = lo; base < hi; base += blockDim.x) {
-        const long i = base + threadIdx.x;
-        const int v = i < hi ? in[i] : 0;
-        int tot;
-        const int inc = block_scan(v, &tot);
-        if (i < hi) out[i] = run + inc - v;
-        run += tot;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[gridDim.x];
 }
 
 // CSR offsets of every request from its own edge base
@@ -470,18 +391,10 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
                        L.in_row, L.zero_row, L.req_sum, L.flags, L.count);
     hipLaunchKernelGGL(k_check_sums, dim3(1), tb, 0, s, hdr, seg, L.req_sum, L.flags);
     // CSR offsets: scan of the neighbour counts, rebased per request
-    hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_CENTERS,
-                       L.numneigh, L.part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, L.part, kScanBlocks);
-    hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_CENTERS, L.numneigh,
-                       L.part, L.scan);
+    enqueue_scan(Count{hdr + H_CENTERS, 0}, L.numneigh, L.part, L.scan, s);
     hipLaunchKernelGGL(k_offsets, gs, tb, 0, s, hdr, seg, L.req_of, L.scan, L.offs);
     // transposed CSR: scan of the counts, fill, per-row sort with the inverse
-    hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_ROWS, L.count,
-                       L.part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, L.part, kScanBlocks);
-    hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), tb, 0, s, hdr, (int)H_ROWS, L.count,
-                       L.part, L.in_start);
+    enqueue_scan(Count{hdr + H_ROWS, 0}, L.count, L.part, L.in_start, s);
     hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.cursor,
                        L.in_edges);
     hipLaunchKernelGGL(k_sort_inverse_t, gs, tb, 0, s, hdr, L.in_start, L.in_edges, L.inv_pos);
@@ -602,14 +515,23 @@ void worker_main(ldsf_server *S)
                 L = &l;
                 break;
             }
-        // admit in arrival order while the batch fits the lane
+        // admit in arrival order while the batch fits the lane.  A batch runs
+        // one kernel family (the general kernels for neighbourhoods above 64,
+        // else the wave kernels), and the two round differently: a request
+        // of the other family closes the batch, so that every request is
+        // computed by the kernels that would compute it alone.
         long C = 0, R = 0, E = 0;
         int maxnn = 0;
+        bool general = false;
         while (!S->queue.empty()) {
             const ldsf_request &q = S->queue.front()->req;
+            const bool qg = bucket_for(S, q.max_neighbors) == ldsf::kGeneralBucket;
             if (!L->batch.empty() &&
-                !fits(S->cfg, C + q.n_atoms, R + q.n_total, E + q.n_neigh, (int)L->batch.size() + 1))
+                (qg != general ||
+                 !fits(S->cfg, C + q.n_atoms, R + q.n_total, E + q.n_neigh,
+                       (int)L->batch.size() + 1)))
                 break;
+            general = qg;
             ldsf_ticket_rec *T = S->queue.front();
             S->queue.pop_front();
             S->queued_centers -= q.n_atoms;
@@ -721,7 +643,8 @@ LDSF_API int ldsf_server_create(ldsf_server **out, const ldsf_sf *sf,
         return rc;
     }
     ldsf::WaveParams wp;
-    if (ldsf::wave_build(sf, &S->wave, &wp)) {
+    // (FP32-bulk sets whose exponents leave the FP32 range run the general kernels)
+    if (ldsf::wave_build(sf, &S->wave, &wp) && ldsf::wave_runs(S->wave, c.precision)) {
         if (hipMalloc((void **)&S->wave_par, sizeof(wp)) != hipSuccess ||
             hipMemcpy(S->wave_par, &wp, sizeof(wp), hipMemcpyHostToDevice) != hipSuccess) {
             if (S->wave_par) (void)hipFree(S->wave_par);

@@ -32,8 +32,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <map>
-#include <vector>
 
 #include "sf_device.h"
 #include "sf_kernels.h"
@@ -1713,6 +1711,24 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
         for (int s = 0; s < wt->nsb; ++s) slots[s * 8 + q] = sf->slot2[q] + s * sf->str2[q];
     }
     wt->nr2 = n2;
+    // FP32-bulk: the wave kernels form each term's exponent x in FP32, with a
+    // relative error of about 3e-7 |x| (the roundings of eta log2(e), r - Rs,
+    // its square and the product); every x over the cutoff sphere must stay
+    // within kFp32ExpMax, where that is at most 1e-5 (and far from FP32's
+    // underflow at 87).  The largest g2 exponent over r in [0, rc] is at an
+    // end of the interval; the g4 factors' product reaches eta 3 rc^2.
+    double rcmax = 0.0;
+    for (double v : sf->rcut) rcmax = std::max(rcmax, v);
+    wt->fp32_exp_ok = 1;
+    for (int q = 0; q < n2; ++q) {
+        const double eta = sf->a2[q], rs = sf->b2[q];
+        const double xm = std::max(eta * rs * rs, eta * (rcmax - rs) * (rcmax - rs));
+        if (!(eta >= 0.0) || !(xm <= kFp32ExpMax)) wt->fp32_exp_ok = 0;
+    }
+    for (size_t q = 0; q < sf->k3.size(); ++q) {
+        const double eta = sf->eta3[q];
+        if (!(eta >= 0.0) || !(3.0 * eta * rcmax * rcmax <= kFp32ExpMax)) wt->fp32_exp_ok = 0;
+    }
     for (int l = 0; l < 2; ++l) {  // neutral lambda = 0 until the grid sets it
         wp->lp[l] = 0.5;
         wp->lm[l] = 0.5;
@@ -1720,31 +1736,39 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
     // three-body rows: g4 forming a zeta x lambda x eta grid
     const int n3 = (int)sf->k3.size();
     if (n3 == 0) return true;
-    std::vector<double> zs, ls, es;
-    auto idx = [](std::vector<double> &v, double x) {
-        for (size_t k = 0; k < v.size(); ++k)
-            if (v[k] == x) return (int)k;
-        v.push_back(x);
-        return (int)v.size() - 1;
+    // the distinct zeta, lambda and eta of the rows (no heap: the allocation-
+    // free entry point builds these tables on every call)
+    double zs[5], ls[3], es[3];
+    int nzs = 0, nls = 0, nes = 0;
+    auto idx = [](double *v, int &n, int cap, double x) {
+        for (int k = 0; k < n; ++k)
+            if (v[k] == x) return k;
+        if (n == cap) return -1;
+        v[n] = x;
+        return n++;
     };
-    std::map<int, int> seen;
+    int seen[16];  // row of grid point (z * 2 + l) * 2 + e, or -1
+    for (int k = 0; k < 16; ++k) seen[k] = -1;
+    int n_seen = 0;
     for (int q = 0; q < n3; ++q) {
         if (sf->k3[q] != G4) return false;
-        const int z = idx(zs, sf->zeta3[q]), l = idx(ls, sf->lam3[q]), e = idx(es, sf->eta3[q]);
-        if (zs.size() > 4 || ls.size() > 2 || es.size() > 2) return false;
+        const int z = idx(zs, nzs, 4, sf->zeta3[q]), l = idx(ls, nls, 2, sf->lam3[q]),
+                  e = idx(es, nes, 2, sf->eta3[q]);
+        if (z < 0 || l < 0 || e < 0) return false;
         const int k = (z * 2 + l) * 2 + e;
-        if (seen.count(k)) return false;
+        if (seen[k] >= 0) return false;
         seen[k] = q;
+        ++n_seen;
     }
-    if ((int)seen.size() != (int)(zs.size() * ls.size() * es.size())) return false;
+    if (n_seen != nzs * nls * nes) return false;
     // the adjoint stages a centre's whole descriptor row (<= kWaveSlots entries)
     if (n2 * wt->nsb + n3 * wt->npb > kWaveSlots) return false;
     wt->has3 = 1;
-    wt->nz = (int)zs.size();
-    wt->nl = (int)ls.size();
-    wt->ne = (int)es.size();
+    wt->nz = nzs;
+    wt->nl = nls;
+    wt->ne = nes;
     // the power-of-two grids with compile-time squaring chains (ZGrid)
-    if (zs.size() == 4 && zs[0] == 1.0 && zs[1] == 2.0 && zs[2] == 4.0)
+    if (nzs == 4 && zs[0] == 1.0 && zs[1] == 2.0 && zs[2] == 4.0)
         wt->zgrid = zs[3] == 16.0 ? 1 : (zs[3] == 8.0 ? 2 : 0);
     for (int z = 0; z < wt->nz; ++z) {
         wp->zeta[z] = zs[z];
@@ -1767,9 +1791,10 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
         wp->lm[l] = 0.5 * (1.0 - wp->lam[l]);
     }
     for (int e = 0; e < wt->ne; ++e) wp->eta3[e] = es[e];
-    for (auto &kv : seen) {
-        const int q = kv.second;
-        for (int p = 0; p < wt->npb; ++p) slots[16 + p * 16 + kv.first] = sf->slot3[q] + p * sf->str3[q];
+    for (int k = 0; k < 16; ++k) {
+        const int q = seen[k];
+        if (q < 0) continue;
+        for (int p = 0; p < wt->npb; ++p) slots[16 + p * 16 + k] = sf->slot3[q] + p * sf->str3[q];
     }
     return true;
 }

@@ -10,14 +10,25 @@ Checks:
   without disturbing the requests batched with it;
   requests submitted from several threads while batches run are all served;
   admission limits: a request beyond the lane capacity and a budget below one
-  lane are errors; graphs are replayed, not re-captured, across batches.
+  lane are errors; graphs are replayed, not re-captured, across batches;
+  the serving workload at its own scale: 200 requests of 16-20,000 atoms
+  (at least three above 10,000) drawn like the benchmark's request pool, plus
+  a dense cluster whose neighbourhoods exceed 64 atoms (the general-kernel
+  bucket), submitted on a seeded Poisson schedule to 3 lanes in both
+  precision forms -- every result bitwise equal to the same request alone,
+  and a sample (the large requests, the dense one and a few others) checked
+  against the FP64 oracle's compute + gradient (src/Descriptors.cpp:263-292):
+  FP64 descriptors <= 1e-12 elementwise and dE/dr <= 1e-12 normwise,
+  FP32-bulk <= 1e-5 in the error statement of libdescriptor_amd.errstat
+  (descriptors per block on the entries that carry signal, dE/dr
+  elementwise with its floor).
 """
 import threading
 
 import numpy as np
 import pytest
 
-from conftest import behler
+from conftest import behler, elem_rel_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -186,3 +197,91 @@ def test_admission_limits():
     with pytest.raises(_capi.LdsfError):  # 150 centres > 100 per lane
         submit(srv, ds, D)
     srv.close()
+
+
+def c5_stream(seed, n_req=200):
+    """Requests drawn from a pool generated like the benchmark's serving pool
+    (log-uniform 16..20,000 atoms, random periodic, 0.05 atoms/A^3), with the
+    pool's three largest configurations (each above 10,000 atoms) always in,
+    and one dense cluster (max_nn > 64)."""
+    from libdescriptor_amd import workloads as w
+    pool = w.config_c5_pool(n_pool=64, seed=seed)
+    sizes = np.array([L["n_contrib"] for L in pool])
+    big = list(np.argsort(sizes)[-3:])
+    assert all(sizes[k] > 10000 for k in big)
+    pos, cell, pbc, sp = w.cluster(400, 0.15, 1.2, seed=seed + 1)
+    dense = w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+    assert int(dense["numneigh"].max()) > 64
+    pool.append(dense)
+    rng = np.random.default_rng(seed)
+    picks = big + [len(pool) - 1] + list(rng.integers(0, len(pool) - 1, n_req - 4))
+    rng.shuffle(picks)
+    return pool, picks
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_c5_stream_against_oracle(prec):
+    import time
+
+    import torch
+    from libdescriptor_amd import _capi, errstat
+    from oracle import orc
+    ds = sf(prec)
+    pool, picks = c5_stream(seed=40 + prec)
+    print(f"case: serving stream prec={prec}: {len(picks)} requests, "
+          f"{sum(pool[k]['n_contrib'] for k in picks)} atoms", flush=True)
+    devs = {}
+    for k in set(picks):
+        D = to_dev(pool[k], ds.width, 0)
+        devs[k] = D
+    rng = np.random.default_rng(prec)
+    reqs = []
+    for q, k in enumerate(picks):
+        D = dict(devs[k])
+        g = rng.normal(size=(pool[k]["n_contrib"], ds.width))
+        D["dE"] = torch.tensor(g, dtype=torch.float64, device="cuda")
+        reqs.append(D)
+    max_rows = max(int(D["x"].shape[0]) for D in reqs)
+    srv = _capi.Server(ds, prec, hbm_budget_bytes=3 << 30, lane_centers=32768,
+                       lane_rows=max(65536, max_rows), lane_edges=32768 * 48, max_requests=64,
+                       max_lanes=3, batch_window_us=200)
+    assert srv.num_lanes == 3
+    # seeded Poisson arrivals (mean gap 0.2 ms): requests arrive while others run
+    gaps = np.random.default_rng(7).exponential(2e-4, len(reqs))
+    tickets = []
+    for D, gap in zip(reqs, gaps):
+        tickets.append(submit(srv, ds, D))
+        time.sleep(float(gap))
+    for t in tickets:
+        srv.wait(t, timeout=120.0)
+    st = srv.stats()
+    print(f"stats {st}", flush=True)
+    assert st["requests"] == len(reqs) and st["batches"] >= 3
+    for q, D in enumerate(reqs):
+        d, g = alone(ds, prec, D)
+        assert torch.equal(D["desc"], d), q
+        assert torch.equal(D["dx"], g), q
+    srv.close()
+    # the oracle on a sample: the three large requests, the dense one, three more
+    types, sizes, params = behler()
+    O = orc.OracleSF(["A", "B"], 5.0, types, sizes, params, species_resolved=True)
+    big = sorted(range(len(picks)), key=lambda q: -pool[picks[q]]["n_contrib"])[:3]
+    dense = [q for q in range(len(picks)) if picks[q] == len(pool) - 1][:1]
+    sample = big + dense + [q for q in range(len(picks)) if q not in big + dense][:3]
+    for q in sample:
+        L, D = pool[picks[q]], reqs[q]
+        n = L["n_contrib"]
+        offs = np.concatenate([[0], np.cumsum(L["numneigh"])]).astype(np.int32)
+        g = D["dE"].cpu().numpy()
+        d0, g0 = O.fwd_adj_omp(n, L["species"], L["neighlist"], offs, L["coords"], g, 16)
+        d, gx = D["desc"].cpu().numpy(), D["dx"].cpu().numpy()
+        if prec == 0:
+            assert elem_rel_err(d, d0, 1e-300) <= 1e-12, q
+            assert rel_err(gx, g0) <= 1e-12, q
+        else:
+            de = errstat.descriptor_errors(d, d0, errstat.BLOCKS_TWO_SPECIES)
+            ge = errstat.gradient_errors(gx, g0)
+            print(f"request {q} ({n} atoms): descriptors max_rel "
+                  f"{errstat.worst(de, 'max_rel'):.2e}, dE/dr {ge}", flush=True)
+            assert errstat.worst(de, "max_rel") <= 1e-5, q
+            assert ge["max_rel_normwise"] <= 1e-5, q
