@@ -21,6 +21,8 @@
 #ifndef LDSF_H_
 #define LDSF_H_
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -238,7 +240,9 @@ LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream);
  * them with the plan; ldsf_plan_set_records hands it a caller's device buffer
  * of at least that size instead (kept, not freed, by the plan; it must
  * outlive it), so that callers who build a plan per call can take the records
- * from their own memory pool. */
+ * from their own memory pool.  d_records must be device memory on the plan's
+ * device, float64 (n_doubles counts doubles) and contiguous; only the count
+ * can be checked here. */
 LDSF_API long ldsf_plan_num_records(const ldsf_plan *plan);
 LDSF_API int ldsf_plan_set_records(ldsf_plan *plan, double *d_records, long n_doubles);
 

@@ -173,6 +173,7 @@ class Plan:
                                          st))
         self.handle = out.value
         self.n_total = n_total
+        self.device = getattr(species, "device", None)
 
     @property
     def n_centers(self):
@@ -196,7 +197,9 @@ class Plan:
 
     def set_records(self, records):
         """Take the adjoint's records (3 * num_records float64 values) from
-        `records`, a device tensor kept alive with the plan."""
+        `records`, a contiguous float64 tensor on the plan's device, kept alive
+        with the plan."""
+        _check_records(records, self.device)
         check(lib().ldsf_plan_set_records(self.handle, _ptr(records), int(records.numel())))
         self._records = records
 
@@ -245,6 +248,21 @@ class Plan:
         if h and _lib is not None:
             _lib.ldsf_plan_destroy(h)
             self.handle = None
+
+
+def _check_records(records, device):
+    """The record buffer the C side writes 3 * num_records float64 values
+    into: a float64, contiguous tensor on the plan's device (the C-ABI sees
+    only an address and a count)."""
+    import torch
+    if not isinstance(records, torch.Tensor):
+        raise TypeError("records must be a torch tensor")
+    if records.dtype != torch.float64:
+        raise ValueError(f"records must be float64, not {records.dtype}")
+    if not records.is_contiguous():
+        raise ValueError("records must be contiguous")
+    if not records.is_cuda or (device is not None and records.device != device):
+        raise ValueError(f"records must be on the plan's device ({device}), not {records.device}")
 
 
 class DeviceLayout:

@@ -154,10 +154,33 @@ class NeighborList:
 
 def assemble_forces(forces: np.ndarray, n: int, padding_image: np.ndarray) -> np.ndarray:
     """Fold gradient rows of padding atoms onto their contributing atoms
-    (neighbor.py:227-263): returns (n, 3)."""
+    (neighbor.py:227-263): returns (n, 3), rounded as the reference rounds it.
+
+    The reference has two summation orders.  With fewer contributing atoms
+    than padding rows it adds, to every atom, numpy's axis-0 sum of its image
+    rows -- a sequential sum from +0 in ascending padding order (+0 for an atom
+    without images, so a -0.0 component becomes +0.0).  Otherwise it adds the
+    padding rows one at a time in padding order.  Both are kept: the first as
+    one vectorised step per image rank (the k-th image of every atom at once),
+    the second with np.add.at, which applies repeated indices in order."""
     forces = np.asarray(forces)
     padding_image = np.asarray(padding_image, dtype=np.intp)
     total = np.array(forces[:n])
-    if padding_image.size:
-        np.add.at(total, padding_image, forces[n:n + padding_image.size])
+    n_pad = padding_image.size
+    if n_pad == 0:
+        return total
+    pad = forces[n:n + n_pad]
+    if n < n_pad:
+        order = np.argsort(padding_image, kind="stable")  # each atom's images, ascending
+        img = padding_image[order]
+        counts = np.bincount(img, minlength=n)
+        first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        rank = np.arange(n_pad) - first[img]  # k: the k-th image of its atom
+        s = np.zeros((n, 3), dtype=total.dtype)
+        for k in range(int(counts.max())):
+            sel = rank == k
+            s[img[sel]] += pad[order[sel]]
+        total += s
+    else:
+        np.add.at(total, padding_image, pad)
     return total

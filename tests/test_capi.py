@@ -113,3 +113,23 @@ def test_cpp_drop_in_caller_links_against_the_library():
     assert out.returncode == 0
     assert "undefined symbol" not in out.stdout + out.stderr
     assert "not found" not in out.stdout
+    # clone_empty (src/SymmetryFunctions.cpp:410-427) keeps the width, species,
+    # cutoffs and descriptor count (host only; its zeroed parameters are checked
+    # through the clone's descriptors in the GPU drop-in test)
+    r = subprocess.run([exe, "--clone-check"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+
+
+def test_record_buffer_checks():
+    """Plan.set_records hands the C side only an address and a count: the
+    Python side refuses any buffer the adjoint could overrun or misread."""
+    import torch
+    ok = torch.empty(12, dtype=torch.float64)
+    with pytest.raises(TypeError):
+        _capi._check_records(np.zeros(12), None)
+    with pytest.raises(ValueError, match="float64"):
+        _capi._check_records(torch.empty(12, dtype=torch.float32), None)
+    with pytest.raises(ValueError, match="contiguous"):
+        _capi._check_records(torch.empty((12, 2), dtype=torch.float64)[:, 0], None)
+    with pytest.raises(ValueError, match="device"):
+        _capi._check_records(ok, None)  # host memory

@@ -343,9 +343,16 @@ def test_cpp_drop_in_caller(c1, tmp_path):
     out = np.fromfile(outp, dtype=np.float64)
     desc = out[: n * 24].reshape(n, 24)
     grad = out[n * 24: n * 24 + nt * 3].reshape(nt, 3)
-    single = out[n * 24 + nt * 3:]
+    single = out[n * 24 + nt * 3: n * 24 + nt * 3 + 24]
+    clone = out[n * 24 + nt * 3 + 24:].reshape(n, 24)
     assert elem_rel_err(desc, c1["desc"], 1e-300) <= TOL64
     assert elem_rel_err(single, c1["desc"][0], 1e-300) <= TOL64
+    # clone_empty (src/SymmetryFunctions.cpp:410-427): the same tables with every
+    # parameter zero -- g2 rows become fc(r), g4 rows 2 fc fc fc per triplet
+    types, sizes, params = behler()
+    Oz = orc.OracleSF(["Si"], 4.5, types, sizes, np.zeros(len(params)))
+    dz = Oz.compute(n, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"])
+    assert elem_rel_err(clone, dz, 1e-300) <= TOL64
     O = oracle_of()
     G0 = O.gradient(n, c1["species"], c1["neighlist"], c1["numneigh"], c1["coords"],
                     c1["dE_dzeta"])

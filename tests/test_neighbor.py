@@ -118,7 +118,52 @@ def test_neighborlist_class_and_fold(c1):
     np.testing.assert_array_equal(nlist, c1["neighlist"])
     np.testing.assert_array_equal(nlc.get_species_code({"Si": 0}), c1["species"])
     f = np.random.default_rng(0).normal(size=(len(nlc.get_coords()), 3))
-    np.testing.assert_allclose(ld.assemble_forces(f, 64, nlc.get_padding_image()),
-                               nbl.assemble_forces(f, 64, nlc.get_padding_image()), rtol=1e-13)
+    # 64 atoms, more padding rows: the per-atom np.sum branch, bitwise
+    assert len(nlc.get_padding_image()) > 64
+    np.testing.assert_array_equal(ld.assemble_forces(f, 64, nlc.get_padding_image()),
+                                  nbl.assemble_forces(f, 64, nlc.get_padding_image()))
     with pytest.raises(ld.NeighborListError):
         nlc.get_numneigh_and_neighlist_1D(request_padding=True)
+
+
+def _signed_forces(rng, rows):
+    f = rng.normal(size=(rows, 3)) * 10.0 ** rng.integers(-6, 6, size=(rows, 3))
+    f[rng.random((rows, 3)) < 0.2] = -0.0
+    return f
+
+
+def test_numpy_axis0_sum_is_sequential_from_plus_zero():
+    """The reference's fold (neighbor.py:245-253) adds np.sum(rows, axis=0);
+    the oracle restates it with the same call, and the device fold sums each
+    atom's image rows one after another from +0.  Pin that the two agree
+    bitwise: numpy's axis-0 sum of an (m, 3) array is the sequential sum from
+    +0, the sign of zero included (a lone -0.0 sums to +0.0)."""
+    rng = np.random.default_rng(3)
+    for m in list(range(0, 40)) + [100, 1000]:
+        x = _signed_forces(rng, m)
+        seq = np.zeros(3)
+        for k in range(m):
+            seq = seq + x[k]
+        s = np.sum(x, axis=0)
+        assert np.array_equal(s, seq) and np.array_equal(np.signbit(s), np.signbit(seq)), m
+    assert not np.signbit(np.sum(np.array([[-0.0, -0.0, -0.0]]), axis=0)).any()
+
+
+@pytest.mark.parametrize("n,n_pad", [(7, 60), (60, 7), (3, 500), (20, 20), (5, 0)])
+def test_fold_branches_bitwise_and_placement(n, n_pad):
+    """Both branches of assemble_forces (neighbor.py:245-256: per-atom np.sum
+    when n < n_padding, row by row otherwise): the product's host fold equals
+    the oracle bitwise (signed zeros included), and on integer-valued forces,
+    where every order is exact, both equal np.add.at -- which pins where each
+    padding row is added."""
+    rng = np.random.default_rng(n * 1000 + n_pad)
+    img = rng.integers(0, n, n_pad).astype(np.int32)
+    f = _signed_forces(rng, n + n_pad)
+    a, b = ld.assemble_forces(f, n, img), nbl.assemble_forces(f, n, img)
+    np.testing.assert_array_equal(a, b)
+    assert np.array_equal(np.signbit(a), np.signbit(b))
+    fi = rng.integers(-1000, 1000, size=(n + n_pad, 3)).astype(np.float64)
+    ref = fi[:n].copy()
+    np.add.at(ref, img, fi[n:])
+    np.testing.assert_array_equal(nbl.assemble_forces(fi, n, img), ref)
+    np.testing.assert_array_equal(ld.assemble_forces(fi, n, img), ref)

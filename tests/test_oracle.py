@@ -314,6 +314,40 @@ def test_batch_omp_equals_serial_batch_walk():
     assert np.abs(G1 - G0).max() <= 1e-13 * np.abs(G0).max()
 
 
+def test_fwd_adj_omp_equals_compute_and_gradient():
+    """orc_fwd_adj_omp (the checker of the full-size GPU tests and the timed CPU
+    baseline) against the plain atom walk: orc_compute + orc_gradient
+    (src/Descriptors.cpp:121-136, 165-190) on a two-species periodic system,
+    over a CSR offsets array built from numneigh.  Descriptors bitwise (the
+    same per-atom routine, in place); dE/dr bitwise on one thread (one buffer
+    added to zero, same order) and <= 1e-13 normwise on several (per-thread
+    buffers summed in thread order)."""
+    from libdescriptor_amd import workloads as w
+    pos, cell, pbc, sp = w.random_system(600, 0.05, 1.8, seed=9)
+    L = w.layout(pos, cell, pbc, sp, 5.0, symbols=["A", "B"])
+    types, sizes, params = behler()
+    O = orc.OracleSF(["A", "B"], 5.0, types, sizes, params, species_resolved=True)
+    n = L["n_contrib"]
+    g = np.random.default_rng(4).normal(size=(n, O.width))
+    args = (L["species"], L["neighlist"], L["numneigh"], L["coords"])
+    d0 = O.compute(n, *args)
+    G0 = O.gradient(n, *args, g)
+    offs = np.concatenate([[0], np.cumsum(L["numneigh"])]).astype(np.int32)
+    d1, G1 = O.fwd_adj_omp(n, L["species"], L["neighlist"], offs, L["coords"], g, 1)
+    np.testing.assert_array_equal(d1, d0)
+    np.testing.assert_array_equal(G1, G0)
+    d4, G4 = O.fwd_adj_omp(n, L["species"], L["neighlist"], offs, L["coords"], g, 4)
+    np.testing.assert_array_equal(d4, d0)
+    assert np.abs(G4 - G0).max() <= 1e-13 * np.abs(G0).max()
+    # a prefix of the centres: only those centres' rows and terms
+    m = n // 3
+    dm, Gm = O.fwd_adj_omp(m, L["species"], L["neighlist"], offs, L["coords"], g[:m], 2)
+    np.testing.assert_array_equal(dm, d0[:m])
+    Gp = O.gradient(m, L["species"], L["neighlist"][:offs[m]], L["numneigh"][:m], L["coords"],
+                    g[:m])
+    assert np.abs(Gm - Gp).max() <= 1e-13 * np.abs(Gp).max()
+
+
 # ---------------- hyperparameter gradients ----------------
 
 @pytest.mark.parametrize("case", ["one_species", "two_species_resolved"])
