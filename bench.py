@@ -297,11 +297,10 @@ def main():
 def bench_layout(L, dev):
     """The input layout of the same system built on the GPU (padding rows and
     the KLIFF-order neighbour list of the contributing atoms, ldsf_layout_create)
-    against the host producers: the product's C++ (OpenMP) and, where
-    oracle/_ref was built, the reference's own single-threaded code."""
+    against the product's host producer (C++, OpenMP).  The reference's own
+    single-threaded producers are timed in the build container, not here."""
     import torch
     from libdescriptor_amd import _capi, workloads as w
-    from oracle import ref
     n = L["n_contrib"]
     pos = np.ascontiguousarray(L["coords"][:n])
     sp = np.ascontiguousarray(L["species"][:n])
@@ -326,15 +325,7 @@ def bench_layout(L, dev):
     out = {"rows": D.n_contrib + D.n_padding, "edges": D.n_neigh, "device_ms": dev_ms,
            "equal_to_host_producer": bool(same), "host_product_ms": host_ms,
            "host_threads": min(len(os.sched_getaffinity(0)), 16)}
-    if ref.available():
-        t = time.perf_counter()
-        pc, _, _ = ref.create_paddings(5.0, cell, pbc, pos, sp)
-        need = np.r_[np.ones(n), np.zeros(len(pc))].astype(np.int32)
-        ref.neighlist(np.concatenate([pos, pc]), 5.0, need, n)
-        out["reference_ms"] = (time.perf_counter() - t) * 1e3
-        out["reference_threads"] = 1
-    log(f"layout: device {dev_ms:.1f} ms (equal to host: {same}), host C++ {host_ms:.0f} ms"
-        + (f", reference {out['reference_ms']:.0f} ms" if "reference_ms" in out else ""))
+    log(f"layout: device {dev_ms:.1f} ms (equal to host: {same}), host C++ {host_ms:.0f} ms")
     return out
 
 

@@ -13,9 +13,19 @@ Fixtures (data only: inputs and the reference's outputs):
   sf_g2g4.params / sf_two_species.params  parameter files, and
   params_ref.npz  widths and descriptors the reference's file reader yields
                   (including its carried-over parameter vector).
+  layout_digests.json  SHA-256 digests of the reference producers' outputs
+                  (nbl_create_paddings: padding coordinates, species, images;
+                  nbl_build + flattening: neighbour counts and list) for the
+                  262,144-atom two-species system (rc = 5.0) and five random
+                  cells (rc = 4.1: cubic, triclinic, one open direction,
+                  cluster, a cell smaller than the cutoff), with digests of
+                  their inputs, so that GPU tests compare bit for bit with the
+                  reference without its build.
 
 Run from the repo root:  python tests/golden/make_golden.py
 """
+import hashlib
+import json
 import os
 import sys
 
@@ -54,6 +64,43 @@ def padded(pos, cell, species, rc, pbc=(1, 1, 1)):
     need = np.r_[np.ones(len(pos)), np.zeros(len(pc))].astype(np.int32)
     nn, nl = ref.neighlist(coords, rc, need, len(pos))
     return coords, allsp, pi, nn, nl
+
+
+# the random cells of the layout tests: (cell, pbc); atoms rng.random((nat, 3)) @ cell
+# with rng = default_rng(case), nat = 40 (6 for the last), species rng.integers(0, 2)
+LAYOUT_CELLS = [
+    (np.eye(3) * 9.0, [1, 1, 1]),
+    (np.array([[8.0, 0, 0], [2.5, 7.5, 0], [1.0, -1.5, 9.0]]), [1, 1, 1]),
+    (np.eye(3) * 7.0, [1, 0, 1]),
+    (np.eye(3) * 6.0, [0, 0, 0]),
+    (np.eye(3) * 3.1, [1, 1, 1]),
+]
+
+
+def layout_cell_case(case):
+    cell, pbc = LAYOUT_CELLS[case]
+    rng = np.random.default_rng(case)
+    nat = 40 if case < 4 else 6
+    pos = rng.random((nat, 3)) @ cell
+    sp = rng.integers(0, 2, nat).astype(np.int32)
+    return pos, cell, np.array(pbc, np.int32), sp
+
+
+def digest(a, dtype):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=dtype).tobytes()).hexdigest()
+
+
+def layout_digests(pos, cell, pbc, sp, rc):
+    """The reference's padding rows and neighbour list of the contributing
+    atoms, as digests (little-endian float64 / int32 bytes)."""
+    pc, ps, pi = ref.create_paddings(rc, cell, np.asarray(pbc, np.int32), pos, sp)
+    need = np.r_[np.ones(len(pos)), np.zeros(len(pc))].astype(np.int32)
+    nn, nl = ref.neighlist(np.concatenate([pos, pc]), rc, need, len(pos))
+    return {"positions": digest(pos, "<f8"), "species_in": digest(sp, "<i4"),
+            "n_padding": int(len(pc)), "n_neigh": int(len(nl)),
+            "padding_coords": digest(pc, "<f8"), "padding_species": digest(ps, "<i4"),
+            "padding_image": digest(pi, "<i4"), "numneigh": digest(nn, "<i4"),
+            "neighlist": digest(nl, "<i4")}
 
 
 def write_params(path, species_pairs, types, sizes, params, note):
@@ -124,6 +171,14 @@ def main():
                         numneigh=nn2, neighlist=nl2, desc_file=desc2)
     np.savez_compressed(os.path.join(HERE, "params_ref.npz"), width_two_species=Rf.width,
                         width_g2g4=Rg.width, desc_g2g4_file=desc3)
+
+    # ---- layout digests ----
+    from libdescriptor_amd import workloads
+    dig = {"cells": [layout_digests(*layout_cell_case(k), 4.1) for k in range(len(LAYOUT_CELLS))]}
+    pos4, cell4, pbc4, sp4 = workloads.random_system(262144, 0.05, 1.8, seed=0)
+    dig["system_262144"] = layout_digests(pos4, cell4, pbc4, sp4, 5.0)
+    with open(os.path.join(HERE, "layout_digests.json"), "w") as f:
+        json.dump(dig, f, indent=1)
     print("wrote fixtures to", HERE)
 
 
