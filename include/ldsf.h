@@ -252,6 +252,35 @@ LDSF_API int ldsf_plan_kernel_family(const ldsf_plan *plan);
 LDSF_API int ldsf_plan_set_kernel_family(ldsf_plan *plan, int family);
 
 /* ------------------------------------------------------------------------
+ * Allocation-free gradient (no reference counterpart).  For callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  ldsf_gradient's semantics
+ * (Descriptor::gradient: d_d_coordinates[n_total][3] += J^T d_dE_dzeta and,
+ * when d_desc is not NULL, d_desc[n_atoms][width] += the descriptors) on
+ * device pointers in the reference's layouts, centres = rows 0..n_atoms-1.
+ * It calls no hipMalloc, hipFree or allocating library routine: the
+ * parameter tables, CSR offsets, the transposed neighbour list, the edge
+ * records and flags live in one static device array of
+ * LDSF_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held until the process
+ * ends.  A call needs about 32 (n_neigh + n_atoms) + 12 n_total + 8 n_atoms
+ * bytes of it plus the parameter tables (64 MiB: about 1.9 M neighbour-list
+ * entries); a larger input is refused with LDSF_ERR_UNSUPPORTED before any
+ * device work.  n_neigh = sum(number_of_neighs) and max_neighbors bounds every
+ * number_of_neighs[i]; both are checked on the device, with the species codes
+ * and neighbour indices, before any kernel reads through them: a rejected
+ * call (LDSF_ERR_ARG) leaves d_desc and d_d_coordinates unchanged.  Work is
+ * ordered on `stream`; the call synchronises it once at the end, to return
+ * the device-side checks.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU race. */
+#define LDSF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+LDSF_API int ldsf_gradient_noalloc(ldsf_sf *sf, int n_atoms, int n_total, int n_neigh,
+                                   int max_neighbors, const int *d_species,
+                                   const int *d_neighbor_list, const int *d_number_of_neighs,
+                                   const double *d_coordinates, double *d_d_coordinates,
+                                   double *d_desc, const double *d_dE_dzeta, int precision,
+                                   void *stream);
+
+/* ------------------------------------------------------------------------
  * Serving loop (no reference counterpart).  Requests of descriptors + dE/dr
  * over device-resident inputs are admitted in arrival order into lanes --
  * device buffers for one batch each, sized by the config, as many lanes as

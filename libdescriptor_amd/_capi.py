@@ -77,6 +77,8 @@ _PROTOS = {
     "ldsf_layout_copy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_layout_assemble_forces": (_i, [_vp, _vp, _vp, _vp]),
     "ldsf_assemble_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
+    "ldsf_gradient_noalloc": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
+                                   _vp]),
     "ldsf_server_create": (_i, [_vp, _vp, _vp]),
     "ldsf_server_num_lanes": (_i, [_vp]),
     "ldsf_server_submit": (_i, [_vp, _vp, _vp]),
