@@ -85,11 +85,13 @@ struct ldsf_sf {
     int n_params = 0;
     std::vector<double> a2, b2, zeta3, lam3, eta3, coef3;
     ldsf::PackedTable packed;  // built by finalize_params
-    // per-device uploads
+    // per-device uploads, each freed by its own release hook (set by the
+    // device runtime, so that this host-side type needs no HIP)
     struct Dev {
         int device = -1;
         void *mem = nullptr;
         ldsf::SFTab tab{};
+        void (*release)(int device, void *mem) = nullptr;
     };
     std::vector<Dev> dev;
     ~ldsf_sf();

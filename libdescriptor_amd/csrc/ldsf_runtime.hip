@@ -62,17 +62,14 @@ struct DevBuf {
 
 }  // namespace
 
-ldsf_sf::~ldsf_sf()
+// frees a parameter table uploaded to `device` (ldsf_sf::Dev::release)
+static void release_table(int device, void *mem)
 {
-    for (auto &d : dev) {
-        if (d.mem) {
-            int cur = 0;
-            (void)hipGetDevice(&cur);
-            (void)hipSetDevice(d.device);
-            (void)hipFree(d.mem);
-            (void)hipSetDevice(cur);
-        }
-    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    (void)hipFree(mem);
+    (void)hipSetDevice(cur);
 }
 
 namespace ldsf {
@@ -134,6 +131,7 @@ int device_table(const ldsf_sf *csf, SFTab *out)
     d.device = device;
     d.mem = mem;
     d.tab = table_view(sf, dd, di);
+    d.release = release_table;
     sf->dev.push_back(d);
     *out = d.tab;
     return LDSF_OK;
@@ -481,8 +479,6 @@ std::vector<int> batch_centers(int n_conf, const int *n_atoms, const int *ptr, i
 }  // namespace
 
 extern "C" {
-
-LDSF_API void ldsf_sf_destroy(ldsf_sf *sf) { delete sf; }
 
 LDSF_API int ldsf_plan_create(ldsf_plan **out, const ldsf_sf *sf, int n_centers,
                               const int *d_centers, int n_total, const int *d_species,

@@ -106,6 +106,12 @@ bool parse_bools(const std::string &line, std::vector<bool> &out, int n)
 
 }  // namespace
 
+ldsf_sf::~ldsf_sf()
+{
+    for (auto &d : dev)
+        if (d.mem && d.release) d.release(d.device, d.mem);
+}
+
 namespace ldsf {
 
 int fail(int code, const std::string &msg)
@@ -352,6 +358,8 @@ LDSF_API int ldsf_sf_create_from_file(ldsf_sf **out, const char *path)
     *out = sf;
     return LDSF_OK;
 }
+
+LDSF_API void ldsf_sf_destroy(ldsf_sf *sf) { delete sf; }
 
 LDSF_API int ldsf_sf_width(const ldsf_sf *sf) { return sf ? sf->width : -1; }
 
