@@ -197,26 +197,27 @@ def main():
         print(json.dumps({"only": args.only, **res[args.only]}), flush=True)
         return
 
-    # FP32-bulk error against FP64, per block
+    # FP32-bulk error against FP64: descriptors per block, dE/dr overall and
+    # with dE/dzeta restricted to one block at a time (libdescriptor_amd.errstat)
+    from libdescriptor_amd import errstat
     d64, g64 = outs["fp64"]
     d32, g32 = outs["fp32_bulk"]
-    blocks = {"g2_A": (0, 8), "g2_B": (8, 16), "g4_AA": (16, 32), "g4_AB": (32, 48),
-              "g4_BB": (48, 64)}
-    err = {}
-    for b, (lo, hi) in blocks.items():
-        a, r = d32[:, lo:hi], d64[:, lo:hi]
-        scale = np.abs(r).max()
-        diff = np.abs(a - r)
-        sig = np.abs(r) >= 1e-6 * scale  # entries that carry signal in this block
-        rel = diff[sig] / np.abs(r)[sig]
-        nz = np.abs(r) > 0
-        err[b] = {"max_rel": float(rel.max()) if rel.size else 0.0,
-                  "rms_rel": float(np.sqrt((rel ** 2).mean())) if rel.size else 0.0,
-                  "max_rel_normwise": float(diff.max() / scale) if scale > 0 else 0.0,
-                  "max_rel_all_nonzero": float((diff[nz] / np.abs(r)[nz]).max()) if nz.any() else 0.0}
-    gscale = np.abs(g64).max()
-    err["dE_dr"] = {"max_rel_normwise": float(np.abs(g32 - g64).max() / gscale),
-                    "rms_rel_normwise": float(np.sqrt(((g32 - g64) ** 2).mean()) / gscale)}
+    err = {"descriptors": errstat.descriptor_errors(d32, d64, errstat.BLOCKS_TWO_SPECIES),
+           "dE_dr": errstat.gradient_errors(g32, g64), "dE_dr_per_block": {}}
+    for b, (lo, hi) in errstat.BLOCKS_TWO_SPECIES.items():
+        gb = torch.zeros_like(g)
+        gb[:, lo:hi] = g[:, lo:hi]
+        gx = {}
+        for prec in (_capi.FP64, _capi.FP32_BULK):
+            desc.zero_()
+            dx.zero_()
+            plan.forward_gradient(x, gb, desc, dx, prec)
+            gx[prec] = dx.cpu().numpy()
+        err["dE_dr_per_block"][b] = errstat.gradient_errors(gx[_capi.FP32_BULK], gx[_capi.FP64])
+    log("FP32-bulk vs FP64: descriptors max_rel (signal) %.2e, dE/dr normwise %.2e, "
+        "elementwise on components >= 10%% / 1%% of the largest %.2e / %.2e"
+        % (errstat.worst(err["descriptors"], "max_rel"), err["dE_dr"]["max_rel_normwise"],
+           errstat.elem_at(err["dE_dr"], 0.1), errstat.elem_at(err["dE_dr"], 0.01)))
 
     # algorithmic FLOPs from exact counts of this input
     Nv, tests, active = triplet_counts(L, 5.0, dev)
@@ -502,9 +503,24 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": 2 * n * 8, "launches": 5}), flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(L, types, sizes, params, g):
     """The oracle's FP64 forward + adjoint (reference loop structure, per-row
-    recomputation kept) on the host cores, over a bounded sample of centres."""
+    recomputation kept) on the host cores, over bounded samples of centres:
+    on the GPU's share of host threads (16 per GPU on these machines, capped
+    by the cores this process may use) and on one thread, the reference's own
+    mode (its OpenMP is commented out, src/Descriptors.cpp:10,466)."""
     from oracle import orc
     O = orc.OracleSF(["A", "B"], 5.0, types, sizes, params, species_resolved=True)
     try:
@@ -514,25 +530,35 @@ def cpu_baseline(L, types, sizes, params, g):
     threads = max(1, min(cores, 16))
     off = np.r_[0, np.cumsum(L["numneigh"])].astype(np.int32)
     n = L["n_contrib"]
-    # calibrate on growing prefixes until one takes >= 1 s, then size the
-    # sample for ~10 s
-    k = 512
-    while True:
+
+    def timed(k, nt):
         t = time.perf_counter()
-        O.fwd_adj_omp(k, L["species"], L["neighlist"], off, L["coords"], g, threads)
-        dt = time.perf_counter() - t
-        if dt >= 1.0 or k >= n:
-            break
-        k = min(n, k * 4)
-    per = dt / k
-    k = int(min(n, max(512, 10.0 / max(per, 1e-9))))
-    t = time.perf_counter()
-    O.fwd_adj_omp(k, L["species"], L["neighlist"], off, L["coords"], g, threads)
-    dt = time.perf_counter() - t
-    log(f"cpu baseline: {k} centres in {dt:.2f} s on {threads} threads")
+        O.fwd_adj_omp(k, L["species"], L["neighlist"], off, L["coords"], g, nt)
+        return time.perf_counter() - t
+
+    def rate(nt, seconds):
+        # grow a prefix until one takes >= 1 s, then size the sample for `seconds`
+        k = 256
+        while True:
+            dt = timed(k, nt)
+            if dt >= 1.0 or k >= n:
+                break
+            k = min(n, k * 4)
+        k = int(min(n, max(256, seconds * k / dt)))
+        dt = timed(k, nt)
+        return k, dt
+
+    k, dt = rate(threads, 10.0)
+    k1, dt1 = rate(1, 6.0)
+    log(f"cpu baseline: {k} centres in {dt:.2f} s on {threads} threads; {k1} centres in "
+        f"{dt1:.2f} s on 1 thread")
     return {"value": k / dt, "unit": "atom-envs/s", "cores": threads, "kind": "port",
             "sample": f"first {k} of {n} centres of the same workload, FP64 forward + adjoint, "
-                      f"OpenMP over centres"}
+                      f"OpenMP over centres",
+            "single_thread_value": k1 / dt1, "single_thread_sample": f"first {k1} centres",
+            "nproc": os.cpu_count(), "usable_cores": cores, "cpu_model": cpu_model(),
+            "threads_note": "16 host threads per GPU is this machine's share (capped by the "
+                            "usable cores); the reference itself is single-threaded"}
 
 
 if __name__ == "__main__":

@@ -10,8 +10,9 @@ FP64.  Checks, on a 3,000-atom two-species system (rc = 5 A):
 
   * g2 rows with Rs != 0 and eta up to 1 (exponents <= 16: the wave kernels):
     FP32-bulk descriptors <= 1e-5 relative on the entries that carry signal
-    and dE/dr <= 1e-5 (elementwise with the floor of libdescriptor_amd.errstat,
-    and normwise); the FP64 form <= 1e-12 of the oracle;
+    and dE/dr <= 1e-5 normwise and elementwise on the components >= 10 % of
+    the largest (libdescriptor_amd.errstat); the FP64 form <= 1e-12 of the
+    oracle;
   * eta = 4 A^-2 g2 rows (exponents up to 100, past FP32's underflow at 87) and
     eta = 0.5 g4 rows: the FP32-bulk call equals, bitwise, the same call with
     the general kernels forced, and meets the same bounds.
@@ -91,3 +92,4 @@ def test_fp32_bulk_against_oracle(case):
               f"dE/dr {ge}", flush=True)
         assert errstat.worst(de, "max_rel") <= 1e-5
         assert ge["max_rel_normwise"] <= 1e-5
+        assert errstat.elem_at(ge, 0.1) <= 1e-5

@@ -7,7 +7,8 @@ Tolerances: FP64 descriptors elementwise relative <= 1e-12 against the oracle
 and <= 1e-14 against the separate forward kernel (the fused kernel forms
 B^zeta as B^(zeta-1) B); FP64 gradients normwise <= 1e-12; FP32-bulk
 descriptors elementwise relative <= 1e-5 on entries carrying signal
-(|ref| >= 1e-6 of the block maximum), gradients normwise <= 1e-5."""
+(|ref| >= 1e-6 of the block maximum), gradients normwise <= 1e-5 and, at
+full size, elementwise <= 1e-5 on the components >= 10 % of the largest."""
 import ctypes
 
 import numpy as np
@@ -242,6 +243,13 @@ def test_fused_full_size_262144():
     d32, G32 = S.fused(1)
     assert signal_rel(d32, d64, BLOCKS) <= TOL32
     assert rel_err(G32, G64) <= TOL32
+    # dE/dr elementwise on the components at least 10 % of the largest (a
+    # component that cancels to far below the largest forces carries the
+    # absolute error of its FP32 terms; the floors below are reported, not bounded)
+    from libdescriptor_amd import errstat
+    ge = errstat.gradient_errors(G32, G64)
+    print(f"FP32-bulk dE/dr vs FP64: {ge}", flush=True)
+    assert errstat.elem_at(ge, 0.1) <= TOL32
     # size-independent properties at full size, both forms: translation
     # invariance (every row, padding included: sum of dE/dr = 0), and
     # linearity in dE/dzeta, exact for a power-of-two scale
