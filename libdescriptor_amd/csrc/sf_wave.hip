@@ -155,16 +155,83 @@ __device__ __forceinline__ void wave_reduce(double (&v)[K], int s)
     Halve<K, K / 2, kW / 2>::run(v, s);
 }
 
-// sums the K values over the wave and adds them into the centre's row staged
-// in LDS (each slot of a row is reached exactly once per centre)
-template <int K>
-__device__ __forceinline__ void reduce_add(double (&v)[K], int s, const int *slots, double *lrow)
+// ceil(c / d) for wave-uniform 0 <= c + d - 1 < 4096 and 1 <= d <= 64 without
+// a division: floor(x m_d / 2^20) = floor(x / d) there, m_d = ceil(2^20 / d)
+// (checked exhaustively over that range; x m_d < 2^32)
+struct Rcp20 {
+    unsigned m[65];
+};
+
+constexpr Rcp20 make_rcp20()
 {
-    wave_reduce<K>(v, s);
-    constexpr int sh = 6 - Log2<K>::v;
-    if ((s & ((1 << sh) - 1)) == 0) {
-        const int slot = slots[s >> sh];
-        if (slot >= 0) lrow[slot] += v[0];
+    Rcp20 t{};
+    for (int d = 1; d <= 64; ++d) t.m[d] = (1u << 20) / d + ((1u << 20) % d != 0 ? 1u : 0u);
+    return t;
+}
+
+__device__ const Rcp20 kRcp20 = make_rcp20();
+
+__device__ __forceinline__ int ceil_div_small(int c, int d)
+{
+    return (int)(((unsigned)(c + d - 1) * kRcp20.m[d]) >> 20);
+}
+
+// The g2 rows' sums over a centre's compacted neighbours, in FP64: terms
+// holds 8 terms per neighbour (row q of neighbour a at 8 a + q, FP32 or FP64).
+// Lane s adds, in list order, row q = s & 7 over one part of its block:
+// with two species blocks (neighbours [0, n0) and [n0, M), species-sorted)
+// block (s >> 3) & 1 in 4 parts by s >> 4; with one block all M neighbours in
+// 8 parts by s >> 3.  The parts are then added across lanes (lane bits 3, 4
+// and 5) in a fixed order, every lane of a group ending with the same total,
+// and the first lane of each (block, row) adds it into the centre's row.
+template <int NSB, typename T>
+__device__ __forceinline__ void g2_rows_sum(const T *terms, int M, int n0, int s, const int *slots,
+                                            double *lrow)
+{
+    const int q = s & 7;
+    int lo = 0, hi = M, part;
+    if constexpr (NSB == 2) {
+        const bool b1 = (s & 8) != 0;
+        lo = b1 ? n0 : 0;
+        hi = b1 ? M : n0;
+        part = s >> 4;
+    } else {
+        part = s >> 3;
+    }
+    constexpr int kParts = NSB == 2 ? 4 : 8;
+    const int len = hi - lo;
+    const int a0 = lo + ((len * part) / kParts), a1 = lo + ((len * (part + 1)) / kParts);
+    // four loads in flight per step; the padding adds +0.0, which leaves the
+    // sum of the non-negative terms bitwise unchanged
+    double sum = 0.0;
+    for (int a = a0; a < a1; a += 4) {
+        T t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = a + k < a1 ? terms[8 * (a + k) + q] : (T)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sum += (double)t[k];
+    }
+    if constexpr (NSB == 1) {
+        // lane ^ 8 within its 16-lane row: DPP row_ror:8
+        const unsigned long long u = __double_as_longlong(sum);
+        const int lo32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x128, 0xF, 0xF, false);
+        const int hi32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false);
+        sum += __longlong_as_double((long long)(((unsigned long long)(unsigned)hi32 << 32) |
+                                                (unsigned)lo32));
+    }
+    {
+        double a = sum, b = sum;
+        xswap<16>(a, b);
+        sum = a + b;
+    }
+    {
+        double a = sum, b = sum;
+        xswap<32>(a, b);
+        sum = a + b;
+    }
+    if (s < (NSB == 2 ? 16 : 8)) {
+        const int slot = slots[NSB == 2 ? (s & 15) : q];
+        if (slot >= 0) lrow[slot] += sum;
     }
 }
 
@@ -341,10 +408,16 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 // base, so one address per neighbour index serves every array through the
 // LDS instructions' immediate offsets; the pair list (tmax entries) follows.
 // (stride 65, not 64: arrays read or updated together fall in different banks)
+// The adjoint kinds put the row weights (w, wv), the centre's dE/dzeta row (g)
+// and, fused, its descriptor row (d) first, so that the triplet loop reaches
+// every weight through a ds_read2 immediate offset (at most 2040 bytes) from
+// one per-lane address.
 constexpr int kStr = kW + 1;
+constexpr int kHeadAdj = 48 + 48 + kWaveSlots;                 // w, wv, g
+constexpr int kHeadFused = kHeadAdj + kWaveSlots;              // ... and d
 constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
-constexpr int kAdjDoubles = 11 * kStr + 1 + 48 + 48 + kWaveSlots;  // px..fz, pad, w, wv, g
-constexpr int kFusedDoubles = kAdjDoubles + kWaveSlots;        // ... and d
+constexpr int kAdjDoubles = kHeadAdj + 11 * kStr + 1;          // w, wv, g, px..fz, pad
+constexpr int kFusedDoubles = kHeadFused + 11 * kStr + 1;      // w, wv, g, d, px..fz, pad
 
 // kernel kinds: the forward, the adjoint, and the fused forward + adjoint
 enum { kFwd = 0, kAdj = 1, kFused = 2 };
@@ -354,10 +427,17 @@ __host__ __device__ constexpr int kind_doubles(int kind)
     return kind == kFwd ? kFwdDoubles : (kind == kAdj ? kAdjDoubles : kFusedDoubles);
 }
 
-__host__ __device__ inline size_t group_bytes(int tmax, int kind)
+// The pair list's region also holds, between the pair phase and the pair
+// enumeration, the g2 terms of every compacted neighbour (8 per neighbour,
+// `term_bytes` = the compute type's size each) for the kinds that form the
+// descriptors.
+__host__ __device__ inline size_t group_bytes(int maxn, int kind, int term_bytes)
 {
-    size_t b = sizeof(double) * kind_doubles(kind) + sizeof(int) * (kind == kFwd ? 1 : 2) * kW +
-               sizeof(unsigned short) * tmax;
+    const int pairs = maxn * (maxn - 1) / 2;
+    size_t list = sizeof(unsigned short) * (size_t)(pairs > 1 ? pairs : 1);
+    const size_t terms = (size_t)8 * term_bytes * maxn;
+    if (kind != kAdj && terms > list) list = terms;
+    size_t b = sizeof(double) * kind_doubles(kind) + sizeof(int) * (kind == kFwd ? 1 : 2) * kW + list;
     return (b + 15) & ~size_t(15);
 }
 
@@ -376,27 +456,31 @@ __device__ __forceinline__ Shared carve_shared(double *base, int ns)
 __device__ __forceinline__ Grp carve_group(double *base, int ns, int kind)
 {
     const bool adjoint = kind != kFwd;
-    double *p = base + shared_doubles(ns);
+    double *p = base + shared_doubles(ns);  // 16-byte aligned
     Grp L;
-    L.px = p;
-    L.py = p + kStr;
-    L.pz = p + 2 * kStr;
-    L.r2 = p + 3 * kStr;
-    L.r = p + 4 * kStr;
-    L.rinv = p + 5 * kStr;
-    L.fc = p + 6 * kStr;
+    double *q = p;
     if (adjoint) {
-        L.dfc = p + 7 * kStr;
-        L.fx = p + 8 * kStr;
-        L.fy = p + 9 * kStr;
-        L.fz = p + 10 * kStr;
-        L.w = p + 11 * kStr + 1;  // 16-byte aligned: read as float4
-        L.wv = L.w + 48;
-        L.g = L.w + 96;
-        L.d = kind == kFused ? p + kAdjDoubles : nullptr;
+        L.w = p;  // read as float4 / double2
+        L.wv = p + 48;
+        L.g = p + 96;
+        L.d = kind == kFused ? p + kHeadAdj : nullptr;
+        q = p + (kind == kFused ? kHeadFused : kHeadAdj);  // even: 16-byte aligned
+    }
+    L.px = q;
+    L.py = q + kStr;
+    L.pz = q + 2 * kStr;
+    L.r2 = q + 3 * kStr;
+    L.r = q + 4 * kStr;
+    L.rinv = q + 5 * kStr;
+    L.fc = q + 6 * kStr;
+    if (adjoint) {
+        L.dfc = q + 7 * kStr;
+        L.fx = q + 8 * kStr;
+        L.fy = q + 9 * kStr;
+        L.fz = q + 10 * kStr;
     } else {
         L.dfc = L.fx = L.fy = L.fz = L.w = L.wv = L.g = nullptr;
-        L.d = p + 7 * kStr;
+        L.d = q + 7 * kStr;
     }
     int *ip = reinterpret_cast<int *>(p + kind_doubles(kind));
     L.sp = ip;
@@ -576,6 +660,37 @@ __device__ __forceinline__ void squares_m1(const float (&sq)[5], float (&mk)[5])
     for (int k = 1; k < 5; ++k) mk[k] = k <= ZGrid<ZG>::nsq ? mk[k - 1] * sq[k - 1] : 0.0f;
 }
 
+// The powers of the power-of-two zeta grids for the FP64 form: B^zeta_z
+// (pw) or B^(zeta_z - 1) (pm) from the same squaring chain, multiplied in the
+// same order as pow_bits (so bitwise its results), and 0 for B <= 0 as the
+// reference's guard (src/SymmetryFunctions.cpp:121).
+template <int ZG>
+__device__ __forceinline__ void grid_powers64(double b, double (&pw)[4])
+{
+    const double bc = b > 0.0 ? b : 0.0;
+    double sq[5];
+    sq[0] = bc;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) sq[k] = k <= ZGrid<ZG>::nsq ? sq[k - 1] * sq[k - 1] : 0.0;
+#pragma unroll
+    for (int z = 0; z < 4; ++z) pw[z] = sq[ZGrid<ZG>::e[z]];
+}
+
+template <int ZG>
+__device__ __forceinline__ void grid_powers64_m1(double b, double (&pm)[4])
+{
+    const double bc = b > 0.0 ? b : 0.0;
+    double sq[5], mk[5];
+    sq[0] = bc;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) sq[k] = k <= ZGrid<ZG>::nsq ? sq[k - 1] * sq[k - 1] : 0.0;
+    mk[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) mk[k] = k <= ZGrid<ZG>::nsq ? mk[k - 1] * sq[k - 1] : 0.0;
+#pragma unroll
+    for (int z = 0; z < 4; ++z) pm[z] = b > 0.0 ? mk[ZGrid<ZG>::e[z]] : 0.0;
+}
+
 // the species-pair index of two-species block p (0-0, 0-1, 1-1: 0, 1, 3)
 __device__ __forceinline__ int blk_pair(int p) { return p + (p >> 1); }
 
@@ -669,15 +784,14 @@ __device__ __forceinline__ float base_f(const Fast &F, int l, double sp, double 
 
 // The g2 rows of one neighbour at distance r in the FP32-bulk form, with
 // phi_q = exp(-eta_q (r - Rs_q)^2) fc(r).  ADJ: returns sum_q dE/dzeta_q
-// d(phi_q)/dr (weights gw[8 sj + q], FP32, in LDS).  FWD: sets the terms
-// phi_q into the neighbour's species block of acc (the other block 0).  The
-// exponent is formed in FP32: relative error <= |eta (r - Rs)^2| * 1e-7 per
-// term, the largest terms having the smallest exponents.  Rows past the set's
-// g2 rows have eta 0, weight 0 and no slot.
+// d(phi_q)/dr (weights gw[8 sj + q], FP32, in LDS).  FWD: writes the terms
+// phi_q to term[0..8) (16-byte aligned LDS).  The exponent is formed in FP32:
+// relative error about 3e-7 |eta (r - Rs)^2| per term, the largest terms
+// having the smallest exponents.  Rows past the set's g2 rows have eta 0,
+// weight 0 and no slot.
 template <int NSB, bool ADJ, bool FWD>
 __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *gw, float r,
-                                             float fc, float dfc, int sj,
-                                             double (&acc)[NSB * 8])
+                                             float fc, float dfc, int sj, float *term)
 {
     float gq[8];
     if (ADJ) {
@@ -689,24 +803,18 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
             gq[2 * k + 1] = v.y;
         }
     }
-    // two species: the term goes to the neighbour's block as v * 1, the other
-    // block gets v * 0 = +0 (v >= 0), both exact
-    const double m1 = NSB == 1 ? 1.0 : (double)sj, m0 = 1.0 - m1;
-    float sum = 0.0f;
+    float sum = 0.0f, t[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const float dr = r - wp.rs2f[q];
         const float ex = __builtin_amdgcn_exp2f(-wp.e2l[q] * (dr * dr));
         if (ADJ) sum = fmaf(gq[q], ex * fmaf(wp.m2e[q] * dr, fc, dfc), sum);
-        if (FWD) {
-            const double v = (double)(ex * fc);
-            if (NSB == 1) {
-                acc[q] = v;
-            } else {
-                acc[q] = v * m0;
-                acc[8 + q] = v * m1;
-            }
-        }
+        t[q] = ex * fc;
+    }
+    if (FWD) {
+        float4 *T4 = reinterpret_cast<float4 *>(term);
+        T4[0] = make_float4(t[0], t[1], t[2], t[3]);
+        T4[1] = make_float4(t[4], t[5], t[6], t[7]);
     }
     return sum;
 }
@@ -982,7 +1090,10 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
 }
 
 // ZG: the zeta grid of the FP32-bulk form (ZGrid)
-template <int P, int NSB, int NPB, int ZG>
+// DC: the launch uses DevCounts (the serving loop's device-side sizes,
+// per-request flags and per-centre rows); without, every DevCounts branch
+// and pointer is compiled out.
+template <int P, int NSB, int NPB, int ZG, bool DC>
 __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int n_centers,
                                                  const int *__restrict__ centers,
                                                  const int *__restrict__ offs,
@@ -990,10 +1101,11 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                                                  const int *__restrict__ species,
                                                  const double *__restrict__ x,
                                                  double *__restrict__ desc, int width, int maxn,
-                                                 DevCounts dc)
+                                                 DevCounts dc_in)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
+    const DevCounts dc = DC ? dc_in : DevCounts();
     extern __shared__ double lds_base[];
     const int s = threadIdx.x;
     const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
@@ -1025,9 +1137,6 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         __syncthreads();
 
         // ---- pairs ----
-        double acc2[NSB * 8];
-#pragma unroll
-        for (int k = 0; k < NSB * 8; ++k) acc2[k] = 0.0;
         bool valid = false;
         double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
         const int sj = B.sj;
@@ -1041,11 +1150,15 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         }
         const unsigned long long m = __ballot(valid);
         const int M = __popcll(m);
-        const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
+        // two species blocks: species-0 neighbours first, list order within each
+        const unsigned long long m1 = NSB == 2 ? __ballot(valid && sj == 1) : 0ull;
+        const int n1 = NSB == 2 ? __popcll(m1) : 0;
         // the enumeration's first pair codes, a phase ahead
         const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
+        T *terms = reinterpret_cast<T *>(L.l0);  // the g2 terms, until the enumeration
         if (valid) {
-            const int pos = below(m, s);
+            const int pos = NSB == 2 ? (sj ? (M - n1) + below(m1, s) : below(m & ~m1, s))
+                                     : below(m, s);
             const int k = pidx(C.si, sj, ns);
             T fc, dfc;
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
@@ -1059,7 +1172,8 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                                      (float)fc * __builtin_amdgcn_exp2f(-F.etal[0] * r2f),
                                      (float)fc * __builtin_amdgcn_exp2f(-F.etal[1] * r2f),
                                      __int_as_float(sj));
-                pair_rows_f<NSB, false, true>(wp, nullptr, (float)r, fc, 0.0f, sj, acc2);
+                pair_rows_f<NSB, false, true>(wp, nullptr, (float)r, fc, 0.0f, sj,
+                                              reinterpret_cast<float *>(terms) + 8 * pos);
             } else {
                 L.px[pos] = B.p0;
                 L.py[pos] = B.p1;
@@ -1069,23 +1183,20 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 L.rinv[pos] = 1.0 / r;
                 L.fc[pos] = (double)fc;
                 L.sp[pos] = sj;
-            }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                if (P == 0 && q < wt.nr2) {
-                    const double dr = r - wp.rs2[q];
-                    const double v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
-                    if (NSB == 1) {
-                        acc2[q] += v;
-                    } else {  // the neighbour's species block gets v, the other 0
-                        acc2[q] += sj ? 0.0 : v;
-                        acc2[8 + q] += sj ? v : 0.0;
+                for (int q = 0; q < 8; ++q) {
+                    double v = 0.0;
+                    if (q < wt.nr2) {
+                        const double dr = r - wp.rs2[q];
+                        v = (double)(dexp(-wp.eta2[q] * dr * dr, R) * fc);
                     }
+                    terms[8 * pos + q] = (T)v;
                 }
             }
         }
         const int jn = load_index(Cn, s, nl);
-        reduce_add<NSB * 8>(acc2, s, wp.slots, L.d);
+        __syncthreads();
+        g2_rows_sum<NSB>(terms, M, M - n1, s, wp.slots, L.d);
         __syncthreads();
 
         // ---- triplets ----
@@ -1106,10 +1217,12 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
             const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
             int passes = (nall + kW - 1) / kW, nc[3] = {16, 0, 0};
             if (NPB > 1 && nall > 0) {
+                // ceil(c / 4I) = ceil(ceil(c / 4) / I)
+                const int c40 = (cnt[0] + 3) >> 2, c41 = (cnt[1] + 3) >> 2, c42 = (cnt[2] + 3) >> 2;
                 auto cols_for = [&](int I) {
-                    nc[0] = (cnt[0] + 4 * I - 1) / (4 * I);
-                    nc[1] = (cnt[1] + 4 * I - 1) / (4 * I);
-                    nc[2] = (cnt[2] + 4 * I - 1) / (4 * I);
+                    nc[0] = ceil_div_small(c40, I);
+                    nc[1] = ceil_div_small(c41, I);
+                    nc[2] = ceil_div_small(c42, I);
                     return nc[0] + nc[1] + nc[2];
                 };
                 while (cols_for(passes) > 16) ++passes;
@@ -1218,7 +1331,10 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                             for (int l = 0; l < 2; ++l) {
                                 if (l < wt.nl) {
                                     T pw[4];
-                                    powers<P>(q.B[l], nsq, wp, wt.nz, pw);
+                                    if constexpr (P == 0 && ZG != 0)
+                                        grid_powers64<ZG>(q.B[l], pw);
+                                    else
+                                        powers<P>(q.B[l], nsq, wp, wt.nz, pw);
 #pragma unroll
                                     for (int z = 0; z < 4; ++z) {
                                         if (z < wt.nz) {
@@ -1287,17 +1403,18 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
 // 4-lane columns so that the row sums reduce the same way.
 // Occupancy floors (waves per SIMD): FP64 fused 2, FP32-bulk fused 3,
 // FP32-bulk adjoint 4 (128 VGPRs).
-template <int P, int NSB, int NPB, bool FWD, int ZG>
+template <int P, int NSB, int NPB, bool FWD, int ZG, bool DC>
 __global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : (P == 0 ? 1 : 4))
 void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const int *__restrict__ offs, const int *__restrict__ nl,
                 const int *__restrict__ species, const double *__restrict__ x,
                 const double *__restrict__ dE, const int *__restrict__ inv_pos,
                 double *__restrict__ edge, int n_edges, double *__restrict__ desc, int width,
-                int maxn, DevCounts dc)
+                int maxn, DevCounts dc_in)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
+    const DevCounts dc = DC ? dc_in : DevCounts();
     extern __shared__ double lds_base[];
     const int s = threadIdx.x;
     const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
@@ -1352,9 +1469,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         __syncthreads();
 
         // ---- pairs: radial part of dE/d r_j, the accumulator's start value ----
-        double acc2[NSB * 8];
-#pragma unroll
-        for (int k = 0; k < NSB * 8; ++k) acc2[k] = 0.0;
         bool valid = false;
         double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
         const int sj = B.sj;
@@ -1374,34 +1488,35 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         }
         const unsigned long long m = __ballot(valid);
         const int M = __popcll(m);
-        const int n1 = NPB > 1 ? __popcll(__ballot(valid && sj == 1)) : 0;
+        // two species blocks: species-0 neighbours first, list order within each
+        const unsigned long long m1 = NSB == 2 ? __ballot(valid && sj == 1) : 0ull;
+        const int n1 = NSB == 2 ? __popcll(m1) : 0;
         // the enumeration's first pair codes, a phase ahead
         const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
+        T *terms = reinterpret_cast<T *>(L.l0);  // FWD: the g2 terms, until the enumeration
         if (valid) {
-            const int pos = below(m, s);
+            const int pos = NSB == 2 ? (sj ? (M - n1) + below(m1, s) : below(m & ~m1, s))
+                                     : below(m, s);
             const int k = pidx(C.si, sj, ns);
             T fc, dfc;
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
             double sum = 0.0;
             if constexpr (P == 1)
-                sum = (double)pair_rows_f<NSB, true, FWD>(wp, gw, (float)r, fc, dfc, sj, acc2);
+                sum = (double)pair_rows_f<NSB, true, FWD>(wp, gw, (float)r, fc, dfc, sj,
+                                                          reinterpret_cast<float *>(terms) + 8 * pos);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                if (P == 0 && q < wt.nr2) {  // L.g[slot] needs a row of the set
-                    const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
-                    const double dr = r - wp.rs2[q];
-                    const T ex = dexp(-wp.eta2[q] * dr * dr, R);
-                    const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
-                    sum += L.g[slot] * (double)dphi;
-                    if (FWD) {  // the g2 term, as the forward kernel forms it
-                        const double v = (double)(ex * fc);
-                        if (NSB == 1) {
-                            acc2[q] += v;
-                        } else {
-                            acc2[q] += sj ? 0.0 : v;
-                            acc2[8 + q] += sj ? v : 0.0;
-                        }
+                if (P == 0) {
+                    double v = 0.0;
+                    if (q < wt.nr2) {  // L.g[slot] needs a row of the set
+                        const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
+                        const double dr = r - wp.rs2[q];
+                        const T ex = dexp(-wp.eta2[q] * dr * dr, R);
+                        const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
+                        sum += L.g[slot] * (double)dphi;
+                        v = (double)(ex * fc);  // the g2 term, as the forward kernel forms it
                     }
+                    if (FWD) terms[8 * pos + q] = (T)v;
                 }
             }
             // 1/r: FP32 (one Newton step) in the FP32-bulk form
@@ -1434,8 +1549,11 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         }
         const int jn = load_index(Cn, s, nl);
         const int ipen = s < Cn.N ? inv_pos[Cn.beg + s] : 0;
-        if (FWD) reduce_add<NSB * 8>(acc2, s, wp.slots, L.d);
         __syncthreads();
+        if (FWD) {
+            g2_rows_sum<NSB>(terms, M, M - n1, s, wp.slots, L.d);
+            __syncthreads();
+        }
 
         // ---- triplets: the active pairs of every block, lanes split over the blocks ----
         Nbr Bn;
@@ -1471,10 +1589,13 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             const int col = s & 15, rowq = s >> 4;
             if (FWD) {
                 if (NPB > 1 && nall > 0) {
+                    // ceil(c / 4I) = ceil(ceil(c / 4) / I)
+                    const int c40 = (cnt[0] + 3) >> 2, c41 = (cnt[1] + 3) >> 2,
+                              c42 = (cnt[2] + 3) >> 2;
                     auto cols_for = [&](int I) {
-                        nc[0] = (cnt[0] + 4 * I - 1) / (4 * I);
-                        nc[1] = (cnt[1] + 4 * I - 1) / (4 * I);
-                        nc[2] = (cnt[2] + 4 * I - 1) / (4 * I);
+                        nc[0] = ceil_div_small(c40, I);
+                        nc[1] = ceil_div_small(c41, I);
+                        nc[2] = ceil_div_small(c42, I);
                         return nc[0] + nc[1] + nc[2];
                     };
                     while (cols_for(passes) > 16) ++passes;
@@ -1489,9 +1610,9 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 int ln[3] = {kW, 0, 0};
                 if (NPB > 1 && nall > 0) {
                     auto lanes_for = [&](int I) {
-                        ln[0] = (cnt[0] + I - 1) / I;
-                        ln[1] = (cnt[1] + I - 1) / I;
-                        ln[2] = (cnt[2] + I - 1) / I;
+                        ln[0] = ceil_div_small(cnt[0], I);
+                        ln[1] = ceil_div_small(cnt[1], I);
+                        ln[2] = ceil_div_small(cnt[2], I);
                         return ln[0] + ln[1] + ln[2];
                     };
                     while (lanes_for(passes) > kW) ++passes;
@@ -1536,7 +1657,10 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                         for (int l = 0; l < 2; ++l) {
                             if (l < wt.nl) {
                                 T pm[4];
-                                powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
+                                if constexpr (P == 0 && ZG != 0)
+                                    grid_powers64_m1<ZG>(q.B[l], pm);
+                                else
+                                    powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
 #pragma unroll
                                 for (int z = 0; z < 4; ++z) {
                                     if (z < wt.nz) {
@@ -1659,11 +1783,11 @@ int grid_for(int n_centers, int cap)
     return std::max(g, 1);
 }
 
-size_t lds_for(int max_nn, int kind, int ns)
+size_t lds_for(int max_nn, int kind, int ns, int precision)
 {
     const int maxn = std::max(max_nn, 1);
-    const int tmax = std::max(maxn * (maxn - 1) / 2, 1);
-    return sizeof(double) * shared_doubles(ns) + group_bytes(tmax, kind);
+    return sizeof(double) * shared_doubles(ns) +
+           group_bytes(maxn, kind, precision == 0 ? (int)sizeof(double) : (int)sizeof(float));
 }
 
 }  // namespace
@@ -1672,13 +1796,13 @@ int wave_group_size(int max_nn, int n_species)
 {
     // one wavefront per centre, one lane per neighbour-list entry
     if (max_nn > kW) return 0;
-    return lds_for(max_nn, kFused, n_species) <= kWaveMaxLds ? kW : 0;
+    return lds_for(max_nn, kFused, n_species, 0) <= kWaveMaxLds ? kW : 0;
 }
 
 size_t wave_lds_bytes(int max_nn, bool adjoint, int n_species, int group)
 {
     (void)group;
-    return lds_for(max_nn, adjoint ? kAdj : kFwd, n_species);
+    return lds_for(max_nn, adjoint ? kAdj : kFwd, n_species, 0);
 }
 
 bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
@@ -1807,14 +1931,22 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (n_centers <= 0) return hipSuccess;
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const size_t lds = lds_for(max_nn, kFwd, wt.n_species);
+    const size_t lds = lds_for(max_nn, kFwd, wt.n_species, precision);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
-#define LDSF_FWD(P, NSB, NPB, ZG)                                                            \
-    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, ZG>), grid, block, lds, s, wt, n_centers,      \
+    const bool use_dc = dc.hdr || dc.req_flag || dc.out_row || dc.in_row;
+#define LDSF_FWD_DC(P, NSB, NPB, ZG, DCV)                                                     \
+    hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, ZG, DCV>), grid, block, lds, s, wt, n_centers, \
                        centers, offs, nl, species, x, desc, width, maxn, dc)
+#define LDSF_FWD(P, NSB, NPB, ZG)                  \
+    do {                                           \
+        if (use_dc) LDSF_FWD_DC(P, NSB, NPB, ZG, true);  \
+        else LDSF_FWD_DC(P, NSB, NPB, ZG, false);        \
+    } while (0)
 #define LDSF_FWD_P(NSB, NPB)                                                        \
     do {                                                                            \
-        if (precision == 0) LDSF_FWD(0, NSB, NPB, 0);                              \
+        if (precision == 0 && wt.zgrid == 1) LDSF_FWD(0, NSB, NPB, 1);             \
+        else if (precision == 0 && wt.zgrid == 2) LDSF_FWD(0, NSB, NPB, 2);        \
+        else if (precision == 0) LDSF_FWD(0, NSB, NPB, 0);                         \
         else if (wt.zgrid == 1) LDSF_FWD(1, NSB, NPB, 1);                          \
         else if (wt.zgrid == 2) LDSF_FWD(1, NSB, NPB, 2);                          \
         else LDSF_FWD(1, NSB, NPB, 0);                                             \
@@ -1822,6 +1954,7 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (wt.nsb == 1) LDSF_FWD_P(1, 1); else LDSF_FWD_P(2, 3);
 #undef LDSF_FWD_P
 #undef LDSF_FWD
+#undef LDSF_FWD_DC
     return hipGetLastError();
 }
 
@@ -1835,15 +1968,23 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
     const bool fused = desc != nullptr || dc.out_row != nullptr;
-    const size_t lds = lds_for(max_nn, fused ? kFused : kAdj, wt.n_species);
+    const size_t lds = lds_for(max_nn, fused ? kFused : kAdj, wt.n_species, precision);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
-#define LDSF_ADJ(P, NSB, NPB, FWD, ZG)                                                         \
-    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD, ZG>), grid, block, lds, s, wt, n_centers,  \
-                       centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, width, \
-                       maxn, dc)
+    const bool use_dc = dc.hdr || dc.req_flag || dc.out_row || dc.in_row;
+#define LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, DCV)                                                    \
+    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD, ZG, DCV>), grid, block, lds, s, wt,          \
+                       n_centers, centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, \
+                       width, maxn, dc)
+#define LDSF_ADJ(P, NSB, NPB, FWD, ZG)                          \
+    do {                                                        \
+        if (use_dc) LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, true);    \
+        else LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, false);          \
+    } while (0)
 #define LDSF_ADJ_P(NSB, NPB, FWD)                                \
     do {                                                         \
-        if (precision == 0) LDSF_ADJ(0, NSB, NPB, FWD, 0);       \
+        if (precision == 0 && wt.zgrid == 1) LDSF_ADJ(0, NSB, NPB, FWD, 1);       \
+        else if (precision == 0 && wt.zgrid == 2) LDSF_ADJ(0, NSB, NPB, FWD, 2);  \
+        else if (precision == 0) LDSF_ADJ(0, NSB, NPB, FWD, 0);                   \
         else if (wt.zgrid == 1) LDSF_ADJ(1, NSB, NPB, FWD, 1);   \
         else if (wt.zgrid == 2) LDSF_ADJ(1, NSB, NPB, FWD, 2);   \
         else LDSF_ADJ(1, NSB, NPB, FWD, 0);                      \
@@ -1855,6 +1996,7 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     }
 #undef LDSF_ADJ_P
 #undef LDSF_ADJ
+#undef LDSF_ADJ_DC
     return hipGetLastError();
 }
 
