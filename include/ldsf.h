@@ -409,6 +409,14 @@ LDSF_API int ldsf_layout_copy(const ldsf_layout *layout, double *d_coords, int *
  * the image rows added one by one in padding order). */
 LDSF_API int ldsf_layout_assemble_forces(const ldsf_layout *layout, const double *d_forces,
                                          double *d_out, void *stream);
+/* ldsf_plan_gather into a zeroed d_coordinates followed by
+ * ldsf_layout_assemble_forces, in one pass and bitwise the same:
+ * d_out[n_contrib][3] = each contributing atom's dE/dr with its padding rows
+ * folded on in the reference's order (neighbor.py:227-263), from the records
+ * of the plan's last adjoint call.  The plan must be over the layout's rows
+ * (ldsf_layout_arrays); the padding rows' dE/dr is never written. */
+LDSF_API int ldsf_plan_gather_fold(ldsf_plan *plan, const ldsf_layout *layout, double *d_out,
+                                   void *stream);
 /* the same fold for any padding_image (device int32 [n_padding], values in
  * [0, n_contrib)); d_forces[n_contrib + n_padding][3].  Synchronises `stream`. */
 LDSF_API int ldsf_assemble_forces(int n_contrib, int n_padding, const double *d_forces,

@@ -77,6 +77,7 @@ _PROTOS = {
     "ldsf_layout_copy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ldsf_layout_assemble_forces": (_i, [_vp, _vp, _vp, _vp]),
     "ldsf_assemble_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
+    "ldsf_plan_gather_fold": (_i, [_vp, _vp, _vp, _vp]),
     "ldsf_gradient_noalloc": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                    _vp]),
     "ldsf_server_create": (_i, [_vp, _vp, _vp]),
@@ -244,6 +245,20 @@ class Plan:
     def gather(self, dcoords, stream=None):
         """Second half of gradient(): dcoords += the per-row sums of the records."""
         check(lib().ldsf_plan_gather(self.handle, _ptr(dcoords), _stream_ptr(stream)))
+
+    def gather_fold(self, layout, out=None, stream=None):
+        """The gather and assemble_forces (neighbor.py:227-263) in one pass, for a
+        plan over a DeviceLayout's rows: returns (n_contrib, 3), bitwise the
+        gather into zeros followed by layout.assemble_forces."""
+        import torch
+        if out is None:
+            out = torch.empty((layout.n_contrib, 3), dtype=torch.float64, device=self.device)
+        if out.dtype != torch.float64 or not out.is_contiguous() or \
+                out.shape != (layout.n_contrib, 3):
+            raise ValueError("out must be a contiguous float64 tensor of shape (n_contrib, 3)")
+        check(lib().ldsf_plan_gather_fold(self.handle, layout.handle, _ptr(out),
+                                          _stream_ptr(stream)))
+        return out
 
     def __del__(self):
         h = getattr(self, "handle", None)

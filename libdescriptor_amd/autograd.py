@@ -82,10 +82,10 @@ class _FromPositionsFn(torch.autograd.Function):
     def backward(ctx, grad_desc):
         g = grad_desc.to(torch.float64).contiguous()
         lay = ctx.lay
-        drows = torch.zeros_like(lay.coords)
-        ctx.plan.gradient(lay.coords, g, drows, ctx.precision)
-        # padding rows are their atoms shifted by lattice vectors: fold them back
-        return lay.assemble_forces(drows), None
+        ctx.plan.adjoint_edges(lay.coords, g, ctx.precision)
+        # padding rows are their atoms shifted by lattice vectors: the gather
+        # folds them back as it sums each atom's records
+        return ctx.plan.gather_fold(lay), None
 
 
 class SymmetryFunctionsFromPositions(torch.nn.Module):
@@ -95,9 +95,9 @@ class SymmetryFunctionsFromPositions(torch.nn.Module):
     Each call builds the padding rows and the KLIFF-order neighbour list on the
     GPU (``ldsf_layout_create``, the device twin of NeighborList,
     python_package/libdescriptor/neighbor.py:35-154), a plan over it and the
-    forward kernel; the backward runs the adjoint kernel, the gather, and the
-    fold of the padding rows onto their atoms (assemble_forces,
-    neighbor.py:227-263).  This is the reference's NeighborList -> compute ->
+    forward kernel; the backward runs the adjoint kernel and the gather with
+    the fold of the padding rows onto their atoms (assemble_forces,
+    neighbor.py:227-263) in its epilogue.  This is the reference's NeighborList -> compute ->
     gradient -> assemble_forces pipeline as one autograd node, with nothing
     leaving the device but the layout's sizes.
 

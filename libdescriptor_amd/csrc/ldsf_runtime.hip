@@ -675,6 +675,18 @@ LDSF_API int ldsf_param_gradient(ldsf_sf *sf, int n_atoms, const int *species,
     return LDSF_OK;
 }
 
+LDSF_API int ldsf_plan_gather_fold(ldsf_plan *plan, const ldsf_layout *layout, double *d_out,
+                                   void *stream)
+{
+    if (!plan || !layout) return ldsf::fail(LDSF_ERR_ARG, "plan_gather_fold: null plan or layout");
+    if (plan->n_total != ldsf::layout_rows(layout))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gather_fold: the plan's rows are not the layout's");
+    if (plan->n_centers > 0 && !d_out) return ldsf::fail(LDSF_ERR_ARG, "plan_gather_fold: null output");
+    if (!plan->edges_ready && plan->n_edges + plan->n_centers > 0)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gather_fold: no adjoint records (run the adjoint first)");
+    return ldsf::layout_gather_fold(layout, plan->in_start, plan->edge, d_out, (hipStream_t)stream);
+}
+
 LDSF_API int ldsf_plan_gather(ldsf_plan *plan, double *d_dcoords, void *stream)
 {
     if (!plan || (plan->n_total > 0 && !d_dcoords))

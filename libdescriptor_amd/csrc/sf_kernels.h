@@ -6,6 +6,8 @@
 
 #include "ldsf_internal.h"
 
+struct ldsf_layout;
+
 namespace ldsf {
 
 // Sizes of a launch read on the device, for launches replayed from a graph
@@ -129,6 +131,14 @@ hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, 
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
                          hipStream_t s, const int *skip = nullptr);
 hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s);
+
+// sf_layout.hip: a device layout's rows, and the gather of adjoint records
+// (row r's records at in_start[r] .. in_start[r+1]) folded onto its
+// contributing atoms (d_out[n_contrib][3]), as the gather into a zeroed
+// d_coordinates followed by ldsf_layout_assemble_forces, bitwise
+int layout_rows(const ldsf_layout *layout);
+int layout_gather_fold(const ldsf_layout *layout, const int *in_start, const double *edge,
+                       double *d_out, hipStream_t s);
 hipError_t launch_scan_int(int n, const int *in, int *out_exclusive_plus_total, void *tmp,
                            size_t *tmp_bytes, hipStream_t s);
 

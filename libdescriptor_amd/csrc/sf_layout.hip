@@ -35,6 +35,8 @@
 
 #include "../../include/ldsf.h"
 #include "ldsf_internal.h"
+#include "sf_device.h"
+#include "sf_kernels.h"
 
 namespace {
 
@@ -511,6 +513,48 @@ int build_fold(int n, int n_pad, const int *d_image, FoldTable &ft, hipStream_t 
     return LDSF_OK;
 }
 
+// The gather and the fold in one pass: atom k's dE/dr is the sum of its own
+// row's adjoint records (ascending edge order, as k_gather sums it into a
+// zeroed row) folded with its padding rows' sums in the reference's order,
+// as k_fold folds them -- bitwise the two launches' result, without writing
+// and re-reading the padding rows.
+__global__ void k_gather_fold(int n, int pairwise, const int *__restrict__ fold_start,
+                              const int *__restrict__ fold_rows, const int *__restrict__ in_start,
+                              const double *__restrict__ edge, double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    ldsf::row_sum3(edge, in_start[k], in_start[k + 1], t0, t1, t2);
+    const int b = fold_start[k], e = fold_start[k + 1];
+    if (pairwise) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int q = b; q < e; ++q) {
+            const long r = (long)n + fold_rows[q];
+            double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+            ldsf::row_sum3(edge, in_start[r], in_start[r + 1], f0, f1, f2);
+            s0 += f0;
+            s1 += f1;
+            s2 += f2;
+        }
+        t0 += s0;
+        t1 += s1;
+        t2 += s2;
+    } else {
+        for (int q = b; q < e; ++q) {
+            const long r = (long)n + fold_rows[q];
+            double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+            ldsf::row_sum3(edge, in_start[r], in_start[r + 1], f0, f1, f2);
+            t0 += f0;
+            t1 += f1;
+            t2 += f2;
+        }
+    }
+    out[3 * (long)k] = t0;
+    out[3 * (long)k + 1] = t1;
+    out[3 * (long)k + 2] = t2;
+}
+
 int run_fold(int n, int n_pad, const FoldTable &ft, const double *d_forces, double *d_out,
              hipStream_t s)
 {
@@ -778,6 +822,22 @@ int make_neighbors(ldsf_layout *L, int n_total, int n_query, double infl, hipStr
 
 }  // namespace
 
+namespace ldsf {
+int layout_gather_fold(const ldsf_layout *L, const int *in_start, const double *edge,
+                       double *d_out, hipStream_t s)
+{
+    const int n = L->n_contrib;
+    if (n <= 0) return LDSF_OK;
+    const int pairwise = (L->n_pad > 0 && n < L->n_pad) ? 1 : 0;
+    hipLaunchKernelGGL(k_gather_fold, dim3(grid_for(n)), dim3(kBlock), 0, s, n, pairwise,
+                       L->fold.start, L->fold.rows, in_start, edge, d_out);
+    HIP_TRY(hipGetLastError(), "gather fold");
+    return LDSF_OK;
+}
+
+int layout_rows(const ldsf_layout *L) { return L->n_contrib + L->n_pad; }
+}  // namespace ldsf
+
 extern "C" {
 
 LDSF_API int ldsf_layout_create(ldsf_layout **out, int n_contrib, double cutoff, const double *cell,
@@ -841,6 +901,7 @@ LDSF_API int ldsf_layout_copy(const ldsf_layout *layout, double *d_coords, int *
     HIP_TRY(cp(d_neighlist, layout->neighlist, sizeof(int) * layout->n_neigh), "copy neighlist");
     return LDSF_OK;
 }
+
 
 LDSF_API int ldsf_layout_assemble_forces(const ldsf_layout *layout, const double *d_forces,
                                          double *d_out, void *stream)

@@ -180,6 +180,67 @@ def test_assemble_forces_bitwise(which, c1):
     np.testing.assert_array_equal(out2.view(np.int64), exp.view(np.int64))
 
 
+@pytest.mark.parametrize("which", ["c1", "large"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32_bulk"])
+def test_gather_fold_bitwise(which, prec, c1):
+    """plan.gather_fold(layout) equals, bitwise, the gradient into zeros followed by
+    layout.assemble_forces, for both summation orders of neighbor.py:227-263
+    (C1: n < n_padding; a 4000-atom periodic system: n >= n_padding)."""
+    import libdescriptor_amd as ld
+    from libdescriptor_amd import _capi, workloads
+    from libdescriptor_amd.workloads import behler_si_params
+    print(f"case: gather_fold {which} {prec}", flush=True)
+    d = _dev()
+    types, sizes, params = behler_si_params()
+    if which == "c1":
+        L = device_layout(c1["positions"], c1["cell"], [1, 1, 1], np.zeros(64, np.int32), 4.5)
+        ds = ld.DescriptorKind.init_descriptor(ld.AvailableDescriptors.SymmetryFunctions, ["Si"],
+                                               "cos", 4.5, types, sizes, params)
+    else:
+        pos, cell, pbc, sp = workloads.random_system(4000, 0.05, 1.8, seed=3)
+        L = device_layout(pos, cell, pbc, sp, 5.0)
+        ds = ld.DescriptorKind.init_symmetry_functions(["A", "B"], 5.0, types, sizes, params,
+                                                       species_resolved=True)
+    p = _capi.FP64 if prec == "fp64" else _capi.FP32_BULK
+    rows = L.n_contrib + L.n_padding
+    plan = _capi.Plan(ds, rows, L.species, L.neighlist, L.numneigh)
+    g = torch.tensor(np.random.default_rng(11).normal(size=(L.n_contrib, ds.width)), device=d)
+    dx = torch.zeros((rows, 3), dtype=torch.float64, device=d)
+    plan.gradient(L.coords, g, dx, p)
+    exp = L.assemble_forces(dx).cpu().numpy()
+    plan.adjoint_edges(L.coords, g, p)
+    out = plan.gather_fold(L).cpu().numpy()
+    np.testing.assert_array_equal(out.view(np.int64), exp.view(np.int64))
+    # a second fold into a caller's buffer, over stale contents
+    buf = torch.full((L.n_contrib, 3), 7.0, dtype=torch.float64, device=d)
+    assert plan.gather_fold(L, out=buf) is buf
+    np.testing.assert_array_equal(buf.cpu().numpy().view(np.int64), exp.view(np.int64))
+
+
+def test_gather_fold_rejects_mismatch(c1):
+    """A plan whose row count is not the layout's, or that has no edge records yet,
+    is refused."""
+    import libdescriptor_amd as ld
+    from libdescriptor_amd import _capi
+    from libdescriptor_amd.workloads import behler_si_params
+    print("case: gather_fold mismatch", flush=True)
+    L = device_layout(c1["positions"], c1["cell"], [1, 1, 1], np.zeros(64, np.int32), 4.5)
+    types, sizes, params = behler_si_params()
+    ds = ld.DescriptorKind.init_descriptor(ld.AvailableDescriptors.SymmetryFunctions, ["Si"],
+                                           "cos", 4.5, types, sizes, params)
+    rows = L.n_contrib + L.n_padding
+    plan = _capi.Plan(ds, rows, L.species, L.neighlist, L.numneigh)
+    with pytest.raises(_capi.LdsfError):
+        plan.gather_fold(L)  # no adjoint_edges yet
+    sp1 = torch.cat([L.species, torch.zeros(1, dtype=torch.int32, device=_dev())])
+    other = _capi.Plan(ds, rows + 1, sp1, L.neighlist, L.numneigh)
+    g = torch.zeros((64, ds.width), dtype=torch.float64, device=_dev())
+    x1 = torch.cat([L.coords, torch.zeros((1, 3), dtype=torch.float64, device=_dev())])
+    other.adjoint_edges(x1, g)
+    with pytest.raises(_capi.LdsfError):
+        other.gather_fold(L)
+
+
 def test_assemble_forces_rejects_bad_image():
     from libdescriptor_amd import _capi
     print("case: assemble_forces bad image", flush=True)
