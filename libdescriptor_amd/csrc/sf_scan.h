@@ -1,6 +1,7 @@
-// Exclusive prefix sum of int32 counts without library scratch or memset
+// Exclusive prefix sums of int32 counts without library scratch or memset
 // nodes, in three fixed-grid launches: per-block partial sums, one block
-// scanning the partials, then block-local scans offset by them.  The element
+// scanning the partials, then block-local scans offset by them; two
+// independent scans can share the three launches.  The element
 // count is read on the device (a pointer into a graph-replayed launch's
 // header) or passed by value, so the same kernels serve the serving loop's
 // replayed graphs and the allocation-free entry point.
@@ -55,21 +56,34 @@ __device__ inline int scan_block_inclusive(int v, int *total)
     return v + before;
 }
 
+// up to two independent scans in the same three launches (blockIdx.y picks
+// one): out = exclusive scan of in[0..n), out[n] = the total; part holds
+// kScanBlocks + 1 ints of each scan's partials
+struct ScanJobs {
+    Count cnt[2];
+    const int *in[2];
+    int *part[2];
+    int *out[2];
+};
+
 // part[b] = sum of block b's contiguous chunk of in[0..n)
-__global__ void k_scan_partial(Count cnt, const int *__restrict__ in, int *part)
+__global__ void k_scan_partial(ScanJobs j)
 {
-    const long n = cnt.get();
+    const int y = blockIdx.y;
+    const long n = j.cnt[y].get();
+    const int *__restrict__ in = j.in[y];
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
     const long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     int s = 0;
     for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += in[i];
     s = scan_block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) j.part[y][blockIdx.x] = s;
 }
 
-// one block: exclusive scan of part[0..nb), total into part[nb]
-__global__ void k_scan_top(int *part, int nb)
+// one block per scan: exclusive scan of part[0..nb), total into part[nb]
+__global__ void k_scan_top(ScanJobs j, int nb)
 {
+    int *part = j.part[blockIdx.y];
     int run = 0;
     for (int base = 0; base < nb; base += blockDim.x) {
         const int i = base + threadIdx.x;
@@ -83,10 +97,13 @@ __global__ void k_scan_top(int *part, int nb)
 }
 
 // out[i] = in[0] + ... + in[i-1] for i < n, out[n] = the total
-__global__ void k_scan_final(Count cnt, const int *__restrict__ in, const int *__restrict__ part,
-                             int *out)
+__global__ void k_scan_final(ScanJobs j)
 {
-    const long n = cnt.get();
+    const int y = blockIdx.y;
+    const long n = j.cnt[y].get();
+    const int *__restrict__ in = j.in[y];
+    const int *__restrict__ part = j.part[y];
+    int *out = j.out[y];
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
     const long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     int run = part[blockIdx.x];
@@ -101,13 +118,20 @@ __global__ void k_scan_final(Count cnt, const int *__restrict__ in, const int *_
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[gridDim.x];
 }
 
-// the three launches on stream s; part holds kScanBlocks + 1 ints
+// the three launches of `jobs` scans (1 or 2) on stream s
+inline void enqueue_scans(const ScanJobs &j, int jobs, hipStream_t s)
+{
+    const dim3 tb(kScanThreads), gb(kScanBlocks, jobs);
+    hipLaunchKernelGGL(k_scan_partial, gb, tb, 0, s, j);
+    hipLaunchKernelGGL(k_scan_top, dim3(1, jobs), tb, 0, s, j, kScanBlocks);
+    hipLaunchKernelGGL(k_scan_final, gb, tb, 0, s, j);
+}
+
+// one scan; part holds kScanBlocks + 1 ints
 inline void enqueue_scan(Count cnt, const int *in, int *part, int *out, hipStream_t s)
 {
-    const dim3 tb(kScanThreads), gb(kScanBlocks);
-    hipLaunchKernelGGL(k_scan_partial, gb, tb, 0, s, cnt, in, part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), tb, 0, s, part, kScanBlocks);
-    hipLaunchKernelGGL(k_scan_final, gb, tb, 0, s, cnt, in, part, out);
+    ScanJobs j{{cnt, cnt}, {in, in}, {part, part}, {out, out}};
+    enqueue_scans(j, 1, s);
 }
 
 }  // namespace

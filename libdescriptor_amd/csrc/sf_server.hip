@@ -67,12 +67,20 @@ struct Seg {
     int centre_base, row_base, edge_base, pad;
 };
 
+// a block's share of k_prepare: [begin, end) of one request's rows, centres
+// or neighbour entries
+struct Tile {
+    int r, kind, begin, end;
+};
+enum { kTileRows = 0, kTileCentres = 1, kTileEdges = 2 };
+
 // header of the table: batch sizes
-enum { H_CENTERS = 0, H_ROWS = 1, H_EDGES = 2, H_REQ = 3, H_INTS = 16 };
+enum { H_CENTERS = 0, H_ROWS = 1, H_EDGES = 2, H_REQ = 3, H_TILES = 4, H_INTS = 16 };
 
 constexpr int kThreads = 256;
+constexpr int kPerThread = 8;
+constexpr int kTile = kThreads * kPerThread;
 constexpr int kGridStride = 2048;
-constexpr int kSegBlocks = 32;     // blocks per request in the copy kernels
 constexpr int kBuckets = 9;        // wave kernels for max_nn <= 8, 16, ..., 64; general kernels
                                    // (max_nn > 64, or a set the wave kernels do not cover)
 constexpr int kGeneralBucket = 8;
@@ -83,75 +91,119 @@ __device__ __forceinline__ long gstride() { return (long)gridDim.x * blockDim.x;
 
 // flags: 1 index/species/count out of range, 2 neighbour counts do not sum
 // to n_neigh
-// count[row] += 1 for every edge and centre self-term pointing at the row (the
-// transposed list's bucket sizes; count starts at zero)
-__global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ seg, int width,
-                          int n_species, int *species, int *numneigh, int *nl, int *centres,
-                          int *req_of, int *row_req, double *x, double **out_row,
-                          const double **in_row, const double *zero_row, int *req_sum, int *flags,
-                          int *count)
+// One tile per block: up to kTile rows, centres or neighbour entries of one
+// request, so that a large request spreads over many blocks and every block
+// has enough independent loads and atomics in flight.  count[row] += 1 for
+// every edge and centre self-term pointing at the row (the transposed list's
+// bucket sizes; count starts at zero), and rank[id] = the count before that
+// edge's (id < n_edges) or self-term's (id = n_edges + centre) increment: its
+// slot in the row's bucket, in an arbitrary order that the per-row sort makes
+// ascending.
+__global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ seg,
+                          const Tile *__restrict__ tiles, int width, int n_species, int *species,
+                          int *numneigh, int *nl, int *centres, int *req_of, int *row_req,
+                          double *x, double **out_row, const double **in_row,
+                          const double *zero_row, int *req_sum, int *flags, int *count, int *rank)
 {
-    const int r = blockIdx.y;
-    if (r >= hdr[H_REQ]) return;
-    const Seg S = seg[r];
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long st = (long)gridDim.x * blockDim.x;
-    int bad = 0;
-    for (long i = t; i < S.n_total; i += st) {
-        int sp = S.species[i];
-        if (sp < 0 || sp >= n_species) {
-            bad = 1;
-            sp = 0;
+    const int nt = hdr[H_TILES];
+    const long ne = hdr[H_EDGES];
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const int r = T.r;
+        const Seg &S = seg[r];
+        int bad = 0, sum = 0;
+        if (T.kind == kTileRows) {
+            const int *sp_in = S.species;
+            const double *xin = S.coords;
+            const int rb = S.row_base;
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                const long i = T.begin + threadIdx.x + k * kThreads;
+                if (i >= T.end) break;
+                int sp = sp_in[i];
+                if (sp < 0 || sp >= n_species) {
+                    bad = 1;
+                    sp = 0;
+                }
+                const long a = rb + i;
+                species[a] = sp;
+                row_req[a] = r;
+                x[3 * a] = xin[3 * i];
+                x[3 * a + 1] = xin[3 * i + 1];
+                x[3 * a + 2] = xin[3 * i + 2];
+            }
+        } else if (T.kind == kTileCentres) {
+            const int rb = S.row_base, cb = S.centre_base, mx = S.max_nn;
+            double *dsc = S.desc;
+            const double *dE = S.dE;
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                const long i = T.begin + threadIdx.x + k * kThreads;
+                if (i >= T.end) break;
+                int nn = S.numneigh[i];
+                if (nn < 0 || nn > mx) {
+                    bad = 1;
+                    nn = 0;
+                }
+                const long c = cb + i;
+                numneigh[c] = nn;
+                centres[c] = rb + (int)i;
+                rank[ne + c] = atomicAdd(count + rb + i, 1);
+                req_of[c] = r;
+                out_row[c] = dsc + i * width;
+                in_row[c] = dE ? dE + i * width : zero_row;
+                sum += nn;
+            }
+        } else {
+            const int *nl_in = S.nl;
+            const int rb = S.row_base, eb = S.edge_base, nt_r = S.n_total;
+            // all loads first, then all atomics: kPerThread of each in flight
+            int v[kPerThread];
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                const long e = T.begin + threadIdx.x + k * kThreads;
+                v[k] = e < T.end ? nl_in[e] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                if (v[k] < 0 || v[k] >= nt_r) {
+                    bad = 1;
+                    v[k] = 0;
+                }
+            }
+            int rk[kPerThread];
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                const long e = T.begin + threadIdx.x + k * kThreads;
+                rk[k] = e < T.end ? atomicAdd(count + rb + v[k], 1) : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k) {
+                const long e = T.begin + threadIdx.x + k * kThreads;
+                if (e < T.end) {
+                    nl[eb + e] = rb + v[k];
+                    rank[eb + e] = rk[k];
+                }
+            }
         }
-        const long a = S.row_base + i;
-        species[a] = sp;
-        row_req[a] = r;
-        x[3 * a] = S.coords[3 * i];
-        x[3 * a + 1] = S.coords[3 * i + 1];
-        x[3 * a + 2] = S.coords[3 * i + 2];
-    }
-    int sum = 0;
-    for (long i = t; i < S.n_atoms; i += st) {
-        int nn = S.numneigh[i];
-        if (nn < 0 || nn > S.max_nn) {
-            bad = 1;
-            nn = 0;
+        // one atomic per wave for the neighbour-count sum
+        if (T.kind == kTileCentres) {
+            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+            if ((threadIdx.x & 63) == 0 && sum) atomicAdd(req_sum + r, sum);
         }
-        const long c = S.centre_base + i;
-        numneigh[c] = nn;
-        centres[c] = S.row_base + (int)i;
-        atomicAdd(count + S.row_base + i, 1);
-        req_of[c] = r;
-        out_row[c] = S.desc + i * width;
-        in_row[c] = S.dE ? S.dE + i * width : zero_row;
-        sum += nn;
+        if (bad) atomicOr(flags + r, 1);
     }
-    for (long e = t; e < S.n_neigh; e += st) {
-        int v = S.nl[e];
-        if (v < 0 || v >= S.n_total) {
-            bad = 1;
-            v = 0;
-        }
-        nl[S.edge_base + e] = S.row_base + v;
-        atomicAdd(count + S.row_base + v, 1);
-    }
-    // one atomic per wave for the neighbour-count sum
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-    if ((threadIdx.x & 63) == 0 && sum) atomicAdd(req_sum + r, sum);
-    if (bad) atomicOr(flags + r, 1);
 }
 
-__global__ void k_check_sums(const int *__restrict__ hdr, const Seg *__restrict__ seg,
-                             const int *__restrict__ req_sum, int *flags)
-{
-    for (int r = threadIdx.x; r < hdr[H_REQ]; r += blockDim.x)
-        if (req_sum[r] != seg[r].n_neigh) atomicOr(flags + r, 2);
-}
-
-// CSR offsets of every request from its own edge base
+// CSR offsets of every request from its own edge base; block 0 also flags
+// the requests whose neighbour counts do not sum to n_neigh
 __global__ void k_offsets(const int *__restrict__ hdr, const Seg *__restrict__ seg,
-                          const int *__restrict__ req_of, const int *__restrict__ scan, int *offs)
+                          const int *__restrict__ req_of, const int *__restrict__ scan, int *offs,
+                          const int *__restrict__ req_sum, int *flags)
 {
+    if (blockIdx.x == 0)
+        for (int r = threadIdx.x; r < hdr[H_REQ]; r += blockDim.x)
+            if (req_sum[r] != seg[r].n_neigh) atomicOr(flags + r, 2);
     const int n = hdr[H_CENTERS];
     for (long c = gtid(); c < n; c += gstride()) {
         const Seg &S = seg[req_of[c]];
@@ -162,16 +214,14 @@ __global__ void k_offsets(const int *__restrict__ hdr, const Seg *__restrict__ s
 
 // ---- transposed CSR: the edges (and centre self-terms) pointing at each row ----
 
-// count[0..n_rows] and cursor[0..n_rows) to zero,
-// with the per-request sums and flags (Q entries each): every counter a
-// batch starts from zero, in one launch
-__global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *cursor, int *req_sum,
-                            int *flags, int Q)
+// count[0..n_rows] to zero, with the per-request sums and flags (Q entries
+// each): every counter a batch starts from zero, in one launch
+__global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *req_sum, int *flags,
+                            int Q)
 {
     const int n = hdr[H_ROWS];
     for (long i = gtid(); i <= n; i += gstride()) {
         count[i] = 0;
-        if (i < n) cursor[i] = 0;
         if (i < Q) {
             req_sum[i] = 0;
             flags[i] = 0;
@@ -183,15 +233,16 @@ __global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *cursor
     }
 }
 
+// every edge and self-term id into its row's bucket, at the slot k_prepare
+// ranked it to
 __global__ void k_fill_t(const int *__restrict__ hdr, const int *__restrict__ nl,
                          const int *__restrict__ centres, const int *__restrict__ in_start,
-                         int *cursor, int *in_edges)
+                         const int *__restrict__ rank, int *in_edges)
 {
     const long ne = hdr[H_EDGES], n = ne + hdr[H_CENTERS];
     for (long e = gtid(); e < n; e += gstride()) {
         const int a = e < ne ? nl[e] : centres[e - ne];
-        const int pos = atomicAdd(cursor + a, 1);
-        in_edges[in_start[a] + pos] = (int)e;
+        in_edges[in_start[a] + rank[e]] = (int)e;
     }
 }
 
@@ -247,7 +298,7 @@ struct ldsf_lane {
     int *species = nullptr, *numneigh = nullptr, *nl = nullptr, *centres = nullptr;
     int *req_of = nullptr, *row_req = nullptr, *req_sum = nullptr, *flags = nullptr;
     int *scan = nullptr, *offs = nullptr, *count = nullptr, *in_start = nullptr;
-    int *cursor = nullptr, *in_edges = nullptr, *inv_pos = nullptr, *part = nullptr;
+    int *in_edges = nullptr, *inv_pos = nullptr, *part = nullptr;
     double *x = nullptr, *edge = nullptr, *zero_row = nullptr;
     double **out_row = nullptr;
     const double **in_row = nullptr;
@@ -269,6 +320,7 @@ struct ldsf_server {
     ldsf::WaveParams *wave_par = nullptr;
     bool wave_ok = false;
     size_t table_bytes = 0;
+    long max_tiles = 0;  // k_prepare's tiles a batch can need
     std::vector<ldsf_lane> lanes;
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_lane;
@@ -305,16 +357,25 @@ hipError_t dalloc(T **p, long n)
     return hipMalloc((void **)p, sizeof(T) * (size_t)std::max<long>(n, 1));
 }
 
+// sum over a batch's requests of ceil(n / kTile) for rows, centres and
+// entries: at most the lane's totals over kTile plus one partial tile per
+// request and kind
+long max_tiles(const ldsf_server_config &c)
+{
+    return ((long)c.lane_rows + c.lane_centers + c.lane_edges) / ldsf::kTile + 3L * c.max_requests +
+           3;
+}
+
 size_t lane_bytes(const ldsf_server_config &c, int width)
 {
     const size_t C = c.lane_centers, R = c.lane_rows, E = c.lane_edges, Q = c.max_requests;
     size_t b = 0;
     b += sizeof(int) * (R /*species*/ + C /*numneigh*/ + E /*nl*/ + C /*centres*/ + C /*req_of*/ +
                         R /*row_req*/ + 2 * Q /*sums, flags*/ + (C + 1) * 2 /*scan, offs*/ +
-                        (R + 1) * 2 /*count, in_start*/ + R /*cursor*/ + 2 * (E + C) +
-                        (ldsf::kScanBlocks + 1));
+                        (R + 1) * 2 /*count, in_start*/ + 2 * (E + C) +
+                        2 * (ldsf::kScanBlocks + 1));
     b += sizeof(double) * (3 * R + 2 * C + (size_t)width + 3 * (E + C));
-    b += ldsf::H_INTS * sizeof(int) + Q * sizeof(Seg);
+    b += ldsf::H_INTS * sizeof(int) + Q * sizeof(Seg) + sizeof(ldsf::Tile) * (size_t)max_tiles(c);
     return b;
 }
 
@@ -337,10 +398,9 @@ int lane_alloc(ldsf_server *S, ldsf_lane &L)
     SRV_TRY(dalloc(&L.offs, C + 1), "hipMalloc");
     SRV_TRY(dalloc(&L.count, R + 1), "hipMalloc");
     SRV_TRY(dalloc(&L.in_start, R + 1), "hipMalloc");
-    SRV_TRY(dalloc(&L.cursor, R), "hipMalloc");
     SRV_TRY(dalloc(&L.in_edges, E + C), "hipMalloc");
     SRV_TRY(dalloc(&L.inv_pos, E + C), "hipMalloc");
-    SRV_TRY(dalloc(&L.part, ldsf::kScanBlocks + 1), "hipMalloc");
+    SRV_TRY(dalloc(&L.part, 2 * (ldsf::kScanBlocks + 1)), "hipMalloc");
     SRV_TRY(dalloc(&L.x, 3 * R), "hipMalloc");
     SRV_TRY(dalloc(&L.out_row, C), "hipMalloc");
     SRV_TRY(dalloc(&L.in_row, C), "hipMalloc");
@@ -359,7 +419,7 @@ void lane_free(ldsf_lane &L)
     for (auto &g : L.exec)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {L.d_table, L.species, L.numneigh, L.nl, L.centres, L.req_of, L.row_req,
-                   L.req_sum, L.flags, L.scan, L.offs, L.count, L.in_start, L.cursor,
+                   L.req_sum, L.flags, L.scan, L.offs, L.count, L.in_start,
                    L.in_edges, L.inv_pos, L.part, L.x, L.edge, L.zero_row, L.out_row,
                    (void *)L.in_row};
     for (void *p : dev)
@@ -379,23 +439,31 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
     const int *hdr = reinterpret_cast<const int *>(L.d_table);
     const Seg *seg = reinterpret_cast<const Seg *>(L.d_table + H_INTS * sizeof(int));
     const int Q = c.max_requests;
-    const dim3 tb(kThreads), gs(kGridStride), gseg(kSegBlocks, Q);
+    const dim3 tb(kThreads), gs(kGridStride);
+    const Tile *tiles = reinterpret_cast<const Tile *>(L.d_table + H_INTS * sizeof(int) +
+                                                       sizeof(Seg) * (size_t)Q);
     hipError_t e;
     if ((e = hipMemcpyAsync(L.d_table, L.h_table, S->table_bytes, hipMemcpyHostToDevice, s)))
         return e;
     // the transposed list's counts start at zero (k_prepare counts as it
     // copies), and so do the per-request sums and flags
-    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.cursor, L.req_sum, L.flags, Q);
-    hipLaunchKernelGGL(k_prepare, gseg, tb, 0, s, hdr, seg, S->width, S->n_species, L.species,
+    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.req_sum, L.flags, Q);
+    // the ranks go into inv_pos, which the per-row sort overwrites after the
+    // fill has read them
+    hipLaunchKernelGGL(k_prepare, gs, tb, 0, s, hdr, seg, tiles, S->width, S->n_species, L.species,
                        L.numneigh, L.nl, L.centres, L.req_of, L.row_req, L.x, L.out_row,
-                       L.in_row, L.zero_row, L.req_sum, L.flags, L.count);
-    hipLaunchKernelGGL(k_check_sums, dim3(1), tb, 0, s, hdr, seg, L.req_sum, L.flags);
-    // CSR offsets: scan of the neighbour counts, rebased per request
-    enqueue_scan(Count{hdr + H_CENTERS, 0}, L.numneigh, L.part, L.scan, s);
-    hipLaunchKernelGGL(k_offsets, gs, tb, 0, s, hdr, seg, L.req_of, L.scan, L.offs);
-    // transposed CSR: scan of the counts, fill, per-row sort with the inverse
-    enqueue_scan(Count{hdr + H_ROWS, 0}, L.count, L.part, L.in_start, s);
-    hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.cursor,
+                       L.in_row, L.zero_row, L.req_sum, L.flags, L.count, L.inv_pos);
+    // two scans in one set of launches: the neighbour counts (CSR offsets,
+    // rebased per request by k_offsets) and the transposed list's counts
+    ScanJobs sj{{Count{hdr + H_CENTERS, 0}, Count{hdr + H_ROWS, 0}},
+                {L.numneigh, L.count},
+                {L.part, L.part + kScanBlocks + 1},
+                {L.scan, L.in_start}};
+    enqueue_scans(sj, 2, s);
+    hipLaunchKernelGGL(k_offsets, gs, tb, 0, s, hdr, seg, L.req_of, L.scan, L.offs, L.req_sum,
+                       L.flags);
+    // transposed CSR: fill at the ranked slots, per-row sort with the inverse
+    hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.inv_pos,
                        L.in_edges);
     hipLaunchKernelGGL(k_sort_inverse_t, gs, tb, 0, s, hdr, L.in_start, L.in_edges, L.inv_pos);
     if ((e = hipGetLastError())) return e;
@@ -570,10 +638,23 @@ void worker_main(ldsf_server *S)
             rb += q.n_total;
             eb += q.n_neigh;
         }
+        // k_prepare's tiles: each request's rows, centres and entries in
+        // chunks of kTile
+        ldsf::Tile *tl = reinterpret_cast<ldsf::Tile *>(
+            L->h_table + ldsf::H_INTS * sizeof(int) + sizeof(Seg) * (size_t)S->cfg.max_requests);
+        int ntiles = 0;
+        for (size_t k = 0; k < L->batch.size(); ++k) {
+            const ldsf_request &q = L->batch[k]->req;
+            const int n[3] = {q.n_total, q.n_atoms, q.n_neigh};
+            for (int kind = 0; kind < 3; ++kind)
+                for (int b = 0; b < n[kind]; b += ldsf::kTile)
+                    tl[ntiles++] = {(int)k, kind, b, std::min(n[kind], b + ldsf::kTile)};
+        }
         h[ldsf::H_CENTERS] = cb;
         h[ldsf::H_ROWS] = rb;
         h[ldsf::H_EDGES] = eb;
         h[ldsf::H_REQ] = (int)L->batch.size();
+        h[ldsf::H_TILES] = ntiles;
         const int bucket = bucket_for(S, maxnn);
         int rc = lane_graph(S, *L, bucket);
         hipError_t e = hipSuccess;
@@ -655,7 +736,9 @@ LDSF_API int ldsf_server_create(ldsf_server **out, const ldsf_sf *sf,
         S->wave.par = S->wave_par;
         S->wave_ok = true;
     }
-    S->table_bytes = ldsf::H_INTS * sizeof(int) + sizeof(Seg) * (size_t)c.max_requests;
+    S->max_tiles = max_tiles(c);
+    S->table_bytes = ldsf::H_INTS * sizeof(int) + sizeof(Seg) * (size_t)c.max_requests +
+                     sizeof(ldsf::Tile) * (size_t)S->max_tiles;
     const size_t per = lane_bytes(c, S->width);
     const long n_lanes = std::min<long>(c.max_lanes, (long)(c.hbm_budget_bytes / per));
     if (n_lanes < 1) {
