@@ -26,7 +26,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["sf_general.hip", "sf_wave.hip", "sf_plan.hip", "ldsf_runtime.hip",
-               "sf_server.hip", "sf_layout.hip", "ldsf_noalloc.hip"]
+               "sf_server.hip", "sf_layout.hip", "ldsf_noalloc.hip", "sf_stage.hip"]
 CXX_SOURCES = ["sf_params.cpp", "neighbor_list.cpp", "descriptors.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.

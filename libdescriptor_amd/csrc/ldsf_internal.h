@@ -2,6 +2,7 @@
 #ifndef LDSF_INTERNAL_H_
 #define LDSF_INTERNAL_H_
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -10,6 +11,18 @@ namespace ldsf {
 // Records `msg` as the calling thread's last error and returns `code`.
 int fail(int code, const std::string &msg);
 void clear_error();
+
+// The number of coordinate rows the caller's arrays hold, for the host-pointer
+// calls this thread makes while the guard lives: a neighbour or centre index
+// at or past it is refused (LDSF_ERR_ARG) before any species or coordinate
+// row is read.  Without a guard the calls read max(index) + 1 rows, as the
+// reference does.
+long host_rows_limit();
+struct HostRowsLimit {
+    explicit HostRowsLimit(long rows);
+    ~HostRowsLimit();
+    long saved;
+};
 
 // Kinds, as SymmetryFunctions::add_descriptor numbers them
 // (src/SymmetryFunctions.cpp:47-51).
@@ -85,15 +98,19 @@ struct ldsf_sf {
     int n_params = 0;
     std::vector<double> a2, b2, zeta3, lam3, eta3, coef3;
     ldsf::PackedTable packed;  // built by finalize_params
-    // per-device uploads, each freed by its own release hook (set by the
-    // device runtime, so that this host-side type needs no HIP)
+    // per-device state, each freed by its own release hook (set by the device
+    // runtime, so that this host-side type needs no HIP): the parameter
+    // table's upload (kind 0) and the host-pointer calls' staging workspace
+    // (kind 1); dev_mu guards the list
     struct Dev {
         int device = -1;
+        int kind = 0;
         void *mem = nullptr;
         ldsf::SFTab tab{};
         void (*release)(int device, void *mem) = nullptr;
     };
     std::vector<Dev> dev;
+    std::mutex dev_mu;
     ~ldsf_sf();
 };
 

@@ -8,12 +8,14 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/ldsf.h"
 #include "ldsf_internal.h"
 #include "sf_kernels.h"
+#include "sf_stage.h"
 
 namespace {
 
@@ -108,8 +110,9 @@ int device_table(const ldsf_sf *csf, SFTab *out)
     auto *sf = const_cast<ldsf_sf *>(csf);
     int device = 0;
     HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(sf->dev_mu);
     for (auto &d : sf->dev)
-        if (d.device == device) {
+        if (d.device == device && d.kind == 0) {
             *out = d.tab;
             return LDSF_OK;
         }
@@ -161,26 +164,51 @@ struct ldsf_plan {
     double *edge = nullptr;
     bool edge_owned = true;        // false: a caller's buffer (ldsf_plan_set_records)
     ldsf::WaveParams *wave_par = nullptr;
+    bool borrowed = false;         // true: the arrays above live in a PlanArena
 
     ~ldsf_plan()
     {
         int cur = 0;
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device);
-        for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
-                        (void *)inv_pos, edge_owned ? (void *)edge : nullptr, (void *)wave_par})
-            if (p) (void)hipFree(p);
+        if (!borrowed)
+            for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
+                            (void *)inv_pos, (void *)wave_par})
+                if (p) (void)hipFree(p);
+        if (edge_owned && edge) (void)hipFree(edge);
         (void)hipSetDevice(cur);
     }
 };
 
+// Device arrays a plan can be built in instead of its own allocations: the
+// host-pointer calls build one plan per call, in one arena per descriptor
+// and device, which keeps the largest sizes asked of it.
+struct PlanArena {
+    ldsf::Grow<int> centers, offs, flags, count, in_start, in_edges, inv_pos;
+    ldsf::Grow<char> tmp;
+    ldsf::Grow<double> edge;
+    ldsf::Grow<ldsf::WaveParams> wpar;
+};
+
 namespace {
 
+// an array of the plan: from the arena when there is one, else its own
+template <typename T>
+hipError_t take(DevBuf<T> &own, ldsf::Grow<T> *arena, size_t n, T **out)
+{
+    hipError_t e = arena ? arena->reserve(n) : own.alloc(n);
+    *out = arena ? arena->p : own.p;
+    return e;
+}
+
+// With an arena, centers (NULL = identity) lives in it and the plan's arrays
+// are taken from it; without, the plan owns centers and allocates its own.
 int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_centers, int n_total,
-               const int *d_species, const int *d_nl, const int *d_numneigh, hipStream_t s)
+               const int *d_species, const int *d_nl, const int *d_numneigh, hipStream_t s,
+               PlanArena *arena = nullptr)
 {
     DevBuf<int> centers_guard;
-    centers_guard.p = owned_centers;
+    if (!arena) centers_guard.p = owned_centers;
     if (!sf) return ldsf::fail(LDSF_ERR_ARG, "plan_create: null descriptor");
     if (n_centers < 0 || n_total < 0 || (n_total > 0 && !d_species) ||
         (n_centers > 0 && !d_numneigh))
@@ -190,32 +218,35 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     auto plan = new ldsf_plan;
     std::unique_ptr<ldsf_plan> guard(plan);
     HIP_TRY(hipGetDevice(&plan->device), "hipGetDevice");
+    plan->borrowed = arena != nullptr;
     plan->sf = sf;
     plan->n_centers = n_centers;
     plan->n_total = n_total;
     plan->species = d_species;
     plan->nl = d_nl;
-    plan->centers = centers_guard.release_ownership();
+    plan->centers = arena ? owned_centers : centers_guard.release_ownership();
 
     DevBuf<int> offs, scratch_i, count;
-    HIP_TRY(offs.alloc(n_centers + 1), "hipMalloc(offsets)");
-    HIP_TRY(scratch_i.alloc(4), "hipMalloc(flags)");
-    HIP_TRY(hipMemsetAsync(scratch_i.p, 0, sizeof(int) * 4, s), "hipMemsetAsync");
+    int *offs_p = nullptr, *flags_p = nullptr, *count_p = nullptr;
+    HIP_TRY(take(offs, arena ? &arena->offs : nullptr, n_centers + 1, &offs_p), "hipMalloc(offsets)");
+    HIP_TRY(take(scratch_i, arena ? &arena->flags : nullptr, 4, &flags_p), "hipMalloc(flags)");
+    HIP_TRY(hipMemsetAsync(flags_p, 0, sizeof(int) * 4, s), "hipMemsetAsync");
     HIP_TRY(ldsf::launch_plan_validate(n_centers, plan->centers, n_total, d_numneigh, d_species,
-                                       tab.n_species, scratch_i.p, s), "validate");
+                                       tab.n_species, flags_p, s), "validate");
     size_t tmp_bytes = 0, tmp2 = 0;
-    HIP_TRY(ldsf::launch_scan_int(n_centers, d_numneigh, offs.p, nullptr, &tmp_bytes, s), "scan size");
-    HIP_TRY(ldsf::launch_scan_int(n_total, d_numneigh, offs.p, nullptr, &tmp2, s), "scan size");
+    HIP_TRY(ldsf::launch_scan_int(n_centers, d_numneigh, offs_p, nullptr, &tmp_bytes, s), "scan size");
+    HIP_TRY(ldsf::launch_scan_int(n_total, d_numneigh, offs_p, nullptr, &tmp2, s), "scan size");
     tmp_bytes = std::max(tmp_bytes, tmp2) + 256;
     DevBuf<char> tmp;
-    HIP_TRY(tmp.alloc(tmp_bytes), "hipMalloc(scan scratch)");
+    char *tmp_p = nullptr;
+    HIP_TRY(take(tmp, arena ? &arena->tmp : nullptr, tmp_bytes, &tmp_p), "hipMalloc(scan scratch)");
     // a negative count would make the scan meaningless; it is caught below
-    HIP_TRY(ldsf::launch_scan_int(n_centers, d_numneigh, offs.p, tmp.p, &tmp_bytes, s), "scan");
-    HIP_TRY(ldsf::launch_max_int(n_centers, d_numneigh, scratch_i.p + 1, s), "max");
+    HIP_TRY(ldsf::launch_scan_int(n_centers, d_numneigh, offs_p, tmp_p, &tmp_bytes, s), "scan");
+    HIP_TRY(ldsf::launch_max_int(n_centers, d_numneigh, flags_p + 1, s), "max");
     int host_flags[4] = {0, 0, 0, 0};
     int total = 0;
-    HIP_TRY(hipMemcpyAsync(host_flags, scratch_i.p, sizeof(int) * 4, hipMemcpyDeviceToHost, s), "copy flags");
-    HIP_TRY(hipMemcpyAsync(&total, offs.p + n_centers, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
+    HIP_TRY(hipMemcpyAsync(host_flags, flags_p, sizeof(int) * 4, hipMemcpyDeviceToHost, s), "copy flags");
+    HIP_TRY(hipMemcpyAsync(&total, offs_p + n_centers, sizeof(int), hipMemcpyDeviceToHost, s), "copy total");
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (host_flags[0] & 2) return ldsf::fail(LDSF_ERR_ARG, "negative number_of_neighs entry");
     if (host_flags[0] & 8) return ldsf::fail(LDSF_ERR_ARG, "centre row out of range");
@@ -226,38 +257,46 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     plan->max_nn = host_flags[1];
 
     // transposed list: count, scan, scatter, sort buckets
-    HIP_TRY(count.alloc(n_total + 1), "hipMalloc(counts)");
-    HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
-    HIP_TRY(hipMemsetAsync(scratch_i.p, 0, sizeof(int), s), "hipMemsetAsync");
+    HIP_TRY(take(count, arena ? &arena->count : nullptr, n_total + 1, &count_p), "hipMalloc(counts)");
+    HIP_TRY(hipMemsetAsync(count_p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
+    HIP_TRY(hipMemsetAsync(flags_p, 0, sizeof(int), s), "hipMemsetAsync");
     HIP_TRY(ldsf::launch_edge_targets_count(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
-                                            count.p, scratch_i.p, s), "count");
+                                            count_p, flags_p, s), "count");
     // out-of-range targets are skipped by the fill as by the count, and the
     // flag is read once the list is built
-    DevBuf<int> in_start, in_edges;
-    HIP_TRY(in_start.alloc(n_total + 1), "hipMalloc(in_start)");
-    HIP_TRY(in_edges.alloc(plan->n_edges + n_centers), "hipMalloc(in_edges)");
-    HIP_TRY(ldsf::launch_scan_int(n_total, count.p, in_start.p, tmp.p, &tmp_bytes, s), "scan");
-    HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
+    DevBuf<int> in_start, in_edges, inv_pos;
+    int *in_start_p = nullptr, *in_edges_p = nullptr, *inv_pos_p = nullptr;
+    HIP_TRY(take(in_start, arena ? &arena->in_start : nullptr, n_total + 1, &in_start_p),
+            "hipMalloc(in_start)");
+    HIP_TRY(take(in_edges, arena ? &arena->in_edges : nullptr, plan->n_edges + n_centers,
+                 &in_edges_p), "hipMalloc(in_edges)");
+    HIP_TRY(ldsf::launch_scan_int(n_total, count_p, in_start_p, tmp_p, &tmp_bytes, s), "scan");
+    HIP_TRY(hipMemsetAsync(count_p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
     HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
-                                           in_start.p, count.p, in_edges.p, s), "fill");
-    DevBuf<int> inv_pos;
-    HIP_TRY(inv_pos.alloc(plan->n_edges + n_centers), "hipMalloc(inv_pos)");
-    HIP_TRY(ldsf::launch_sort_inverse(n_total, in_start.p, in_edges.p, inv_pos.p, s),
+                                           in_start_p, count_p, in_edges_p, s), "fill");
+    HIP_TRY(take(inv_pos, arena ? &arena->inv_pos : nullptr, plan->n_edges + n_centers, &inv_pos_p),
+            "hipMalloc(inv_pos)");
+    HIP_TRY(ldsf::launch_sort_inverse(n_total, in_start_p, in_edges_p, inv_pos_p, s),
             "sort and inverse");
-    HIP_TRY(hipMemcpyAsync(host_flags, scratch_i.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flags");
+    HIP_TRY(hipMemcpyAsync(host_flags, flags_p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flags");
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (host_flags[0] & 1) return ldsf::fail(LDSF_ERR_ARG, "neighbour index out of range [0, n_total)");
-    plan->offs = offs.release_ownership();
-    plan->in_start = in_start.release_ownership();
-    plan->in_edges = in_edges.release_ownership();
-    plan->inv_pos = inv_pos.release_ownership();
+    plan->offs = arena ? offs_p : offs.release_ownership();
+    plan->in_start = arena ? in_start_p : in_start.release_ownership();
+    plan->in_edges = arena ? in_edges_p : in_edges.release_ownership();
+    plan->inv_pos = arena ? inv_pos_p : inv_pos.release_ownership();
     // wave kernels: parameter set covered and the largest neighbourhood fits in LDS
     ldsf::WaveParams wpar;
     int group = 0;
     if (ldsf::wave_build(sf, &plan->wave, &wpar) &&
         (group = ldsf::wave_group_size(plan->max_nn, plan->wave.n_species)) > 0) {
         plan->wave_group = group;
-        HIP_TRY(hipMalloc((void **)&plan->wave_par, sizeof(wpar)), "hipMalloc(wave parameters)");
+        if (arena) {
+            HIP_TRY(arena->wpar.reserve(1), "hipMalloc(wave parameters)");
+            plan->wave_par = arena->wpar.p;
+        } else {
+            HIP_TRY(hipMalloc((void **)&plan->wave_par, sizeof(wpar)), "hipMalloc(wave parameters)");
+        }
         HIP_TRY(hipMemcpy(plan->wave_par, &wpar, sizeof(wpar), hipMemcpyHostToDevice),
                 "hipMemcpy(wave parameters)");
         plan->wave.rc = tab.rc;
@@ -396,66 +435,165 @@ int host_prepare(HostCall &c)
         if (c.nl[e] < 0) return ldsf::fail(LDSF_ERR_ARG, "negative neighbour index");
         rows = std::max(rows, c.nl[e] + 1);
     }
+    const long limit = ldsf::host_rows_limit();
+    if (limit >= 0 && rows > limit) return ldsf::fail(LDSF_ERR_ARG, "neighbour or centre index out of range");
     c.n_rows = rows;
     c.n_edges = total;
     return LDSF_OK;
 }
 
+// the counts and centres of a call (O(n_centers) on the host): n_edges, and
+// n_rows as far as the centres fix it
+int host_counts(HostCall &c)
+{
+    if (!c.sf) return ldsf::fail(LDSF_ERR_ARG, "null descriptor");
+    if (c.n_centers < 0) return ldsf::fail(LDSF_ERR_ARG, "negative atom count");
+    if (c.n_centers > 0 && (!c.species || !c.numneigh || !c.coords))
+        return ldsf::fail(LDSF_ERR_ARG, "null input array");
+    long total = 0;
+    int rows = 0;
+    for (int t = 0; t < c.n_centers; ++t) {
+        if (c.numneigh[t] < 0) return ldsf::fail(LDSF_ERR_ARG, "negative number_of_neighs entry");
+        total += c.numneigh[t];
+        const int ctr = c.centers.empty() ? t : c.centers[t];
+        if (ctr < 0) return ldsf::fail(LDSF_ERR_ARG, "negative centre index");
+        rows = std::max(rows, ctr + 1);
+    }
+    if (total > 0 && !c.nl) return ldsf::fail(LDSF_ERR_ARG, "null neighbour list");
+    if (total > 0x7fffffffL) return ldsf::fail(LDSF_ERR_UNSUPPORTED, "more than 2^31-1 neighbour-list entries");
+    c.n_rows = rows;
+    c.n_edges = total;
+    return LDSF_OK;
+}
+
+// The host-pointer calls' workspace on one device: a stream, the staging
+// copies, the staged arrays and the plan arena, all kept at the largest
+// size a call has needed until the descriptor is destroyed.  `mu` makes the
+// calls that share it run one at a time.
+struct HostWork {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::unique_ptr<ldsf::Stager> st;
+    ldsf::Grow<int> species, nl, nn;
+    ldsf::Grow<double> x, desc, dE, dx;
+    PlanArena arena;
+    ~HostWork()
+    {
+        st.reset();  // waits for the DMAs still reading or writing pinned chunks
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+void release_host_work(int device, void *mem)
+{
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    delete static_cast<HostWork *>(mem);
+    (void)hipSetDevice(cur);
+}
+
+int host_work(const ldsf_sf *csf, HostWork **out)
+{
+    auto *sf = const_cast<ldsf_sf *>(csf);
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(sf->dev_mu);
+    for (auto &d : sf->dev)
+        if (d.device == device && d.kind == 1) {
+            *out = static_cast<HostWork *>(d.mem);
+            return LDSF_OK;
+        }
+    auto W = std::make_unique<HostWork>();
+    W->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking), "hipStreamCreate");
+    W->st = std::make_unique<ldsf::Stager>(W->stream);
+    ldsf_sf::Dev d;
+    d.device = device;
+    d.kind = 1;
+    d.mem = W.get();
+    d.release = release_host_work;
+    sf->dev.push_back(d);
+    *out = W.release();
+    return LDSF_OK;
+}
+
 // desc_io: [n_centers][width] (+=, or = with assign_desc) or NULL; dE +
-// dcoords_io for the adjoint
+// dcoords_io for the adjoint.  The neighbour list goes up first, with its
+// smallest and largest index taken on the way (they fix how many species
+// and coordinate rows are read); the plan is built in the workspace's
+// arena; results come back through the same pinned chunks.
 int host_run(HostCall &c, double *desc_io, const double *dE, double *dcoords_io, int precision,
              bool assign_desc = false)
 {
-    if (int rc = host_prepare(c)) return rc;
+    if (int rc = host_counts(c)) return rc;
     if (c.n_centers == 0) return LDSF_OK;
     const int W = c.sf->width;
-    hipStream_t s = nullptr;
-    DevBuf<int> d_species, d_nl, d_nn, d_centers;
-    DevBuf<double> d_x, d_desc, d_dE, d_dx;
-    HIP_TRY(d_species.alloc(c.n_rows), "hipMalloc");
-    HIP_TRY(d_x.alloc(3 * (size_t)c.n_rows), "hipMalloc");
-    HIP_TRY(d_nl.alloc(c.n_edges), "hipMalloc");
-    HIP_TRY(d_nn.alloc(c.n_centers), "hipMalloc");
-    HIP_TRY(hipMemcpy(d_species.p, c.species, sizeof(int) * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
-    HIP_TRY(hipMemcpy(d_x.p, c.coords, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
-    if (c.n_edges)
-        HIP_TRY(hipMemcpy(d_nl.p, c.nl, sizeof(int) * c.n_edges, hipMemcpyHostToDevice), "hipMemcpy");
-    HIP_TRY(hipMemcpy(d_nn.p, c.numneigh, sizeof(int) * c.n_centers, hipMemcpyHostToDevice), "hipMemcpy");
+    HostWork *H = nullptr;
+    if (int rc = host_work(c.sf, &H)) return rc;
+    std::lock_guard<std::mutex> lk(H->mu);
+    ldsf::Stager &st = *H->st;
+    hipStream_t s = H->stream;
+    if (c.n_edges > 0) {
+        HIP_TRY(H->nl.reserve(c.n_edges), "hipMalloc(neighbour list)");
+        int mn = 0, mx = 0;
+        HIP_TRY(st.h2d_minmax(H->nl.p, c.nl, c.n_edges, &mn, &mx), "upload neighbour list");
+        if (mn < 0) return ldsf::fail(LDSF_ERR_ARG, "negative neighbour index");
+        c.n_rows = std::max(c.n_rows, mx + 1);
+    }
+    const long limit = ldsf::host_rows_limit();
+    if (limit >= 0 && c.n_rows > limit)
+        return ldsf::fail(LDSF_ERR_ARG, "neighbour or centre index out of range");
+    const size_t C = c.n_centers, R = c.n_rows;
+    HIP_TRY(H->species.reserve(R), "hipMalloc(species)");
+    HIP_TRY(H->x.reserve(3 * R), "hipMalloc(coordinates)");
+    HIP_TRY(H->nn.reserve(C), "hipMalloc(neighbour counts)");
+    HIP_TRY(st.h2d(H->species.p, c.species, sizeof(int) * R), "upload species");
+    HIP_TRY(st.h2d(H->x.p, c.coords, sizeof(double) * 3 * R), "upload coordinates");
+    HIP_TRY(st.h2d(H->nn.p, c.numneigh, sizeof(int) * C), "upload neighbour counts");
+    int *d_centers = nullptr;
     if (!c.centers.empty()) {
-        HIP_TRY(d_centers.alloc(c.n_centers), "hipMalloc");
-        HIP_TRY(hipMemcpy(d_centers.p, c.centers.data(), sizeof(int) * c.n_centers,
-                          hipMemcpyHostToDevice), "hipMemcpy");
+        HIP_TRY(H->arena.centers.reserve(C), "hipMalloc(centres)");
+        HIP_TRY(st.h2d(H->arena.centers.p, c.centers.data(), sizeof(int) * C), "upload centres");
+        d_centers = H->arena.centers.p;
     }
     ldsf_plan *plan = nullptr;
-    int rc = plan_build(&plan, c.sf, c.n_centers, d_centers.release_ownership(), c.n_rows,
-                        d_species.p, d_nl.p, d_nn.p, s);
+    int rc = plan_build(&plan, c.sf, c.n_centers, d_centers, c.n_rows, H->species.p, H->nl.p,
+                        H->nn.p, s, &H->arena);
     if (rc) return rc;
     std::unique_ptr<ldsf_plan> guard(plan);
+    const size_t DW = C * (size_t)W;
     if (desc_io) {
-        HIP_TRY(d_desc.alloc((size_t)c.n_centers * W), "hipMalloc");
+        HIP_TRY(H->desc.reserve(DW), "hipMalloc(descriptors)");
         if (assign_desc)  // starts from zero on the device: no upload
-            HIP_TRY(hipMemsetAsync(d_desc.p, 0, sizeof(double) * c.n_centers * W, s), "hipMemsetAsync");
+            HIP_TRY(hipMemsetAsync(H->desc.p, 0, sizeof(double) * DW, s), "hipMemsetAsync");
         else
-            HIP_TRY(hipMemcpy(d_desc.p, desc_io, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
+            HIP_TRY(st.h2d(H->desc.p, desc_io, sizeof(double) * DW), "upload descriptors");
     }
     if (dE) {
-        HIP_TRY(d_dE.alloc((size_t)c.n_centers * W), "hipMalloc");
-        HIP_TRY(d_dx.alloc(3 * (size_t)c.n_rows), "hipMalloc");
-        HIP_TRY(hipMemcpy(d_dE.p, dE, sizeof(double) * c.n_centers * W, hipMemcpyHostToDevice), "hipMemcpy");
-        HIP_TRY(hipMemcpy(d_dx.p, dcoords_io, sizeof(double) * 3 * c.n_rows, hipMemcpyHostToDevice), "hipMemcpy");
+        HIP_TRY(H->dE.reserve(DW), "hipMalloc(dE/dzeta)");
+        HIP_TRY(H->dx.reserve(3 * R), "hipMalloc(dE/dr)");
+        HIP_TRY(st.h2d(H->dE.p, dE, sizeof(double) * DW), "upload dE/dzeta");
+        HIP_TRY(st.h2d(H->dx.p, dcoords_io, sizeof(double) * 3 * R), "upload dE/dr");
+        HIP_TRY(H->arena.edge.reserve(3 * (size_t)(plan->n_edges + plan->n_centers)),
+                "hipMalloc(edge records)");
+        plan->edge = H->arena.edge.p;
+        plan->edge_owned = false;
     }
     if (desc_io && dE)  // the reference's gradient: the forward runs too (one fused pass)
-        rc = plan_forward_gradient(plan, d_x.p, d_dE.p, d_desc.p, d_dx.p, precision, s);
+        rc = plan_forward_gradient(plan, H->x.p, H->dE.p, H->desc.p, H->dx.p, precision, s);
     else if (desc_io)
-        rc = plan_compute(plan, d_x.p, d_desc.p, precision, s);
+        rc = plan_compute(plan, H->x.p, H->desc.p, precision, s);
     else if (dE)
-        rc = plan_gradient(plan, d_x.p, d_dE.p, d_dx.p, precision, s);
-    if (rc) return rc;
-    if (desc_io)
-        HIP_TRY(hipMemcpy(desc_io, d_desc.p, sizeof(double) * c.n_centers * W, hipMemcpyDeviceToHost), "hipMemcpy");
-    if (dE)
-        HIP_TRY(hipMemcpy(dcoords_io, d_dx.p, sizeof(double) * 3 * c.n_rows, hipMemcpyDeviceToHost), "hipMemcpy");
-    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        rc = plan_gradient(plan, H->x.p, H->dE.p, H->dx.p, precision, s);
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    if (desc_io) HIP_TRY(st.d2h(desc_io, H->desc.p, sizeof(double) * DW), "download descriptors");
+    if (dE) HIP_TRY(st.d2h(dcoords_io, H->dx.p, sizeof(double) * 3 * R), "download dE/dr");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     return LDSF_OK;
 }
 

@@ -17,6 +17,7 @@
 #include "../../include/Descriptors.hpp"
 #include "../../include/SymmetryFunctions.hpp"
 #include "../../include/ldsf.h"
+#include "ldsf_internal.h"
 
 namespace py = pybind11;
 using namespace Descriptor;
@@ -50,11 +51,19 @@ py::array_t<double> zeros(std::vector<py::ssize_t> shape)
     return a;
 }
 
-// runs f with the GIL released when the descriptor is a SymmetryFunctions
+// the coordinate rows of the arrays check_layout last saw on this thread
+thread_local long g_layout_rows = -1;
+
+// runs f with the GIL released when the descriptor is a SymmetryFunctions,
+// with the row count check_layout found as the limit of every index the
+// call's host-pointer staging reads
 template <typename F>
 void run(DescriptorKind &k, F &&f)
 {
+    const long rows = g_layout_rows;
+    g_layout_rows = -1;
     if (sf_of(k)) {
+        ldsf::HostRowsLimit limit(rows);
         py::gil_scoped_release nogil;
         f();
     } else {
@@ -64,18 +73,19 @@ void run(DescriptorKind &k, F &&f)
 
 // The reference trusts its inputs (out-of-range indices are undefined
 // behaviour there); numpy arrays carry their sizes, so check the layout here
-// and raise ValueError before any work is staged.
+// and raise ValueError before any work is staged.  Neighbour indices are
+// checked against the row count while the call stages the neighbour list
+// (run passes it on), not in a separate pass here.
 void check_layout(const IArr &species, const IArr &neighbors, long n_neigh_used, const DArr &coords,
                   long n_centers, const int *centers)
 {
     long rows = coords.ndim() == 2 ? (long)coords.shape(0) : (long)coords.size() / 3;
+    g_layout_rows = -1;
     if (coords.size() != rows * 3) throw std::invalid_argument("coordinates must have shape (n, 3)");
     if ((long)species.size() < rows) throw std::invalid_argument("species must cover every coordinate row");
     if (n_neigh_used > (long)neighbors.size())
         throw std::invalid_argument("neighbor list is shorter than the sum of the neighbour counts");
-    const int *nb = neighbors.data();
-    for (long e = 0; e < n_neigh_used; ++e)
-        if (nb[e] < 0 || nb[e] >= rows) throw std::invalid_argument("neighbour index out of range");
+    g_layout_rows = rows;
     for (long t = 0; t < n_centers; ++t) {
         long c = centers ? centers[t] : t;
         if (c < 0 || c >= rows) throw std::invalid_argument("centre index out of range");
@@ -250,7 +260,9 @@ PYBIND11_MODULE(libdescriptor, m)
             if (sf_of(ds))
                 check_layout(species, neighbors, sum_counts(number_of_neighbors, n_atoms), coordinates,
                              n_atoms, nullptr);
-            auto desc = zeros({n_atoms, ds.width});
+            // the symmetry functions overwrite every entry: no zero fill
+            auto desc = sf_of(ds) ? py::array_t<double>(std::vector<py::ssize_t>{n_atoms, ds.width})
+                                  : zeros({n_atoms, ds.width});
             double *out = desc.mutable_data();
             if (auto *sf = sf_of(ds)) {
                 // a new array: overwritten on the device, no upload of its zeros
@@ -282,7 +294,8 @@ PYBIND11_MODULE(libdescriptor, m)
                 check_layout(species, neighbors, sum_counts(number_of_neighbors, total), coordinates,
                              total, ctr.data());
             }
-            auto desc = zeros({total, ds.width});
+            auto desc = sf_of(ds) ? py::array_t<double>(std::vector<py::ssize_t>{total, ds.width})
+                                  : zeros({total, ds.width});
             double *out = desc.mutable_data();
             if (auto *sf = sf_of(ds)) {
                 int rc = 0;

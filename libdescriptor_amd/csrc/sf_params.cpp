@@ -122,6 +122,16 @@ int fail(int code, const std::string &msg)
 
 void clear_error() { g_last_error.clear(); }
 
+namespace {
+thread_local long g_rows_limit = -1;
+}
+
+long host_rows_limit() { return g_rows_limit; }
+
+HostRowsLimit::HostRowsLimit(long rows) : saved(g_rows_limit) { g_rows_limit = rows; }
+
+HostRowsLimit::~HostRowsLimit() { g_rows_limit = saved; }
+
 int finalize_params(ldsf_sf *sf)
 {
     const int ns = (int)sf->species.size();

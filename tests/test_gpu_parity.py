@@ -322,6 +322,59 @@ def test_adjoint_edges_then_gather_equals_gradient():
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_host_path_workspace_reuse(precision):
+    """The host-pointer calls stage through one workspace per descriptor and
+    device, kept at the largest size used.  Calls on 32,768, 64 and 64,000-atom
+    systems in turn (the largest arrays split over several copy threads, each
+    over several pinned chunks; the last call larger than the first) equal,
+    bitwise, the device plan's kernels on the same inputs, and ldsf_compute /
+    ldsf_gradient keep the reference's += on the caller's arrays."""
+    import torch
+    from libdescriptor_amd import _capi, workloads as w
+    ds = sf_desc(("A", "B"), 5.0, resolved=True, precision=precision)
+    lib = _capi.lib()
+    P = lambda a: a.ctypes.data  # noqa: E731
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device=dev)  # noqa: E731
+    for ncell, seed in ((16, 1), (2, 2), (20, 3)):
+        pos, cell, pbc = w.diamond_si(ncell, seed=seed)
+        spc = (np.random.default_rng(seed).random(len(pos)) < 0.3).astype(np.int32)
+        L = w.layout(pos, cell, pbc, spc, 5.0, symbols=["A", "B"])
+        n = L["n_contrib"]
+        print(f"case: host workspace {n} atoms precision {precision}", flush=True)
+        sp = np.ascontiguousarray(L["species"], np.int32)
+        nl = np.ascontiguousarray(L["neighlist"], np.int32)
+        nn = np.ascontiguousarray(L["numneigh"], np.int32)
+        x = np.ascontiguousarray(L["coords"], np.float64)
+        g = np.random.default_rng(seed).normal(size=(n, 64))
+        plan = _capi.Plan(ds, len(x), t(sp), t(nl), t(nn))
+        xd, gd = t(x, torch.float64), t(g, torch.float64)
+        zd = lambda: torch.zeros((n, 64), dtype=torch.float64, device=dev)  # noqa: E731
+        zx = lambda: torch.zeros((len(x), 3), dtype=torch.float64, device=dev)  # noqa: E731
+        ref_c, ref_g, ref_fd, ref_fx = zd(), zx(), zd(), zx()
+        plan.compute(xd, ref_c, precision)
+        plan.gradient(xd, gd, ref_g, precision)
+        plan.forward_gradient(xd, gd, ref_fd, ref_fx, precision)
+        ref_c, ref_g = ref_c.cpu().numpy(), ref_g.cpu().numpy()
+        ref_fd, ref_fx = ref_fd.cpu().numpy(), ref_fx.cpu().numpy()
+        # the module: overwrite compute, gradient without a forward
+        np.testing.assert_array_equal(ld.compute(ds, n, sp, nl, nn, x), ref_c)
+        np.testing.assert_array_equal(ld.gradient(ds, n, sp, nl, nn, x, ref_c, g), ref_g)
+        # the C-ABI: the reference's gradient (forward too) into zeroed arrays
+        d0, x0 = np.zeros((n, 64)), np.zeros_like(x)
+        assert lib.ldsf_gradient(ds._handle, n, P(sp), P(nl), P(nn), P(x), P(x0), P(d0), P(g)) == 0
+        np.testing.assert_array_equal(d0, ref_fd)
+        np.testing.assert_array_equal(x0, ref_fx)
+        # += onto values already there
+        d1 = np.full((n, 64), 0.5)
+        assert lib.ldsf_compute(ds._handle, n, P(sp), P(nl), P(nn), P(x), P(d1)) == 0
+        assert rel_err(d1 - 0.5, ref_c) <= 1e-14
+        x1 = np.full_like(x, -0.25)
+        assert lib.ldsf_gradient(ds._handle, n, P(sp), P(nl), P(nn), P(x), P(x1), None, P(g)) == 0
+        assert rel_err(x1 + 0.25, ref_g) <= 1e-14
+
+
 def test_cpp_drop_in_caller(c1, tmp_path):
     """A reference-style C++ program (tests/cpp/drop_in.cpp) through
     Descriptor::compute / gradient / compute_single_atom on the GPU."""
