@@ -134,13 +134,14 @@ __device__ inline void sort_bucket_wave(int *ids, long b, int m, int lane)
 }
 
 // One wavefront's share of the transposed list's last step: rows w, w + nw,
-// ... of [0, n_rows) (w: this wave's index, nw: the waves of the grid), two
-// rows at a time.  Each row's ids[in_start[a] .. in_start[a+1]) are sorted
-// ascending and inv[id] = position written for every id.  Two rows of at
-// most 32 ids each take a 32-lane bitonic network in each half-wave; longer
-// ones take sort_bucket_wave one row after the other.
+// ... of [0, n_rows) (w: this wave's index, nw: the waves sharing the rows),
+// two rows at a time.  Each row's ids[in_start[a] .. in_start[a+1]) are
+// sorted ascending and inv[id] = base + position written for every id
+// (in_start and ids may sit in LDS: base then places them in the whole
+// list).  Two rows of at most 32 ids each take a 32-lane bitonic network in
+// each half-wave; longer ones take sort_bucket_wave one row after the other.
 __device__ inline void sort_rows_inverse(int n_rows, const int *in_start, int *ids, int *inv,
-                                         long w, long nw, int lane)
+                                         long w, long nw, int lane, long base = 0)
 {
     const int half = lane >> 5, hl = lane & 31;
     for (long a0 = 2 * w; a0 < n_rows; a0 += 2 * nw) {
@@ -161,7 +162,7 @@ __device__ inline void sort_rows_inverse(int n_rows, const int *in_start, int *i
             }
             if (hl < m) {
                 ids[b + hl] = v;
-                inv[v] = b + hl;
+                inv[v] = base + b + hl;
             }
         } else {
             for (int r = 0; r < 2; ++r) {
@@ -170,7 +171,7 @@ __device__ inline void sort_rows_inverse(int n_rows, const int *in_start, int *i
                 // the sorted ids, re-read after the sort's own stores (m > 64
                 // is sorted by lane 0 alone)
                 __threadfence_block();
-                for (int k = lane; k < mr; k += 64) inv[ids[br + k]] = br + k;
+                for (int k = lane; k < mr; k += 64) inv[ids[br + k]] = base + br + k;
             }
         }
     }

@@ -16,12 +16,14 @@
 // The graphs of a lane (one per bucket, all captured when the server is
 // created), replayed for every batch (sizes are read on the device
 // from the table's header, grids are fixed by the lane's capacity):
-//   table H2D -> prepare (copy and validate every request's inputs into the
-//   lane: rows, centres, neighbour entries offset to batch rows; per centre
-//   the addresses of its descriptor and dE/dzeta rows in the request's own
-//   buffers; the transposed list's bucket counts) -> neighbour sums check ->
-//   CSR offsets per request -> transposed CSR (scan, fill, per-row sort with
-//   the inverse) -> the fused forward +
+//   table H2D -> prepare (one block per tile of a request's rows, centres or
+//   neighbour entries: copy and validate them into the lane, offset to batch
+//   rows; per centre the addresses of its descriptor and dE/dzeta rows in the
+//   request's own buffers; the transposed list's entry counts per bucket of
+//   256 rows) -> scans of the neighbour counts and the bucket counts ->
+//   CSR offsets per request, neighbour sums check -> transposed CSR (entries
+//   scattered into their buckets, then one block per bucket: in_start, each
+//   row's ids sorted, the inverse) -> the fused forward +
 //   adjoint kernel (descriptors straight into each request's desc; the
 //   general kernels of the largest bucket run as forward, then adjoint) ->
 //   gather straight into each request's d_coordinates -> flags D2H.
@@ -75,11 +77,22 @@ struct Tile {
 enum { kTileRows = 0, kTileCentres = 1, kTileEdges = 2 };
 
 // header of the table: batch sizes
-enum { H_CENTERS = 0, H_ROWS = 1, H_EDGES = 2, H_REQ = 3, H_TILES = 4, H_INTS = 16 };
+enum { H_CENTERS = 0, H_ROWS = 1, H_EDGES = 2, H_REQ = 3, H_TILES = 4, H_BUCKETS = 5,
+       H_INTS = 16 };
 
 constexpr int kThreads = 256;
 constexpr int kPerThread = 8;
 constexpr int kTile = kThreads * kPerThread;
+// The transposed list is built bucket by bucket: a bucket is kBucketRows
+// consecutive batch rows.  A tile counts its entries per bucket in LDS
+// (kHistCap buckets: requests of up to 1M rows; past that, per entry in
+// global memory), so the global atomics are one per tile and bucket.
+constexpr int kBucketShift = 8;
+constexpr int kBucketRows = 1 << kBucketShift;
+constexpr int kHistCap = 4096;
+constexpr int kBucketThreads = 512;
+constexpr int kBucketCap = 12288;  // ids of a bucket sorted in LDS (48 KiB); more: in place
+constexpr int kBucketPer = 8;      // staged entries per thread and load round
 constexpr int kGridStride = 2048;
 constexpr int kBuckets = 9;        // wave kernels for max_nn <= 8, 16, ..., 64; general kernels
                                    // (max_nn > 64, or a set the wave kernels do not cover)
@@ -89,33 +102,46 @@ constexpr int kWaveBlocks = 4096;  // persistent grid of the wave kernels (~16 w
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ long gstride() { return (long)gridDim.x * blockDim.x; }
 
+// the LDS histogram of a tile's entries over its request's buckets
+// [b0, b0 + nbk): zeroed, counted (valid[k]: entry k is real), ready for the
+// caller after the barrier
+__device__ __forceinline__ void tile_hist(int *hist, int b0, int nbk, const int *tg, const bool *valid)
+{
+    for (int j = threadIdx.x; j < nbk; j += kThreads) hist[j] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k)
+        if (valid[k]) atomicAdd(&hist[(tg[k] >> kBucketShift) - b0], 1);
+    __syncthreads();
+}
+
 // flags: 1 index/species/count out of range, 2 neighbour counts do not sum
 // to n_neigh
 // One tile per block: up to kTile rows, centres or neighbour entries of one
 // request, so that a large request spreads over many blocks and every block
-// has enough independent loads and atomics in flight.  count[row] += 1 for
-// every edge and centre self-term pointing at the row (the transposed list's
-// bucket sizes; count starts at zero), and rank[id] = the count before that
-// edge's (id < n_edges) or self-term's (id = n_edges + centre) increment: its
-// slot in the row's bucket, in an arbitrary order that the per-row sort makes
-// ascending.
+// has enough independent loads in flight.  The centre and entry tiles add
+// their entries' counts per bucket of target rows into bcount (the entries
+// are the transposed list's: an edge points at its neighbour's row, a
+// centre's self-term at the centre's own; bcount starts at zero).
 __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ seg,
                           const Tile *__restrict__ tiles, int width, int n_species, int *species,
                           int *numneigh, int *nl, int *centres, int *req_of, int *row_req,
                           double *x, double **out_row, const double **in_row,
-                          const double *zero_row, int *req_sum, int *flags, int *count, int *rank)
+                          const double *zero_row, int *req_sum, int *flags, int *bcount)
 {
+    __shared__ int hist[kHistCap];
     const int nt = hdr[H_TILES];
-    const long ne = hdr[H_EDGES];
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const int r = T.r;
         const Seg &S = seg[r];
+        const int rb = S.row_base;
         int bad = 0, sum = 0;
+        int tg[kPerThread];
+        bool valid[kPerThread];
         if (T.kind == kTileRows) {
             const int *sp_in = S.species;
             const double *xin = S.coords;
-            const int rb = S.row_base;
 #pragma unroll
             for (int k = 0; k < kPerThread; ++k) {
                 const long i = T.begin + threadIdx.x + k * kThreads;
@@ -133,13 +159,15 @@ __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ s
                 x[3 * a + 2] = xin[3 * i + 2];
             }
         } else if (T.kind == kTileCentres) {
-            const int rb = S.row_base, cb = S.centre_base, mx = S.max_nn;
+            const int cb = S.centre_base, mx = S.max_nn;
             double *dsc = S.desc;
             const double *dE = S.dE;
 #pragma unroll
             for (int k = 0; k < kPerThread; ++k) {
                 const long i = T.begin + threadIdx.x + k * kThreads;
-                if (i >= T.end) break;
+                valid[k] = i < T.end;
+                tg[k] = rb + (int)i;
+                if (!valid[k]) continue;
                 int nn = S.numneigh[i];
                 if (nn < 0 || nn > mx) {
                     bad = 1;
@@ -148,7 +176,6 @@ __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ s
                 const long c = cb + i;
                 numneigh[c] = nn;
                 centres[c] = rb + (int)i;
-                rank[ne + c] = atomicAdd(count + rb + i, 1);
                 req_of[c] = r;
                 out_row[c] = dsc + i * width;
                 in_row[c] = dE ? dE + i * width : zero_row;
@@ -156,34 +183,38 @@ __global__ void k_prepare(const int *__restrict__ hdr, const Seg *__restrict__ s
             }
         } else {
             const int *nl_in = S.nl;
-            const int rb = S.row_base, eb = S.edge_base, nt_r = S.n_total;
-            // all loads first, then all atomics: kPerThread of each in flight
+            const int eb = S.edge_base, nt_r = S.n_total;
             int v[kPerThread];
 #pragma unroll
             for (int k = 0; k < kPerThread; ++k) {
                 const long e = T.begin + threadIdx.x + k * kThreads;
-                v[k] = e < T.end ? nl_in[e] : 0;
+                valid[k] = e < T.end;
+                v[k] = valid[k] ? nl_in[e] : 0;
             }
 #pragma unroll
             for (int k = 0; k < kPerThread; ++k) {
-                if (v[k] < 0 || v[k] >= nt_r) {
+                if (valid[k] && (v[k] < 0 || v[k] >= nt_r)) {
                     bad = 1;
                     v[k] = 0;
                 }
-            }
-            int rk[kPerThread];
-#pragma unroll
-            for (int k = 0; k < kPerThread; ++k) {
+                tg[k] = rb + v[k];
                 const long e = T.begin + threadIdx.x + k * kThreads;
-                rk[k] = e < T.end ? atomicAdd(count + rb + v[k], 1) : 0;
+                if (valid[k]) nl[eb + e] = tg[k];
             }
+        }
+        if (T.kind != kTileRows) {
+            // the tile's counts per bucket of target rows
+            const int b0 = rb >> kBucketShift;
+            const int nbk = ((rb + S.n_total - 1) >> kBucketShift) - b0 + 1;
+            if (nbk <= kHistCap) {
+                tile_hist(hist, b0, nbk, tg, valid);
+                for (int j = threadIdx.x; j < nbk; j += kThreads)
+                    if (hist[j]) atomicAdd(bcount + b0 + j, hist[j]);
+                __syncthreads();  // hist is the next tile's
+            } else {
 #pragma unroll
-            for (int k = 0; k < kPerThread; ++k) {
-                const long e = T.begin + threadIdx.x + k * kThreads;
-                if (e < T.end) {
-                    nl[eb + e] = rb + v[k];
-                    rank[eb + e] = rk[k];
-                }
+                for (int k = 0; k < kPerThread; ++k)
+                    if (valid[k]) atomicAdd(bcount + (tg[k] >> kBucketShift), 1);
             }
         }
         // one atomic per wave for the neighbour-count sum
@@ -214,44 +245,157 @@ __global__ void k_offsets(const int *__restrict__ hdr, const Seg *__restrict__ s
 
 // ---- transposed CSR: the edges (and centre self-terms) pointing at each row ----
 
-// count[0..n_rows] to zero, with the per-request sums and flags (Q entries
-// each): every counter a batch starts from zero, in one launch
-__global__ void k_zero_rows(const int *__restrict__ hdr, int *count, int *req_sum, int *flags,
-                            int Q)
+// bcount[0..n_buckets] and bcursor[0..n_buckets) to zero, with the
+// per-request sums and flags (Q entries each): every counter a batch starts
+// from zero, in one launch
+__global__ void k_zero_rows(const int *__restrict__ hdr, int *bcount, int *bcursor, int *req_sum,
+                            int *flags, int Q)
 {
-    const int n = hdr[H_ROWS];
-    for (long i = gtid(); i <= n; i += gstride()) {
-        count[i] = 0;
+    const long nb = hdr[H_BUCKETS], n = max(nb + 1, (long)Q);
+    for (long i = gtid(); i < n; i += gstride()) {
+        if (i <= nb) bcount[i] = 0;
+        if (i < nb) bcursor[i] = 0;
         if (i < Q) {
             req_sum[i] = 0;
             flags[i] = 0;
         }
     }
-    for (long i = gtid() + n + 1; i < Q; i += gstride()) {  // Q > n + 1: the rest
-        req_sum[i] = 0;
-        flags[i] = 0;
+}
+
+// Every entry of the transposed list (id: edge e, or n_edges + centre for a
+// self-term; target row) into its bucket's range [bbase[b], bbase[b + 1]) of
+// stage: each tile claims a run per bucket with one atomic, and its entries
+// take the run's slots in an arbitrary order (the per-bucket pass sorts).
+__global__ void k_scatter_t(const int *__restrict__ hdr, const Seg *__restrict__ seg,
+                            const Tile *__restrict__ tiles, const int *__restrict__ nl,
+                            const int *__restrict__ bbase, int *bcursor, int2 *stage)
+{
+    __shared__ int hist[kHistCap];
+    const int nt = hdr[H_TILES];
+    const long ne = hdr[H_EDGES];
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        if (T.kind == kTileRows) continue;
+        const Seg &S = seg[T.r];
+        const int rb = S.row_base;
+        int id[kPerThread], tg[kPerThread];
+        bool valid[kPerThread];
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const long i = T.begin + threadIdx.x + k * kThreads;
+            valid[k] = i < T.end;
+            if (T.kind == kTileCentres) {
+                id[k] = (int)(ne + S.centre_base + i);
+                tg[k] = rb + (int)i;
+            } else {
+                id[k] = (int)(S.edge_base + i);
+                tg[k] = valid[k] ? nl[S.edge_base + i] : rb;
+            }
+        }
+        const int b0 = rb >> kBucketShift;
+        const int nbk = ((rb + S.n_total - 1) >> kBucketShift) - b0 + 1;
+        if (nbk <= kHistCap) {
+            tile_hist(hist, b0, nbk, tg, valid);
+            for (int j = threadIdx.x; j < nbk; j += kThreads)
+                if (hist[j]) hist[j] = bbase[b0 + j] + atomicAdd(bcursor + b0 + j, hist[j]);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k)
+                if (valid[k]) {
+                    const int slot = atomicAdd(&hist[(tg[k] >> kBucketShift) - b0], 1);
+                    stage[slot] = make_int2(id[k], tg[k]);
+                }
+            __syncthreads();  // hist is the next tile's
+        } else {
+#pragma unroll
+            for (int k = 0; k < kPerThread; ++k)
+                if (valid[k]) {
+                    const int b = tg[k] >> kBucketShift;
+                    stage[bbase[b] + atomicAdd(bcursor + b, 1)] = make_int2(id[k], tg[k]);
+                }
+        }
     }
 }
 
-// every edge and self-term id into its row's bucket, at the slot k_prepare
-// ranked it to
-__global__ void k_fill_t(const int *__restrict__ hdr, const int *__restrict__ nl,
-                         const int *__restrict__ centres, const int *__restrict__ in_start,
-                         const int *__restrict__ rank, int *in_edges)
+// One block per bucket: its entries counted per row in LDS, in_start of its
+// rows, the ids placed per row (in LDS when they fit, else in place in
+// in_edges), each row sorted ascending with inv_pos[id] = its position, and
+// the sorted ids written out.
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_t(const int *__restrict__ hdr,
+                                                            const int2 *__restrict__ stage,
+                                                            const int *__restrict__ bbase,
+                                                            int *in_start, int *in_edges,
+                                                            int *inv_pos)
 {
-    const long ne = hdr[H_EDGES], n = ne + hdr[H_CENTERS];
-    for (long e = gtid(); e < n; e += gstride()) {
-        const int a = e < ne ? nl[e] : centres[e - ne];
-        in_edges[in_start[a] + rank[e]] = (int)e;
+    __shared__ int cnt[kBucketRows], start[kBucketRows + 1], wsum[kBucketRows / 64];
+    __shared__ int buf_l[kBucketCap];
+    const int R = hdr[H_ROWS], nb = hdr[H_BUCKETS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int lo = bbase[b], hi = bbase[b + 1], m = hi - lo;
+        const int r0 = b << kBucketShift, nr = min(kBucketRows, R - r0);
+        if (threadIdx.x < kBucketRows) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        // kBucketPer loads in flight per thread, then their counts
+        for (int base = lo; base < hi; base += kBucketThreads * kBucketPer) {
+            int row[kBucketPer];
+#pragma unroll
+            for (int k = 0; k < kBucketPer; ++k) {
+                const int i = base + threadIdx.x + k * kBucketThreads;
+                row[k] = i < hi ? stage[i].y - r0 : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kBucketPer; ++k)
+                if (row[k] >= 0) atomicAdd(&cnt[row[k]], 1);
+        }
+        __syncthreads();
+        // exclusive scan of the row counts (the first kBucketRows threads)
+        int incl = 0;
+        if (threadIdx.x < kBucketRows) {
+            incl = cnt[threadIdx.x];
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(incl, off);
+                if (lane >= off) incl += u;
+            }
+            if (lane == 63) wsum[wv] = incl;
+        }
+        __syncthreads();
+        if (threadIdx.x < kBucketRows) {
+            int add = 0;
+            for (int k = 0; k < wv; ++k) add += wsum[k];
+            const int ex = add + incl - cnt[threadIdx.x];
+            start[threadIdx.x] = ex;
+            if (threadIdx.x == kBucketRows - 1) start[kBucketRows] = add + incl;
+            if (threadIdx.x < nr) in_start[r0 + threadIdx.x] = lo + ex;
+            cnt[threadIdx.x] = 0;  // the rows' cursors from here on
+        }
+        if (b == nb - 1 && threadIdx.x == 0) in_start[R] = hi;
+        __syncthreads();
+        const bool in_lds = m <= kBucketCap;
+        int *buf = in_lds ? buf_l : in_edges + lo;
+        for (int base = lo; base < hi; base += kBucketThreads * kBucketPer) {
+            int2 q[kBucketPer];
+#pragma unroll
+            for (int k = 0; k < kBucketPer; ++k) {
+                const int i = base + threadIdx.x + k * kBucketThreads;
+                q[k] = i < hi ? stage[i] : make_int2(-1, -1);
+            }
+#pragma unroll
+            for (int k = 0; k < kBucketPer; ++k)
+                if (q[k].x >= 0) {
+                    const int r = q[k].y - r0;
+                    buf[start[r] + atomicAdd(&cnt[r], 1)] = q[k].x;
+                }
+        }
+        __threadfence_block();
+        __syncthreads();
+        sort_rows_inverse(nr, start, buf, inv_pos, wv, kBucketThreads / 64, lane, lo);
+        __threadfence_block();
+        __syncthreads();
+        if (in_lds)
+            for (int i = threadIdx.x; i < m; i += kBucketThreads) in_edges[lo + i] = buf_l[i];
+        __syncthreads();  // cnt, start and buf_l are the next bucket's
     }
-}
-
-// every row's bucket sorted, and inv_pos[edge id] = its position
-__global__ void k_sort_inverse_t(const int *__restrict__ hdr, const int *__restrict__ in_start,
-                                 int *in_edges, int *inv_pos)
-{
-    sort_rows_inverse(hdr[H_ROWS], in_start, in_edges, inv_pos, gtid() >> 6, gstride() >> 6,
-                      threadIdx.x & 63);
 }
 
 // per-row sums of the records, written (=) into the owning request's output
@@ -297,7 +441,8 @@ struct ldsf_lane {
     char *d_table = nullptr;  // header + Seg[max_req]
     int *species = nullptr, *numneigh = nullptr, *nl = nullptr, *centres = nullptr;
     int *req_of = nullptr, *row_req = nullptr, *req_sum = nullptr, *flags = nullptr;
-    int *scan = nullptr, *offs = nullptr, *count = nullptr, *in_start = nullptr;
+    int *scan = nullptr, *offs = nullptr, *in_start = nullptr;
+    int *bcount = nullptr, *bbase = nullptr, *bcursor = nullptr;  // per bucket of rows
     int *in_edges = nullptr, *inv_pos = nullptr, *part = nullptr;
     double *x = nullptr, *edge = nullptr, *zero_row = nullptr;
     double **out_row = nullptr;
@@ -366,14 +511,20 @@ long max_tiles(const ldsf_server_config &c)
            3;
 }
 
+// buckets of kBucketRows rows a lane's rows span
+long n_buckets(const ldsf_server_config &c)
+{
+    return ((long)c.lane_rows + ldsf::kBucketRows - 1) >> ldsf::kBucketShift;
+}
+
 size_t lane_bytes(const ldsf_server_config &c, int width)
 {
     const size_t C = c.lane_centers, R = c.lane_rows, E = c.lane_edges, Q = c.max_requests;
     size_t b = 0;
     b += sizeof(int) * (R /*species*/ + C /*numneigh*/ + E /*nl*/ + C /*centres*/ + C /*req_of*/ +
                         R /*row_req*/ + 2 * Q /*sums, flags*/ + (C + 1) * 2 /*scan, offs*/ +
-                        (R + 1) * 2 /*count, in_start*/ + 2 * (E + C) +
-                        2 * (ldsf::kScanBlocks + 1));
+                        (R + 1) /*in_start*/ + 3 * (n_buckets(c) + 1) /*bcount, bbase, bcursor*/ +
+                        2 * (E + C) + 2 * (ldsf::kScanBlocks + 1));
     b += sizeof(double) * (3 * R + 2 * C + (size_t)width + 3 * (E + C));
     b += ldsf::H_INTS * sizeof(int) + Q * sizeof(Seg) + sizeof(ldsf::Tile) * (size_t)max_tiles(c);
     return b;
@@ -396,7 +547,9 @@ int lane_alloc(ldsf_server *S, ldsf_lane &L)
     SRV_TRY(dalloc(&L.flags, Q), "hipMalloc");
     SRV_TRY(dalloc(&L.scan, C + 1), "hipMalloc");
     SRV_TRY(dalloc(&L.offs, C + 1), "hipMalloc");
-    SRV_TRY(dalloc(&L.count, R + 1), "hipMalloc");
+    SRV_TRY(dalloc(&L.bcount, n_buckets(c) + 1), "hipMalloc");
+    SRV_TRY(dalloc(&L.bbase, n_buckets(c) + 1), "hipMalloc");
+    SRV_TRY(dalloc(&L.bcursor, n_buckets(c) + 1), "hipMalloc");
     SRV_TRY(dalloc(&L.in_start, R + 1), "hipMalloc");
     SRV_TRY(dalloc(&L.in_edges, E + C), "hipMalloc");
     SRV_TRY(dalloc(&L.inv_pos, E + C), "hipMalloc");
@@ -419,7 +572,7 @@ void lane_free(ldsf_lane &L)
     for (auto &g : L.exec)
         if (g) (void)hipGraphExecDestroy(g);
     void *dev[] = {L.d_table, L.species, L.numneigh, L.nl, L.centres, L.req_of, L.row_req,
-                   L.req_sum, L.flags, L.scan, L.offs, L.count, L.in_start,
+                   L.req_sum, L.flags, L.scan, L.offs, L.bcount, L.bbase, L.bcursor, L.in_start,
                    L.in_edges, L.inv_pos, L.part, L.x, L.edge, L.zero_row, L.out_row,
                    (void *)L.in_row};
     for (void *p : dev)
@@ -445,27 +598,28 @@ hipError_t lane_enqueue(ldsf_server *S, ldsf_lane &L, int bucket)
     hipError_t e;
     if ((e = hipMemcpyAsync(L.d_table, L.h_table, S->table_bytes, hipMemcpyHostToDevice, s)))
         return e;
-    // the transposed list's counts start at zero (k_prepare counts as it
-    // copies), and so do the per-request sums and flags
-    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.count, L.req_sum, L.flags, Q);
-    // the ranks go into inv_pos, which the per-row sort overwrites after the
-    // fill has read them
+    // the bucket counters start at zero (k_prepare counts as it copies), and
+    // so do the per-request sums and flags
+    hipLaunchKernelGGL(k_zero_rows, gs, tb, 0, s, hdr, L.bcount, L.bcursor, L.req_sum, L.flags, Q);
     hipLaunchKernelGGL(k_prepare, gs, tb, 0, s, hdr, seg, tiles, S->width, S->n_species, L.species,
                        L.numneigh, L.nl, L.centres, L.req_of, L.row_req, L.x, L.out_row,
-                       L.in_row, L.zero_row, L.req_sum, L.flags, L.count, L.inv_pos);
+                       L.in_row, L.zero_row, L.req_sum, L.flags, L.bcount);
     // two scans in one set of launches: the neighbour counts (CSR offsets,
-    // rebased per request by k_offsets) and the transposed list's counts
-    ScanJobs sj{{Count{hdr + H_CENTERS, 0}, Count{hdr + H_ROWS, 0}},
-                {L.numneigh, L.count},
+    // rebased per request by k_offsets) and the buckets' entry counts
+    ScanJobs sj{{Count{hdr + H_CENTERS, 0}, Count{hdr + H_BUCKETS, 0}},
+                {L.numneigh, L.bcount},
                 {L.part, L.part + kScanBlocks + 1},
-                {L.scan, L.in_start}};
+                {L.scan, L.bbase}};
     enqueue_scans(sj, 2, s);
     hipLaunchKernelGGL(k_offsets, gs, tb, 0, s, hdr, seg, L.req_of, L.scan, L.offs, L.req_sum,
                        L.flags);
-    // transposed CSR: fill at the ranked slots, per-row sort with the inverse
-    hipLaunchKernelGGL(k_fill_t, gs, tb, 0, s, hdr, L.nl, L.centres, L.in_start, L.inv_pos,
-                       L.in_edges);
-    hipLaunchKernelGGL(k_sort_inverse_t, gs, tb, 0, s, hdr, L.in_start, L.in_edges, L.inv_pos);
+    // transposed CSR: entries into their buckets (staged in the record
+    // buffer, which the kernels below only write later), then per bucket:
+    // in_start, the rows' ids sorted, inv_pos
+    int2 *stage = reinterpret_cast<int2 *>(L.edge);
+    hipLaunchKernelGGL(k_scatter_t, gs, tb, 0, s, hdr, seg, tiles, L.nl, L.bbase, L.bcursor, stage);
+    hipLaunchKernelGGL(k_bucket_t, dim3(kGridStride), dim3(kBucketThreads), 0, s, hdr, stage,
+                       L.bbase, L.in_start, L.in_edges, L.inv_pos);
     if ((e = hipGetLastError())) return e;
     DevCounts dc;
     dc.hdr = hdr;
@@ -655,6 +809,7 @@ void worker_main(ldsf_server *S)
         h[ldsf::H_EDGES] = eb;
         h[ldsf::H_REQ] = (int)L->batch.size();
         h[ldsf::H_TILES] = ntiles;
+        h[ldsf::H_BUCKETS] = (rb + ldsf::kBucketRows - 1) >> ldsf::kBucketShift;
         const int bucket = bucket_for(S, maxnn);
         int rc = lane_graph(S, *L, bucket);
         hipError_t e = hipSuccess;
