@@ -131,11 +131,11 @@ def main():
 
     # a step: descriptors and dE/dr of every atom, from the fused kernel (the
     # reference's gradient semantics: the forward runs with the adjoint) and
-    # the gather; the separate forward and adjoint kernels are timed beside it
+    # the gather, both writing their outputs (=, bitwise the reference's +=
+    # into zeroed arrays); the separate forward and adjoint kernels (+=, after
+    # zeroing) are timed beside it
     def step(prec):
-        desc.zero_()
-        dx.zero_()
-        plan.forward_gradient(x, g, desc, dx, prec)
+        plan.forward_gradient_assign(x, g, desc, dx, prec)
 
     def step_split(prec):
         desc.zero_()
@@ -175,11 +175,10 @@ def main():
             ev[1].record(s)
             plan.adjoint_edges(x, g, prec)
             ev[2].record(s)
-            desc.zero_()
             ev[3].record(s)
-            plan.forward_adjoint_edges(x, g, desc, prec)
+            plan.forward_adjoint_edges(x, g, desc, prec, assign=True)
             ev[4].record(s)
-            plan.gather(dx)
+            plan.gather(dx, assign=True)
             ev[5].record(s)
             torch.cuda.synchronize()
             fwd_ms += ev[0].elapsed_time(ev[1]) / reps
@@ -353,9 +352,7 @@ def bench_batched(args, ds):
            "kernel_family": plan.kernel_family}
     for name, prec in (("fp64", _capi.FP64), ("fp32_bulk", _capi.FP32_BULK)):
         def step():
-            desc.zero_()
-            dx.zero_()
-            plan.forward_gradient(x, g, desc, dx, prec)
+            plan.forward_gradient_assign(x, g, desc, dx, prec)
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()

@@ -217,6 +217,20 @@ LDSF_API int ldsf_plan_forward_gradient(ldsf_plan *plan, const double *d_coordin
 LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
                                              const double *d_dE_dzeta, double *d_desc,
                                              int precision, void *stream);
+/* No reference counterpart: ldsf_plan_forward_gradient with the outputs
+ * assigned instead of added, d_desc = descriptors and d_dcoords = J^T
+ * d_dE_dzeta (every entry written; bitwise the += into zeroed arrays), for
+ * callers that want fresh outputs without clearing them first. */
+LDSF_API int ldsf_plan_forward_gradient_assign(ldsf_plan *plan, const double *d_coordinates,
+                                               const double *d_dE_dzeta, double *d_desc,
+                                               double *d_dcoords, int precision, void *stream);
+/* Its two halves (no reference counterpart): the fused kernel with d_desc
+ * assigned, and the gather with d_dcoords assigned (rows without records
+ * set to 0). */
+LDSF_API int ldsf_plan_forward_adjoint_edges_assign(ldsf_plan *plan, const double *d_coordinates,
+                                                    const double *d_dE_dzeta, double *d_desc,
+                                                    int precision, void *stream);
+LDSF_API int ldsf_plan_gather_assign(ldsf_plan *plan, double *d_dcoords, void *stream);
 
 /* ldsf_param_gradient on the device: d_dparams[ldsf_sf_num_params] and
  * d_drcut[n_species^2] (may be NULL) +=.  Allocates its partial sums and

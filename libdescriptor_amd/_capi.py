@@ -52,6 +52,9 @@ _PROTOS = {
     "ldsf_plan_adjoint_edges": (_i, [_vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_gather": (_i, [_vp, _vp, _vp]),
     "ldsf_plan_forward_gradient": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_forward_gradient_assign": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_forward_adjoint_edges_assign": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "ldsf_plan_gather_assign": (_i, [_vp, _vp, _vp]),
     "ldsf_plan_forward_adjoint_edges": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "ldsf_plan_kernel_family": (_i, [_vp]),
     "ldsf_plan_set_kernel_family": (_i, [_vp, _i]),
@@ -230,11 +233,22 @@ class Plan:
                                                _ptr(desc), _ptr(dcoords), int(precision),
                                                _stream_ptr(stream)))
 
-    def forward_adjoint_edges(self, coords, dE_dzeta, desc, precision=FP64, stream=None):
-        """First half of forward_gradient(): the fused kernel alone."""
-        check(lib().ldsf_plan_forward_adjoint_edges(self.handle, _ptr(coords), _ptr(dE_dzeta),
-                                                    _ptr(desc), int(precision),
-                                                    _stream_ptr(stream)))
+    def forward_gradient_assign(self, coords, dE_dzeta, desc, dcoords, precision=FP64,
+                                stream=None):
+        """forward_gradient() with the outputs assigned: desc = descriptors and
+        dcoords = J^T dE_dzeta, bitwise the += into zeroed arrays."""
+        check(lib().ldsf_plan_forward_gradient_assign(self.handle, _ptr(coords), _ptr(dE_dzeta),
+                                                      _ptr(desc), _ptr(dcoords), int(precision),
+                                                      _stream_ptr(stream)))
+
+    def forward_adjoint_edges(self, coords, dE_dzeta, desc, precision=FP64, stream=None,
+                              assign=False):
+        """First half of forward_gradient(): the fused kernel alone (assign:
+        desc = descriptors instead of +=)."""
+        fn = lib().ldsf_plan_forward_adjoint_edges_assign if assign else \
+            lib().ldsf_plan_forward_adjoint_edges
+        check(fn(self.handle, _ptr(coords), _ptr(dE_dzeta), _ptr(desc), int(precision),
+                 _stream_ptr(stream)))
 
     def param_gradient(self, coords, dE_dzeta, dparams, drcut=None, stream=None):
         """dparams += dE/d(parameters) (initDescriptor's flat layout), drcut +=
@@ -242,9 +256,11 @@ class Plan:
         check(lib().ldsf_plan_param_gradient(self.handle, _ptr(coords), _ptr(dE_dzeta),
                                              _ptr(dparams), _ptr(drcut), _stream_ptr(stream)))
 
-    def gather(self, dcoords, stream=None):
-        """Second half of gradient(): dcoords += the per-row sums of the records."""
-        check(lib().ldsf_plan_gather(self.handle, _ptr(dcoords), _stream_ptr(stream)))
+    def gather(self, dcoords, stream=None, assign=False):
+        """Second half of gradient(): dcoords += the per-row sums of the records
+        (assign: = them, rows without records 0)."""
+        fn = lib().ldsf_plan_gather_assign if assign else lib().ldsf_plan_gather
+        check(fn(self.handle, _ptr(dcoords), _stream_ptr(stream)))
 
     def gather_fold(self, layout, out=None, stream=None):
         """The gather and assemble_forces (neighbor.py:227-263) in one pass, for a

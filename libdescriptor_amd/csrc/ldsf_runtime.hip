@@ -351,7 +351,7 @@ int plan_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, int p
 // the fused kernel: desc += descriptors and the adjoint's records, from one
 // pass over the pairs and triplets (general family: the two kernels in turn)
 int plan_forward_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, double *desc,
-                               int precision, hipStream_t s)
+                               int precision, hipStream_t s, bool assign = false)
 {
     ldsf::SFTab tab;
     if (int rc = ldsf::device_table(plan->sf, &tab)) return rc;
@@ -363,10 +363,13 @@ int plan_forward_adjoint_edges(ldsf_plan *plan, const double *x, const double *d
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
                                   plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
                                   plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,
-                                  plan->wave_group, s, ldsf::DevCounts(), desc);
+                                  plan->wave_group, s, ldsf::DevCounts(), desc, assign);
     } else {
-        e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
-                                     plan->nl, plan->species, x, desc, s);
+        // the general forward adds: an assigning call starts its rows from zero
+        e = assign ? ldsf::launch_zero_f64((long)plan->n_centers * tab.width, desc, s) : hipSuccess;
+        if (e == hipSuccess)
+            e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
+                                         plan->nl, plan->species, x, desc, s);
         if (e == hipSuccess)
             e = ldsf::launch_adj_general(tab, precision, plan->n_centers, plan->centers,
                                          plan->offs, plan->nl, plan->species, x, dE,
@@ -377,11 +380,12 @@ int plan_forward_adjoint_edges(ldsf_plan *plan, const double *x, const double *d
     return LDSF_OK;
 }
 
-int plan_gather(ldsf_plan *plan, double *dcoords, hipStream_t s)
+int plan_gather(ldsf_plan *plan, double *dcoords, hipStream_t s, bool assign = false)
 {
     if (!plan->edges_ready && plan->n_edges + plan->n_centers > 0)
         return ldsf::fail(LDSF_ERR_ARG, "plan_gather: no adjoint records (run the adjoint first)");
-    HIP_TRY(ldsf::launch_gather(plan->n_total, plan->in_start, plan->edge, dcoords, s),
+    HIP_TRY(ldsf::launch_gather(plan->n_total, plan->in_start, plan->edge, dcoords, s, nullptr,
+                                assign),
             "gather launch");
     return LDSF_OK;
 }
@@ -394,10 +398,10 @@ int plan_gradient(ldsf_plan *plan, const double *x, const double *dE, double *dc
 }
 
 int plan_forward_gradient(ldsf_plan *plan, const double *x, const double *dE, double *desc,
-                          double *dcoords, int precision, hipStream_t s)
+                          double *dcoords, int precision, hipStream_t s, bool assign = false)
 {
-    if (int rc = plan_forward_adjoint_edges(plan, x, dE, desc, precision, s)) return rc;
-    return plan_gather(plan, dcoords, s);
+    if (int rc = plan_forward_adjoint_edges(plan, x, dE, desc, precision, s, assign)) return rc;
+    return plan_gather(plan, dcoords, s, assign);
 }
 
 // ---- host-pointer entry points: stage, run a plan, copy back ----
@@ -743,6 +747,18 @@ LDSF_API int ldsf_plan_forward_gradient(ldsf_plan *plan, const double *d_coordin
                                  (hipStream_t)stream);
 }
 
+LDSF_API int ldsf_plan_forward_gradient_assign(ldsf_plan *plan, const double *d_coordinates,
+                                               const double *d_dE_dzeta, double *d_desc,
+                                               double *d_dcoords, int precision, void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta || !d_desc || !d_dcoords)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_gradient_assign: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_gradient_assign: unknown precision");
+    return plan_forward_gradient(plan, d_coordinates, d_dE_dzeta, d_desc, d_dcoords, precision,
+                                 (hipStream_t)stream, true);
+}
+
 LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_coordinates,
                                              const double *d_dE_dzeta, double *d_desc,
                                              int precision, void *stream)
@@ -753,6 +769,25 @@ LDSF_API int ldsf_plan_forward_adjoint_edges(ldsf_plan *plan, const double *d_co
         return ldsf::fail(LDSF_ERR_ARG, "plan_forward_adjoint_edges: unknown precision");
     return plan_forward_adjoint_edges(plan, d_coordinates, d_dE_dzeta, d_desc, precision,
                                       (hipStream_t)stream);
+}
+
+LDSF_API int ldsf_plan_forward_adjoint_edges_assign(ldsf_plan *plan, const double *d_coordinates,
+                                                    const double *d_dE_dzeta, double *d_desc,
+                                                    int precision, void *stream)
+{
+    if (!plan || (plan->n_centers > 0 && (!d_coordinates || !d_dE_dzeta || !d_desc)))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_adjoint_edges_assign: bad arguments");
+    if (precision != LDSF_FP64 && precision != LDSF_FP32_BULK)
+        return ldsf::fail(LDSF_ERR_ARG, "plan_forward_adjoint_edges_assign: unknown precision");
+    return plan_forward_adjoint_edges(plan, d_coordinates, d_dE_dzeta, d_desc, precision,
+                                      (hipStream_t)stream, true);
+}
+
+LDSF_API int ldsf_plan_gather_assign(ldsf_plan *plan, double *d_dcoords, void *stream)
+{
+    if (!plan || (plan->n_total > 0 && !d_dcoords))
+        return ldsf::fail(LDSF_ERR_ARG, "plan_gather_assign: bad arguments");
+    return plan_gather(plan, d_dcoords, (hipStream_t)stream, true);
 }
 
 LDSF_API int ldsf_plan_param_gradient(ldsf_plan *plan, const double *d_coordinates,

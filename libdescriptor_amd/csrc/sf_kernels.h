@@ -109,11 +109,14 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
 // With desc (or dc.out_row) set, the fused kernel runs: it also adds each
 // centre's descriptors into desc (or assigns them to dc.out_row[c]), from the
 // same pair and triplet work.
+// assign (fused only): desc[c] = the descriptors instead of +=, as dc.out_row
+// does per centre.
 hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
                            int width, int max_nn, int group, hipStream_t s,
-                           DevCounts dc = DevCounts(), double *desc = nullptr);
+                           DevCounts dc = DevCounts(), double *desc = nullptr,
+                           bool assign = false);
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,
@@ -127,10 +130,13 @@ hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, 
 // sorts every row's bucket of in_edges and writes inv_pos[edge id] = its position
 hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, int *inv_pos,
                                hipStream_t s);
-// dcoords[a] += the sum of row a's records; nothing when *skip != 0 (skip may be NULL)
+// dcoords[a] += the sum of row a's records (assign: = it, rows without records
+// 0); nothing when *skip != 0 (skip may be NULL)
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
-                         hipStream_t s, const int *skip = nullptr);
+                         hipStream_t s, const int *skip = nullptr, bool assign = false);
 hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s);
+// v[0 .. n) = 0 by a kernel (no memset node where a caller captures a graph)
+hipError_t launch_zero_f64(long n, double *v, hipStream_t s);
 
 // sf_layout.hip: a device layout's rows, and the gather of adjoint records
 // (row r's records at in_start[r] .. in_start[r+1]) folded onto its

@@ -93,17 +93,24 @@ __global__ void k_sort_inverse(int n_total, const int *__restrict__ in_start, in
 // is non-zero (skip may be NULL)
 __global__ void k_gather(int n_total, const int *__restrict__ in_start,
                          const double *__restrict__ edge, double *__restrict__ dcoords,
-                         const int *__restrict__ skip)
+                         const int *__restrict__ skip, int assign)
 {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_total || (skip && *skip)) return;
     int b = in_start[a], e = in_start[a + 1];
-    if (b == e) return;
+    if (b == e && !assign) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     row_sum3(edge, b, e, s0, s1, s2);
-    dcoords[3 * (long)a] += s0;
-    dcoords[3 * (long)a + 1] += s1;
-    dcoords[3 * (long)a + 2] += s2;
+    double *o = dcoords + 3 * (long)a;
+    if (assign) {  // = the row sum, bitwise the += into zeros
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
+    } else {
+        o[0] += s0;
+        o[1] += s1;
+        o[2] += s2;
+    }
 }
 
 __global__ void k_max(int n, const int *__restrict__ v, int *out)
@@ -166,12 +173,26 @@ hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, 
     return hipGetLastError();
 }
 
+__global__ void k_zero_f64(long n, double *__restrict__ v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        v[i] = 0.0;
+}
+
+hipError_t launch_zero_f64(long n, double *v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const long blocks = std::min<long>((n + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)blocks), dim3(kBlock), 0, s, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
-                         hipStream_t s, const int *skip)
+                         hipStream_t s, const int *skip, bool assign)
 {
     if (n_total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start, edge,
-                       dcoords, skip);
+                       dcoords, skip, assign ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -1410,7 +1410,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const int *__restrict__ species, const double *__restrict__ x,
                 const double *__restrict__ dE, const int *__restrict__ inv_pos,
                 double *__restrict__ edge, int n_edges, double *__restrict__ desc, int width,
-                int maxn, DevCounts dc_in)
+                int maxn, DevCounts dc_in, int assign_rows)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -1437,8 +1437,9 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
     };
     auto self_pos = [&](const Ctr &C) -> int { return C.live ? inv_pos[n_edges + C.c] : 0; };
     // FWD: entry s of the centre's descriptor row before this call (+=), or 0
-    // where the serving loop assigns (=)
-    const bool assign = dc.out_row != nullptr;
+    // where the rows are assigned (=): the serving loop's per-centre rows, or
+    // a plan's assigning call
+    const bool assign = dc.out_row != nullptr || assign_rows != 0;
     auto row_in = [&](const Ctr &C) -> double {
         return (FWD && !assign && C.work && s < width) ? desc[(size_t)C.c * width + s] : 0.0;
     };
@@ -1962,7 +1963,7 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
                            const int *offs, const int *nl, const int *species, const double *x,
                            const double *dE, const int *inv_pos, double *edge, int n_edges,
                            int width, int max_nn, int group, hipStream_t s, DevCounts dc,
-                           double *desc)
+                           double *desc, bool assign)
 {
     if (n_centers <= 0) return hipSuccess;
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
@@ -1974,7 +1975,7 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
 #define LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, DCV)                                                    \
     hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD, ZG, DCV>), grid, block, lds, s, wt,          \
                        n_centers, centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, \
-                       width, maxn, dc)
+                       width, maxn, dc, assign ? 1 : 0)
 #define LDSF_ADJ(P, NSB, NPB, FWD, ZG)                          \
     do {                                                        \
         if (use_dc) LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, true);    \

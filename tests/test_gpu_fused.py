@@ -153,6 +153,27 @@ def test_fused_accumulates_into_desc(c1):
     assert rel_err(G, G0) <= TOL64
 
 
+@pytest.mark.parametrize("family", [1, 0])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_forward_gradient_assign(family, prec):
+    """ldsf_plan_forward_gradient_assign over stale outputs (NaN) equals,
+    bitwise, ldsf_plan_forward_gradient into zeroed outputs; both kernel
+    families (the wave kernels assign in the kernel, the general ones clear
+    the rows first), both precisions, a 1,000-atom two-species system with
+    padding rows that no edge points at."""
+    print(f"case: forward_gradient_assign family {family} precision {prec}", flush=True)
+    S = Setup(two_species(1000, seed=14))
+    S.plan.kernel_family = family
+    d0, g0 = S.fused(prec)
+    desc = torch.full((S.n, S.W), float("nan"), dtype=torch.float64, device=S.x.device)
+    dx = torch.full_like(S.x, float("nan"))
+    for _ in range(2):  # twice: the second call over the first one's results
+        S.plan.forward_gradient_assign(S.x, S.g, desc, dx, prec)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(desc.cpu().numpy().view(np.int64), d0.view(np.int64))
+        np.testing.assert_array_equal(dx.cpu().numpy().view(np.int64), g0.view(np.int64))
+
+
 def test_host_gradient_readds_forward(c1):
     """ldsf_gradient with a desc buffer: desc += descriptors, d_coords += J^T g."""
     from libdescriptor_amd import _capi
