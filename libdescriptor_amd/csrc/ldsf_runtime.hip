@@ -184,7 +184,7 @@ struct ldsf_plan {
 // host-pointer calls build one plan per call, in one arena per descriptor
 // and device, which keeps the largest sizes asked of it.
 struct PlanArena {
-    ldsf::Grow<int> centers, offs, flags, count, in_start, in_edges, inv_pos;
+    ldsf::Grow<int> centers, offs, flags, count, in_start, in_edges, inv_pos, stage;
     ldsf::Grow<char> tmp;
     ldsf::Grow<double> edge;
     ldsf::Grow<ldsf::WaveParams> wpar;
@@ -256,28 +256,45 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
     plan->n_edges = total;
     plan->max_nn = host_flags[1];
 
-    // transposed list: count, scan, scatter, sort buckets
-    HIP_TRY(take(count, arena ? &arena->count : nullptr, n_total + 1, &count_p), "hipMalloc(counts)");
-    HIP_TRY(hipMemsetAsync(count_p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
-    HIP_TRY(hipMemsetAsync(flags_p, 0, sizeof(int), s), "hipMemsetAsync");
-    HIP_TRY(ldsf::launch_edge_targets_count(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
-                                            count_p, flags_p, s), "count");
-    // out-of-range targets are skipped by the fill as by the count, and the
+    // transposed list; out-of-range targets are left out and flagged, and the
     // flag is read once the list is built
-    DevBuf<int> in_start, in_edges, inv_pos;
-    int *in_start_p = nullptr, *in_edges_p = nullptr, *inv_pos_p = nullptr;
+    HIP_TRY(hipMemsetAsync(flags_p, 0, sizeof(int), s), "hipMemsetAsync");
+    DevBuf<int> in_start, in_edges, inv_pos, stage;
+    int *in_start_p = nullptr, *in_edges_p = nullptr, *inv_pos_p = nullptr, *stage_p = nullptr;
+    const long n_ent = plan->n_edges + n_centers;
     HIP_TRY(take(in_start, arena ? &arena->in_start : nullptr, n_total + 1, &in_start_p),
             "hipMalloc(in_start)");
-    HIP_TRY(take(in_edges, arena ? &arena->in_edges : nullptr, plan->n_edges + n_centers,
-                 &in_edges_p), "hipMalloc(in_edges)");
-    HIP_TRY(ldsf::launch_scan_int(n_total, count_p, in_start_p, tmp_p, &tmp_bytes, s), "scan");
-    HIP_TRY(hipMemsetAsync(count_p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
-    HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers, n_total,
-                                           in_start_p, count_p, in_edges_p, s), "fill");
-    HIP_TRY(take(inv_pos, arena ? &arena->inv_pos : nullptr, plan->n_edges + n_centers, &inv_pos_p),
+    HIP_TRY(take(in_edges, arena ? &arena->in_edges : nullptr, n_ent, &in_edges_p),
+            "hipMalloc(in_edges)");
+    HIP_TRY(take(inv_pos, arena ? &arena->inv_pos : nullptr, n_ent, &inv_pos_p),
             "hipMalloc(inv_pos)");
-    HIP_TRY(ldsf::launch_sort_inverse(n_total, in_start_p, in_edges_p, inv_pos_p, s),
-            "sort and inverse");
+    if (ldsf::bucketed_fits(n_total)) {
+        // by buckets of rows: one global atomic per tile and bucket
+        const size_t nb1 = (size_t)ldsf::bucket_count_of(n_total) + 1;
+        HIP_TRY(take(count, arena ? &arena->count : nullptr, 3 * nb1, &count_p),
+                "hipMalloc(bucket counts)");
+        HIP_TRY(take(stage, arena ? &arena->stage : nullptr, 2 * (size_t)n_ent, &stage_p),
+                "hipMalloc(staging)");
+        HIP_TRY(ldsf::launch_transpose_bucketed(plan->n_edges, n_centers, d_nl, plan->centers,
+                                                n_total, count_p, count_p + nb1,
+                                                count_p + 2 * nb1, reinterpret_cast<int2 *>(stage_p),
+                                                in_start_p, in_edges_p, inv_pos_p, flags_p, tmp_p,
+                                                tmp_bytes, s),
+                "transposed list");
+    } else {
+        // more than 4096 buckets: count, scan, fill with per-entry atomics, sort
+        HIP_TRY(take(count, arena ? &arena->count : nullptr, n_total + 1, &count_p),
+                "hipMalloc(counts)");
+        HIP_TRY(hipMemsetAsync(count_p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
+        HIP_TRY(ldsf::launch_edge_targets_count(plan->n_edges, n_centers, d_nl, plan->centers,
+                                                n_total, count_p, flags_p, s), "count");
+        HIP_TRY(ldsf::launch_scan_int(n_total, count_p, in_start_p, tmp_p, &tmp_bytes, s), "scan");
+        HIP_TRY(hipMemsetAsync(count_p, 0, sizeof(int) * (n_total + 1), s), "hipMemsetAsync");
+        HIP_TRY(ldsf::launch_edge_targets_fill(plan->n_edges, n_centers, d_nl, plan->centers,
+                                               n_total, in_start_p, count_p, in_edges_p, s), "fill");
+        HIP_TRY(ldsf::launch_sort_inverse(n_total, in_start_p, in_edges_p, inv_pos_p, s),
+                "sort and inverse");
+    }
     HIP_TRY(hipMemcpyAsync(host_flags, flags_p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flags");
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (host_flags[0] & 1) return ldsf::fail(LDSF_ERR_ARG, "neighbour index out of range [0, n_total)");

@@ -127,6 +127,20 @@ hipError_t launch_edge_targets_count(long n_edges, int n_centers, const int *nl,
 hipError_t launch_edge_targets_fill(long n_edges, int n_centers, const int *nl, const int *centers,
                                     int n_total, const int *in_start, int *cursor, int *in_edges,
                                     hipStream_t s);
+// The transposed list in one go, by buckets of 256 rows (no per-entry global
+// atomics): in_start[0..n_total], in_edges (ids ascending per row) and
+// inv_pos, the same arrays as count + scan + fill + sort.  Needs
+// bucket_count_of(n_total) <= 4096 (bucketed_fits); bcount and bbase hold
+// nb + 1 ints, bcursor nb, stage n_edges + n_centers pairs; tmp is
+// launch_scan_int's scratch for nb entries.  Out-of-range targets set flags
+// bit 1 and are left out, as the count does.
+int bucket_count_of(int n_total);
+bool bucketed_fits(int n_total);
+hipError_t launch_transpose_bucketed(long n_edges, int n_centers, const int *nl,
+                                     const int *centers, int n_total, int *bcount, int *bbase,
+                                     int *bcursor, int2 *stage, int *in_start, int *in_edges,
+                                     int *inv_pos, int *flags, void *tmp, size_t tmp_bytes,
+                                     hipStream_t s);
 // sorts every row's bucket of in_edges and writes inv_pos[edge id] = its position
 hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, int *inv_pos,
                                hipStream_t s);

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 
+#include "sf_bucket.h"
 #include "sf_device.h"
 #include "sf_kernels.h"
 
@@ -51,6 +52,100 @@ __device__ __forceinline__ int edge_target(long e, long n_edges, const int *nl, 
     if (e < n_edges) return nl[e];
     int t = (int)(e - n_edges);
     return centers ? centers[t] : t;
+}
+
+// ---- the transposed list by buckets of rows (sf_bucket.h) ----
+// A tile is kPlanTile consecutive entries (edges, then the centres'
+// self-terms); each counts its entries per bucket in LDS, so that a plan's
+// random targets cost one global atomic per tile and bucket, not per entry.
+constexpr int kPlanPer = 64;                   // entries per thread and tile
+constexpr long kPlanTile = (long)kBlock * kPlanPer;
+
+// the tile's valid entries per bucket in hist[0..nb) (zeroed first); an
+// entry whose target is outside [0, n_total) sets flags bit 1 and is skipped
+__device__ __forceinline__ void plan_tile_hist(int *hist, int nb, long base, long n_ent,
+                                               long n_edges, const int *nl, const int *centers,
+                                               int n_total, int *flags)
+{
+    for (int j = threadIdx.x; j < nb; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int r = 0; r < kPlanPer; r += 8) {
+        int tg[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long i = base + (long)(r + k) * kBlock + threadIdx.x;
+            tg[k] = i < n_ent ? edge_target(i, n_edges, nl, centers) : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long i = base + (long)(r + k) * kBlock + threadIdx.x;
+            if (i >= n_ent) continue;
+            if (tg[k] < 0 || tg[k] >= n_total) bad = 1;
+            else atomicAdd(&hist[tg[k] >> kBucketShift], 1);
+        }
+    }
+    if (bad) atomicOr(flags, 1);
+    __syncthreads();
+}
+
+__global__ void k_bucket_count(long n_ent, long n_edges, const int *__restrict__ nl,
+                               const int *__restrict__ centers, int n_total, int nb, int *bcount,
+                               int *flags)
+{
+    __shared__ int hist[kHistCap];
+    const long ntiles = (n_ent + kPlanTile - 1) / kPlanTile;
+    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        plan_tile_hist(hist, nb, t * kPlanTile, n_ent, n_edges, nl, centers, n_total, flags);
+        for (int j = threadIdx.x; j < nb; j += blockDim.x)
+            if (hist[j]) atomicAdd(bcount + j, hist[j]);
+        __syncthreads();  // hist is the next tile's
+    }
+}
+
+// every valid entry's (id, row) into its bucket's range of stage: a run per
+// tile and bucket, claimed with one atomic, its slots taken in any order
+__global__ void k_bucket_scatter(long n_ent, long n_edges, const int *__restrict__ nl,
+                                 const int *__restrict__ centers, int n_total, int nb,
+                                 const int *__restrict__ bbase, int *bcursor, int2 *stage,
+                                 int *flags)
+{
+    __shared__ int hist[kHistCap];
+    const long ntiles = (n_ent + kPlanTile - 1) / kPlanTile;
+    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const long base = t * kPlanTile;
+        plan_tile_hist(hist, nb, base, n_ent, n_edges, nl, centers, n_total, flags);
+        for (int j = threadIdx.x; j < nb; j += blockDim.x)
+            if (hist[j]) hist[j] = bbase[j] + atomicAdd(bcursor + j, hist[j]);
+        __syncthreads();
+        for (int r = 0; r < kPlanPer; r += 8) {
+            int tg[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const long i = base + (long)(r + k) * kBlock + threadIdx.x;
+                tg[k] = i < n_ent ? edge_target(i, n_edges, nl, centers) : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const long i = base + (long)(r + k) * kBlock + threadIdx.x;
+                if (i >= n_ent || tg[k] < 0 || tg[k] >= n_total) continue;
+                const int slot = atomicAdd(&hist[tg[k] >> kBucketShift], 1);
+                stage[slot] = make_int2((int)i, tg[k]);
+            }
+        }
+        __syncthreads();  // hist is the next tile's
+    }
+}
+
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_rows(int nb, int n_total,
+                                                               const int2 *__restrict__ stage,
+                                                               const int *__restrict__ bbase,
+                                                               int *in_start, int *in_edges,
+                                                               int *inv_pos)
+{
+    __shared__ BucketLds sh;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x)
+        bucket_pass(b, nb, n_total, stage, bbase, in_start, in_edges, inv_pos, sh);
 }
 
 __global__ void k_count(long n_edges, int n_centers, const int *__restrict__ nl,
@@ -193,6 +288,39 @@ hipError_t launch_gather(int n_total, const int *in_start, const double *edge, d
     if (n_total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, grid_for(n_total), dim3(kBlock), 0, s, n_total, in_start, edge,
                        dcoords, skip, assign ? 1 : 0);
+    return hipGetLastError();
+}
+
+int bucket_count_of(int n_total) { return (n_total + kBucketRows - 1) >> kBucketShift; }
+
+bool bucketed_fits(int n_total) { return bucket_count_of(n_total) <= kHistCap; }
+
+hipError_t launch_transpose_bucketed(long n_edges, int n_centers, const int *nl,
+                                     const int *centers, int n_total, int *bcount, int *bbase,
+                                     int *bcursor, int2 *stage, int *in_start, int *in_edges,
+                                     int *inv_pos, int *flags, void *tmp, size_t tmp_bytes,
+                                     hipStream_t s)
+{
+    const int nb = bucket_count_of(n_total);
+    if (nb > kHistCap) return hipErrorInvalidValue;
+    const long n_ent = n_edges + n_centers;
+    hipError_t e;
+    if (n_total <= 0) return hipSuccess;
+    if ((e = hipMemsetAsync(bcount, 0, sizeof(int) * (nb + 1), s))) return e;
+    if ((e = hipMemsetAsync(bcursor, 0, sizeof(int) * nb, s))) return e;
+    const long ntiles = (n_ent + kPlanTile - 1) / kPlanTile;
+    const unsigned tg = (unsigned)std::max<long>(1, std::min<long>(ntiles, 2048));
+    if (n_ent > 0)
+        hipLaunchKernelGGL(k_bucket_count, dim3(tg), dim3(kBlock), 0, s, n_ent, n_edges, nl,
+                           centers, n_total, nb, bcount, flags);
+    if ((e = hipGetLastError())) return e;
+    size_t tb = tmp_bytes;
+    if ((e = launch_scan_int(nb, bcount, bbase, tmp, &tb, s))) return e;
+    if (n_ent > 0)
+        hipLaunchKernelGGL(k_bucket_scatter, dim3(tg), dim3(kBlock), 0, s, n_ent, n_edges, nl,
+                           centers, n_total, nb, bbase, bcursor, stage, flags);
+    hipLaunchKernelGGL(k_bucket_rows, dim3((unsigned)std::min(nb, 4096)), dim3(kBucketThreads), 0,
+                       s, nb, n_total, stage, bbase, in_start, in_edges, inv_pos);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@
 #include "ldsf_internal.h"
 #include "sf_device.h"
 #include "sf_kernels.h"
+#include "sf_bucket.h"
 #include "sf_scan.h"
 
 namespace ldsf {
@@ -83,16 +84,10 @@ enum { H_CENTERS = 0, H_ROWS = 1, H_EDGES = 2, H_REQ = 3, H_TILES = 4, H_BUCKETS
 constexpr int kThreads = 256;
 constexpr int kPerThread = 8;
 constexpr int kTile = kThreads * kPerThread;
-// The transposed list is built bucket by bucket: a bucket is kBucketRows
-// consecutive batch rows.  A tile counts its entries per bucket in LDS
-// (kHistCap buckets: requests of up to 1M rows; past that, per entry in
-// global memory), so the global atomics are one per tile and bucket.
-constexpr int kBucketShift = 8;
-constexpr int kBucketRows = 1 << kBucketShift;
-constexpr int kHistCap = 4096;
-constexpr int kBucketThreads = 512;
-constexpr int kBucketCap = 12288;  // ids of a bucket sorted in LDS (48 KiB); more: in place
-constexpr int kBucketPer = 8;      // staged entries per thread and load round
+// The transposed list is built bucket by bucket (sf_bucket.h): a bucket is
+// kBucketRows consecutive batch rows.  A tile counts its entries per bucket
+// in LDS (kHistCap buckets: requests of up to 1M rows; past that, per entry
+// in global memory), so the global atomics are one per tile and bucket.
 constexpr int kGridStride = 2048;
 constexpr int kBuckets = 9;        // wave kernels for max_nn <= 8, 16, ..., 64; general kernels
                                    // (max_nn > 64, or a set the wave kernels do not cover)
@@ -317,85 +312,17 @@ __global__ void k_scatter_t(const int *__restrict__ hdr, const Seg *__restrict__
     }
 }
 
-// One block per bucket: its entries counted per row in LDS, in_start of its
-// rows, the ids placed per row (in LDS when they fit, else in place in
-// in_edges), each row sorted ascending with inv_pos[id] = its position, and
-// the sorted ids written out.
+// One block per bucket (the pass of sf_bucket.h), sizes from the table
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_t(const int *__restrict__ hdr,
                                                             const int2 *__restrict__ stage,
                                                             const int *__restrict__ bbase,
                                                             int *in_start, int *in_edges,
                                                             int *inv_pos)
 {
-    __shared__ int cnt[kBucketRows], start[kBucketRows + 1], wsum[kBucketRows / 64];
-    __shared__ int buf_l[kBucketCap];
+    __shared__ BucketLds sh;
     const int R = hdr[H_ROWS], nb = hdr[H_BUCKETS];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-        const int lo = bbase[b], hi = bbase[b + 1], m = hi - lo;
-        const int r0 = b << kBucketShift, nr = min(kBucketRows, R - r0);
-        if (threadIdx.x < kBucketRows) cnt[threadIdx.x] = 0;
-        __syncthreads();
-        // kBucketPer loads in flight per thread, then their counts
-        for (int base = lo; base < hi; base += kBucketThreads * kBucketPer) {
-            int row[kBucketPer];
-#pragma unroll
-            for (int k = 0; k < kBucketPer; ++k) {
-                const int i = base + threadIdx.x + k * kBucketThreads;
-                row[k] = i < hi ? stage[i].y - r0 : -1;
-            }
-#pragma unroll
-            for (int k = 0; k < kBucketPer; ++k)
-                if (row[k] >= 0) atomicAdd(&cnt[row[k]], 1);
-        }
-        __syncthreads();
-        // exclusive scan of the row counts (the first kBucketRows threads)
-        int incl = 0;
-        if (threadIdx.x < kBucketRows) {
-            incl = cnt[threadIdx.x];
-            for (int off = 1; off < 64; off <<= 1) {
-                const int u = __shfl_up(incl, off);
-                if (lane >= off) incl += u;
-            }
-            if (lane == 63) wsum[wv] = incl;
-        }
-        __syncthreads();
-        if (threadIdx.x < kBucketRows) {
-            int add = 0;
-            for (int k = 0; k < wv; ++k) add += wsum[k];
-            const int ex = add + incl - cnt[threadIdx.x];
-            start[threadIdx.x] = ex;
-            if (threadIdx.x == kBucketRows - 1) start[kBucketRows] = add + incl;
-            if (threadIdx.x < nr) in_start[r0 + threadIdx.x] = lo + ex;
-            cnt[threadIdx.x] = 0;  // the rows' cursors from here on
-        }
-        if (b == nb - 1 && threadIdx.x == 0) in_start[R] = hi;
-        __syncthreads();
-        const bool in_lds = m <= kBucketCap;
-        int *buf = in_lds ? buf_l : in_edges + lo;
-        for (int base = lo; base < hi; base += kBucketThreads * kBucketPer) {
-            int2 q[kBucketPer];
-#pragma unroll
-            for (int k = 0; k < kBucketPer; ++k) {
-                const int i = base + threadIdx.x + k * kBucketThreads;
-                q[k] = i < hi ? stage[i] : make_int2(-1, -1);
-            }
-#pragma unroll
-            for (int k = 0; k < kBucketPer; ++k)
-                if (q[k].x >= 0) {
-                    const int r = q[k].y - r0;
-                    buf[start[r] + atomicAdd(&cnt[r], 1)] = q[k].x;
-                }
-        }
-        __threadfence_block();
-        __syncthreads();
-        sort_rows_inverse(nr, start, buf, inv_pos, wv, kBucketThreads / 64, lane, lo);
-        __threadfence_block();
-        __syncthreads();
-        if (in_lds)
-            for (int i = threadIdx.x; i < m; i += kBucketThreads) in_edges[lo + i] = buf_l[i];
-        __syncthreads();  // cnt, start and buf_l are the next bucket's
-    }
+    for (int b = blockIdx.x; b < nb; b += gridDim.x)
+        bucket_pass(b, nb, R, stage, bbase, in_start, in_edges, inv_pos, sh);
 }
 
 // per-row sums of the records, written (=) into the owning request's output
