@@ -375,6 +375,36 @@ def test_host_path_workspace_reuse(precision):
         assert rel_err(x1 + 0.25, ref_g) <= 1e-14
 
 
+def test_plan_past_the_bucketed_build():
+    """A plan over more than 4096 x 256 rows builds its transposed list with the
+    count/fill/sort path instead of buckets: a 300-atom system with 1.1 M
+    unreferenced rows appended gives, bitwise, the gradient of the plain
+    system on its rows and zero on the rest, in both precisions."""
+    import torch
+    from libdescriptor_amd import _capi
+    print("case: plan over 1.1M rows", flush=True)
+    L = small_two_species(300, seed=21)
+    ds = sf_desc(("A", "B"), 5.0, resolved=True)
+    d = torch.device("cuda:0")
+    t = lambda a, dt=torch.int32: torch.tensor(a, dtype=dt, device=d)  # noqa: E731
+    R = len(L["coords"])
+    extra = 1_100_000
+    sp = np.concatenate([L["species"], np.zeros(extra, np.int32)])
+    x = np.concatenate([L["coords"], np.full((extra, 3), 1e6)])
+    n = L["n_contrib"]
+    g = t(np.random.default_rng(3).normal(size=(n, ds.width)), torch.float64)
+    small = _capi.Plan(ds, R, t(L["species"]), t(L["neighlist"]), t(L["numneigh"]))
+    big = _capi.Plan(ds, R + extra, t(sp), t(L["neighlist"]), t(L["numneigh"]))
+    for prec in (0, 1):
+        a = torch.zeros((R, 3), dtype=torch.float64, device=d)
+        b = torch.zeros((R + extra, 3), dtype=torch.float64, device=d)
+        small.gradient(t(L["coords"], torch.float64), g, a, prec)
+        big.gradient(t(x, torch.float64), g, b, prec)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b[:R])
+        assert not torch.any(b[R:])
+
+
 def test_cpp_drop_in_caller(c1, tmp_path):
     """A reference-style C++ program (tests/cpp/drop_in.cpp) through
     Descriptor::compute / gradient / compute_single_atom on the GPU."""
