@@ -394,6 +394,7 @@ struct Grp {
     double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (w, wv: row weights of up to
                                                 // 3 blocks; g: the centre's dE/dzeta row)
     double *d;                                  // forward only: the centre's descriptor row
+    double *gB, *dB;                            // paired: the second centre's g and d rows
     int *sp, *ip;                               // species; adjoint: record position of the edge
     unsigned short *l0;                         // active pairs, one region per species-pair block
 };
@@ -411,21 +412,28 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 // The adjoint kinds put the row weights (w, wv), the centre's dE/dzeta row (g)
 // and, fused, its descriptor row (d) first, so that the triplet loop reaches
 // every weight through a ds_read2 immediate offset (at most 2040 bytes) from
-// one per-lane address.
+// one per-lane address.  The fused kind holds two centres at a time: the
+// second centre's neighbours at slots 32..63, its dE/dzeta and descriptor
+// rows (gB, dB) after the neighbour arrays.
 constexpr int kStr = kW + 1;
 constexpr int kHeadAdj = 48 + 48 + kWaveSlots;                 // w, wv, g
-constexpr int kHeadFused = kHeadAdj + kWaveSlots;              // ... and d
+constexpr int kHeadPair = kHeadAdj + kWaveSlots;               // ... and d
 constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
 constexpr int kAdjDoubles = kHeadAdj + 11 * kStr + 1;          // w, wv, g, px..fz, pad
-constexpr int kFusedDoubles = kHeadFused + 11 * kStr + 1;      // w, wv, g, d, px..fz, pad
+constexpr int kPairRows = kHeadPair + 11 * kStr + 1;           // w, wv, g, d, px..fz, pad
+constexpr int kPairDoubles = kPairRows + 2 * kWaveSlots;       // ..., gB, dB
 
-// kernel kinds: the forward, the adjoint, and the fused forward + adjoint
-enum { kFwd = 0, kAdj = 1, kFused = 2 };
+// kernel kinds: the forward, the adjoint, the fused forward + adjoint
+enum { kFwd = 0, kAdj = 1, kPair = 2 };
 
 __host__ __device__ constexpr int kind_doubles(int kind)
 {
-    return kind == kFwd ? kFwdDoubles : (kind == kAdj ? kAdjDoubles : kFusedDoubles);
+    return kind == kFwd ? kFwdDoubles : (kind == kAdj ? kAdjDoubles : kPairDoubles);
 }
+
+// The paired kind's pair-list region also holds the g2 terms of both centres
+// (slots 0..63) and, after a centre's triplet passes, its 16 x 16 column sums.
+constexpr size_t kPairListMin = 16 * 16 * sizeof(double);
 
 // The pair list's region also holds, between the pair phase and the pair
 // enumeration, the g2 terms of every compacted neighbour (8 per neighbour,
@@ -435,8 +443,9 @@ __host__ __device__ inline size_t group_bytes(int maxn, int kind, int term_bytes
 {
     const int pairs = maxn * (maxn - 1) / 2;
     size_t list = sizeof(unsigned short) * (size_t)(pairs > 1 ? pairs : 1);
-    const size_t terms = (size_t)8 * term_bytes * maxn;
+    const size_t terms = (size_t)8 * term_bytes * (kind == kPair ? kW : maxn);
     if (kind != kAdj && terms > list) list = terms;
+    if (kind == kPair && list < kPairListMin) list = kPairListMin;
     size_t b = sizeof(double) * kind_doubles(kind) + sizeof(int) * (kind == kFwd ? 1 : 2) * kW + list;
     return (b + 15) & ~size_t(15);
 }
@@ -463,9 +472,11 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, int kind)
         L.w = p;  // read as float4 / double2
         L.wv = p + 48;
         L.g = p + 96;
-        L.d = kind == kFused ? p + kHeadAdj : nullptr;
-        q = p + (kind == kFused ? kHeadFused : kHeadAdj);  // even: 16-byte aligned
+        L.d = kind == kPair ? p + kHeadAdj : nullptr;
+        q = p + (kind == kPair ? kHeadPair : kHeadAdj);  // even: 16-byte aligned
     }
+    L.gB = kind == kPair ? p + kPairRows : nullptr;
+    L.dB = kind == kPair ? p + kPairRows + kWaveSlots : nullptr;
     L.px = q;
     L.py = q + kStr;
     L.pz = q + 2 * kStr;
@@ -594,11 +605,10 @@ __device__ __forceinline__ void cut_pair(double r, double rc, double rcinv, doub
 //              with g_e = fc(r_a) exp(-eta_e r_a^2)
 //              adjoint: (1 / r_a, fc(r_a), dfc/dr(r_a), species bits)
 //   W[a]       adjoint: (exp(-eta_0 r_a^2), exp(-eta_1 r_a^2))     float2
-// r_jk^2 and the base 1 + lambda cos(theta) are formed in FP64 (the base's
-// cancellation-free numerator would lose ~1e-6 absolute from FP32
-// displacements, which zeta = 16 turns into 1e-5 relative error of a small
-// row); everything after them -- exp, log, powers, cutoff, products -- is
-// FP32.  exp(-eta (r_ij^2 + r_ik^2 + r_jk^2)) factors into the two
+// r_jk^2, d_a . d_b and the base's numerator r_ij r_ik + lambda d_a . d_b are
+// formed in FP64 (from FP32 displacements the numerator would lose ~1e-6
+// absolute, which zeta = 16 turns into 1e-5 relative error of a small row);
+// everything after them -- exp, log, powers, cutoff, products -- is FP32.  exp(-eta (r_ij^2 + r_ik^2 + r_jk^2)) factors into the two
 // neighbours' stored terms and one exp of r_jk.
 __device__ __forceinline__ float uniform_f(double v)
 {
@@ -771,15 +781,32 @@ __device__ __forceinline__ void geom_f(const double2 &a0, const double2 &a1, con
     t.sn = sin_q(t.y, t.y * t.y);
 }
 
-// the base 1 + lambda_l cos(theta) = (lp (r_ij + r_ik)^2 - lm (r_ij - r_ik)^2 -
-// lambda r_jk^2) / (2 r_ij r_ik), numerator in FP64; a base <= 0 becomes 0,
-// and its powers and their derivatives 0 as the reference's base <= 0 branch
-// (src/SymmetryFunctions.cpp:118-119)
-__device__ __forceinline__ float base_f(const Fast &F, int l, double sp, double sm, double rjk2,
-                                        float inv)
+// the base 1 + lambda_l cos(theta) = (r_ij r_ik + lambda_l d_j . d_k) /
+// (r_ij r_ik), numerator in FP64 from the FP64 displacements (rr = r_ij r_ik,
+// dot = d_j . d_k; for lambda = -1 and a narrow angle the two nearly cancel,
+// which FP64 carries to ~1e-15 of r_ij r_ik), inv1 = 1 / (r_ij r_ik) in FP32;
+// a base <= 0 becomes 0, and its powers and their derivatives 0 as the
+// reference's base <= 0 branch (src/SymmetryFunctions.cpp:118-119)
+__device__ __forceinline__ float base_f(const Fast &F, int l, double rr, double dot, float inv1)
+{
+    return fmaxf((float)fma(F.lam[l], dot, rr) * inv1, 0.0f);
+}
+
+// the same base as (lp (r_ij + r_ik)^2 - lm (r_ij - r_ik)^2 - lambda r_jk^2) /
+// (2 r_ij r_ik) (sp, sm: the two squares; inv = 1 / (2 r_ij r_ik)): the
+// forward kernel's form, which holds fewer registers
+__device__ __forceinline__ float base_sq_f(const Fast &F, int l, double sp, double sm, double rjk2,
+                                           float inv)
 {
     const double num = fma(-F.lam[l], rjk2, fma(F.lp[l], sp, -F.lm[l] * sm));
     return fmaxf((float)num * inv, 0.0f);
+}
+
+// d_a . d_b of two compacted neighbours of the FP32-bulk path (D layout)
+__device__ __forceinline__ double dot_ab(const double2 &a0, const double2 &a1, const double2 &b0,
+                                         const double2 &b1)
+{
+    return fma(a1.x, b1.x, fma(a0.y, b0.y, a0.x * b0.x));
 }
 
 // The g2 rows of one neighbour at distance r in the FP32-bulk form, with
@@ -825,11 +852,13 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
 // n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.  UNI: every species pair has the cutoff
-// of pair 0 (no per-pair choice).
+// of pair 0 (no per-pair choice).  The neighbours sit at slots off .. off +
+// M - 1 (off = 0, or 32 for the second centre of a paired wave); the stored
+// codes carry the absolute slots.
 template <int P, int NPB, bool UNI>
 __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, const Fast &F,
                                                 int ns, int M, int n1, int s, int code0,
-                                                int (&cnt)[3], int (&base)[3])
+                                                int (&cnt)[3], int (&base)[3], int off = 0)
 {
     const int n0 = M - n1;
     base[0] = 0;
@@ -846,7 +875,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
         bool act = false;
         int blk = 0;
         if (t < T) {
-            const int a = code & 63, b = code >> 6;
+            const int a = (code & 63) + off, b = (code >> 6) + off;
             int sa, sb;
             if constexpr (P == 0) {
                 const double rjk2 =
@@ -874,7 +903,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
         }
         if (NPB == 1) {
             const unsigned long long m = __builtin_amdgcn_ballot_w64(act);
-            if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
+            if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)(code + 65 * off);
             cnt[0] += __popcll(m);
         } else {
             const unsigned long long m0 = __builtin_amdgcn_ballot_w64(act && blk == 0);
@@ -884,7 +913,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                 const int p0 = base[0] + cnt[0] + below(m0, s);
                 const int p1 = base[1] + cnt[1] + below(m1, s);
                 const int p2 = base[2] + cnt[2] + below(m2, s);
-                L.l0[blk == 0 ? p0 : (blk == 1 ? p1 : p2)] = (unsigned short)code;
+                L.l0[blk == 0 ? p0 : (blk == 1 ? p1 : p2)] = (unsigned short)(code + 65 * off);
             }
             cnt[0] += __popcll(m0);
             cnt[1] += __popcll(m1);
@@ -1007,8 +1036,8 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     const float fjk = g.sn * g.sn, dfjk = -2.0f * rcs * g.sn * cs;
     const float ra = (float)a1.y, rb = (float)b1.y, ria = va.x, rib = vb.x;
     const float fa = va.y, fb = vb.y, dfa = va.z, dfb = vb.z;
-    const float inv = 0.5f * ria * rib;  // 1 / (2 r_ij r_ik)
-    const double sp = (a1.y + b1.y) * (a1.y + b1.y), sm = (a1.y - b1.y) * (a1.y - b1.y);
+    const float inv1 = ria * rib;  // 1 / (r_ij r_ik)
+    const double rr = a1.y * b1.y, dot = dot_ab(a0, a1, b0, b1);
     // the block's row weights: the adjoint reads all 32 up front, four to an
     // LDS read; the fused kernel, short of registers, two at their use
     float wl[16], wvl[16];
@@ -1040,7 +1069,7 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     }
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-        const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
+        const float bl = base_f(F, l, rr, dot, inv1);
         // power-of-two grids: B^(2^k) and B^(2^k - 1) by squaring; else
         // B^(zeta - 1) = exp2((zeta - 1) log2 B)
         // (B = 0: B^0 = 0 as well, the derivative of the reference's 0 branch)
@@ -1072,10 +1101,9 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     const float SB = ue0 + ue1;
     const float SC = fmaf(F.eta[0], ue0, F.eta[1] * ue1);
     // d cos(theta) / d r_ij = (r_ij^2 - r_ik^2 + r_jk^2) / (2 r_ij^2 r_ik), whose
-    // numerator is -2 d_a . (d_b - d_a); likewise for r_ik with 2 d_b . (d_b - d_a)
-    const float nij = (float)(-2.0 * fma(a1.x, g.ez, fma(a0.y, g.ey, a0.x * g.ex)));
-    const float nik = (float)(2.0 * fma(b1.x, g.ez, fma(b0.y, g.ey, b0.x * g.ex)));
-    const float dcij = nij * inv * ria, dcik = nik * inv * rib, dcjk = -2.0f * g.rjk * inv;
+    // numerator is 2 (r_ij^2 - d_a . d_b); likewise for r_ik with 2 (r_ik^2 - d_a . d_b)
+    const float nij = (float)fma(a1.y, a1.y, -dot), nik = (float)fma(b1.y, b1.y, -dot);
+    const float dcij = nij * inv1 * ria, dcik = nik * inv1 * rib, dcjk = -g.rjk * inv1;
     const float fc3 = fa * fb * fjk;
     const float Aij = fmaf(fc3, fmaf(dcij, SA, -2.0f * ra * SC), fb * fjk * dfa * SB);
     const float Aik = fmaf(fc3, fmaf(dcik, SA, -2.0f * rb * SC), fa * fjk * dfb * SB);
@@ -1279,7 +1307,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                             te[1] = (double)(va.z * vb.z * (fjk * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2)));
 #pragma unroll
                             for (int l = 0; l < 2; ++l) {
-                                const float bl = base_f(F, l, sp, sm, g.rjk2d, inv);
+                                const float bl = base_sq_f(F, l, sp, sm, g.rjk2d, inv);
                                 float sq[5], lb = 0.0f;
                                 if constexpr (ZG != 0) squares<ZG>(bl, sq);
                                 else lb = __builtin_amdgcn_logf(fmaxf(bl, 1e-30f));
@@ -1396,21 +1424,15 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
     }
 }
 
-// The adjoint kernel.  FWD = true is the fused forward + adjoint: the
-// descriptor row is accumulated from the same pair and triplet work (the
-// reference's gradient adds the forward into desc as well,
-// src/Descriptors.cpp:165-190), its triplet lanes dealt the forward kernel's
-// 4-lane columns so that the row sums reduce the same way.
-// Occupancy floors (waves per SIMD): FP64 fused 2, FP32-bulk fused 3,
-// FP32-bulk adjoint 4 (128 VGPRs).
-template <int P, int NSB, int NPB, bool FWD, int ZG, bool DC>
-__global__ __launch_bounds__(64, FWD ? (P == 0 ? 2 : 3) : (P == 0 ? 1 : 4))
+// The adjoint kernel (the fused forward + adjoint is k_fused_pair).
+// Occupancy floors (waves per SIMD): FP64 1, FP32-bulk 4 (128 VGPRs).
+template <int P, int NSB, int NPB, int ZG, bool DC>
+__global__ __launch_bounds__(64, P == 0 ? 1 : 4)
 void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const int *__restrict__ offs, const int *__restrict__ nl,
                 const int *__restrict__ species, const double *__restrict__ x,
                 const double *__restrict__ dE, const int *__restrict__ inv_pos,
-                double *__restrict__ edge, int n_edges, double *__restrict__ desc, int width,
-                int maxn, DevCounts dc_in, int assign_rows)
+                double *__restrict__ edge, int n_edges, int width, int maxn, DevCounts dc_in)
 {
     using T = typename Real<P>::T;
     const Real<P> R;
@@ -1419,7 +1441,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
     const int s = threadIdx.x;
     const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
     const Shared S = carve_shared(lds_base, ns);
-    const Grp L = carve_group(lds_base, ns, FWD ? kFused : kAdj);
+    const Grp L = carve_group(lds_base, ns, kAdj);
     stage_shared(S, wt, s);
     const WaveParams &wp = *S.par;
     const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
@@ -1436,29 +1458,17 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         return dc.in_row ? dc.in_row[C.c][s] : dE[(size_t)C.c * width + s];
     };
     auto self_pos = [&](const Ctr &C) -> int { return C.live ? inv_pos[n_edges + C.c] : 0; };
-    // FWD: entry s of the centre's descriptor row before this call (+=), or 0
-    // where the rows are assigned (=): the serving loop's per-centre rows, or
-    // a plan's assigning call
-    const bool assign = dc.out_row != nullptr || assign_rows != 0;
-    auto row_in = [&](const Ctr &C) -> double {
-        return (FWD && !assign && C.work && s < width) ? desc[(size_t)C.c * width + s] : 0.0;
-    };
     Ctr C = load_centre(tr.first, n_centers, centers, offs, species, x, dc);
     double gs = grad_of(C);
-    double rin = row_in(C);
     int ips = self_pos(C);
     int ipe = s < C.N ? inv_pos[C.beg + s] : 0;  // record position of lane s's edge
     Nbr B = load_nbr(load_index(C, s, nl), species, x);
     for (int c = tr.first; c < tr.hi; c += tr.step) {
         const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
         const double gsn = grad_of(Cn);
-        const double rinn = row_in(Cn);
         const int ipsn = self_pos(Cn);
-        double *row = (FWD && C.work) ? (dc.out_row ? dc.out_row[c] : desc + (size_t)c * width)
-                                      : nullptr;
         // a skipped centre (flagged request) writes only its zero self record
         if (C.work && s < width) L.g[s] = gs;
-        if (FWD && s < kWaveSlots) L.d[s] = rin;
         // FP32-bulk: the g2 rows' weights per species block, from the lanes
         // that hold them (past the triplet weights in L.w)
         float *gw = reinterpret_cast<float *>(L.w) + 48;
@@ -1494,7 +1504,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         const int n1 = NSB == 2 ? __popcll(m1) : 0;
         // the enumeration's first pair codes, a phase ahead
         const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
-        T *terms = reinterpret_cast<T *>(L.l0);  // FWD: the g2 terms, until the enumeration
         if (valid) {
             const int pos = NSB == 2 ? (sj ? (M - n1) + below(m1, s) : below(m & ~m1, s))
                                      : below(m, s);
@@ -1503,21 +1512,15 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
             double sum = 0.0;
             if constexpr (P == 1)
-                sum = (double)pair_rows_f<NSB, true, FWD>(wp, gw, (float)r, fc, dfc, sj,
-                                                          reinterpret_cast<float *>(terms) + 8 * pos);
+                sum = (double)pair_rows_f<NSB, true, false>(wp, gw, (float)r, fc, dfc, sj, nullptr);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                if (P == 0) {
-                    double v = 0.0;
-                    if (q < wt.nr2) {  // L.g[slot] needs a row of the set
-                        const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
-                        const double dr = r - wp.rs2[q];
-                        const T ex = dexp(-wp.eta2[q] * dr * dr, R);
-                        const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
-                        sum += L.g[slot] * (double)dphi;
-                        v = (double)(ex * fc);  // the g2 term, as the forward kernel forms it
-                    }
-                    if (FWD) terms[8 * pos + q] = (T)v;
+                if (P == 0 && q < wt.nr2) {  // L.g[slot] needs a row of the set
+                    const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
+                    const double dr = r - wp.rs2[q];
+                    const T ex = dexp(-wp.eta2[q] * dr * dr, R);
+                    const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
+                    sum += L.g[slot] * (double)dphi;
                 }
             }
             // 1/r: FP32 (one Newton step) in the FP32-bulk form
@@ -1551,10 +1554,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         const int jn = load_index(Cn, s, nl);
         const int ipen = s < Cn.N ? inv_pos[Cn.beg + s] : 0;
         __syncthreads();
-        if (FWD) {
-            g2_rows_sum<NSB>(terms, M, M - n1, s, wp.slots, L.d);
-            __syncthreads();
-        }
 
         // ---- triplets: the active pairs of every block, lanes split over the blocks ----
         Nbr Bn;
@@ -1577,54 +1576,26 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             __syncthreads();
             const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
             int passes = (nall + kW - 1) / kW;
-            int pl, u, stride, nb;
-            const unsigned short *lst;
-            bool mine;
-            // FWD: columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48; column s & 15)
-            // dealt to the blocks in proportion to their active pairs, the
-            // fewest passes I with sum_p ceil(cnt_p / 4I) <= 16 columns.
-            // Otherwise lanes dealt in proportion: the fewest passes I with
-            // sum_p ceil(cnt_p / I) <= 64 lanes; lane s serves block p with
-            // stride lanes(p).
-            int nc[3] = {16, 0, 0};
-            const int col = s & 15, rowq = s >> 4;
-            if (FWD) {
-                if (NPB > 1 && nall > 0) {
-                    // ceil(c / 4I) = ceil(ceil(c / 4) / I)
-                    const int c40 = (cnt[0] + 3) >> 2, c41 = (cnt[1] + 3) >> 2,
-                              c42 = (cnt[2] + 3) >> 2;
-                    auto cols_for = [&](int I) {
-                        nc[0] = ceil_div_small(c40, I);
-                        nc[1] = ceil_div_small(c41, I);
-                        nc[2] = ceil_div_small(c42, I);
-                        return nc[0] + nc[1] + nc[2];
-                    };
-                    while (cols_for(passes) > 16) ++passes;
-                }
-                pl = NPB == 1 ? 0 : (col < nc[0] ? 0 : (col < nc[0] + nc[1] ? 1 : 2));
-                const int cs = pl == 0 ? 0 : (pl == 1 ? nc[0] : nc[0] + nc[1]);
-                const int ncol = pl == 0 ? nc[0] : (pl == 1 ? nc[1] : nc[2]);
-                u = NPB == 1 ? s : (col - cs) + rowq * ncol;
-                stride = NPB == 1 ? kW : 4 * ncol;
-                mine = NPB == 1 || col < cs + ncol;
-            } else {
-                int ln[3] = {kW, 0, 0};
-                if (NPB > 1 && nall > 0) {
-                    auto lanes_for = [&](int I) {
-                        ln[0] = ceil_div_small(cnt[0], I);
-                        ln[1] = ceil_div_small(cnt[1], I);
-                        ln[2] = ceil_div_small(cnt[2], I);
-                        return ln[0] + ln[1] + ln[2];
-                    };
-                    while (lanes_for(passes) > kW) ++passes;
-                }
-                pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
-                u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
-                stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
-                mine = u < stride;
+            // lanes dealt to the blocks in proportion to their active pairs:
+            // the fewest passes I with sum_p ceil(cnt_p / I) <= 64 lanes; lane
+            // s serves block p with stride lanes(p)
+            int ln[3] = {kW, 0, 0};
+            if (NPB > 1 && nall > 0) {
+                auto lanes_for = [&](int I) {
+                    ln[0] = ceil_div_small(cnt[0], I);
+                    ln[1] = ceil_div_small(cnt[1], I);
+                    ln[2] = ceil_div_small(cnt[2], I);
+                    return ln[0] + ln[1] + ln[2];
+                };
+                while (lanes_for(passes) > kW) ++passes;
             }
-            nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
-            lst = L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
+            const int pl = NPB == 1 ? 0 : (s < ln[0] ? 0 : (s < ln[0] + ln[1] ? 1 : 2));
+            const int u = NPB == 1 ? s : s - (pl == 0 ? 0 : (pl == 1 ? ln[0] : ln[0] + ln[1]));
+            const int stride = NPB == 1 ? kW : (pl == 0 ? ln[0] : (pl == 1 ? ln[1] : ln[2]));
+            const bool mine = u < stride;
+            const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
+            const unsigned short *lst =
+                L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
             const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
             const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
             // this lane's species-pair cutoff (FP32-bulk)
@@ -1632,27 +1603,19 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             const float rcl = P == 1 ? (float)S.rc[kp] : 0.0f;
             const double rc2l = P == 1 ? S.rc2[kp] : 0.0;
             const float rcsl = P == 1 ? (float)S.rcs[kp] : 0.0f;
-            // FWD: the lane's row sums, FP64 in both forms
-            double accd[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) accd[k] = 0.0;
+            double accd[16];  // (the fused kernel's row sums; unused here)
             for (int i = 0; i < passes; ++i) {
                 const int t = u + i * stride;
                 if (mine && t < nb) {
                     const int code = lst[t];
                     const int a = code & 63, b = (code >> 6) & 63;
                     if constexpr (P == 1) {
-                        adj_triplet_f<NPB, FWD, ZG>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
+                        adj_triplet_f<NPB, false, ZG>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
                     } else {
                         Trip<P> q;
                         triplet<P, true>(S, wp, wt, L, ns, a, b, q);
                         const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
                         const T fc3 = q.fa * q.fb * q.fjk;
-                        double te[2] = {0.0, 0.0};
-                        if (FWD) {
-#pragma unroll
-                            for (int e = 0; e < 2; ++e) te[e] = (double)(q.E[e] * fc3);
-                        }
                         T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
 #pragma unroll
                         for (int l = 0; l < 2; ++l) {
@@ -1671,14 +1634,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                                         for (int e = 0; e < 2; ++e) {
                                             U[e] += w[(z * 2 + l) * 2 + e] * pw;
                                             V[e] += wv[(z * 2 + l) * 2 + e] * pm1;
-                                        }
-                                        if (FWD) {
-                                            const double pz = (double)((T)wp.coef[z] * pw);
-#pragma unroll
-                                            for (int e = 0; e < 2; ++e)
-                                                if (e < wt.ne)
-                                                    accd[(z * 2 + l) * 2 + e] =
-                                                        fma(pz, te[e], accd[(z * 2 + l) * 2 + e]);
                                         }
                                     }
                                 }
@@ -1713,41 +1668,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 }
             }
             __syncthreads();
-            if (FWD) {
-                // the row sums: a column's 4 lanes with two register swaps, then
-                // the columns of a block through LDS, in order, all in FP64 --
-                // the forward kernel's reduction; lane s keeps values
-                // 4 (s >> 4) .. +3.  The neighbour arrays are dead here.
-                double v4[4];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    double a = accd[q], b = accd[q + 8];
-                    xswap<32>(a, b);
-                    accd[q] = a + b;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    double a = accd[q], b = accd[q + 4];
-                    xswap<16>(a, b);
-                    accd[q] = a + b;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v4[q] = accd[q];
-                double *buf = L.px;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) buf[20 * (4 * rowq + q) + col] = v4[q];
-                __syncthreads();
-                if (rowq < NPB) {
-                    const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
-                    const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
-                    double sum = 0.0;
-                    for (int cc = c0; cc < c1; ++cc) sum += buf[20 * col + cc];
-                    const int slot = wp.slots[16 + s];
-                    // the FP32-bulk rows take their 2^(1 - zeta) here
-                    if (P == 1) sum *= wp.coef[col >> 2];
-                    if (c1 > c0 && slot >= 0) L.d[slot] += sum;
-                }
-            }
         } else {
             Bn = load_nbr(jn, species, x);
         }
@@ -1767,13 +1687,475 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         wave_reduce<4>(sv, s);
         if (C.live && (s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ips + (s >> 4)] = sv[0];
         __syncthreads();
-        if (FWD && row && s < width) row[s] = L.d[s];
         C = Cn;
         B = Bn;
         gs = gsn;
-        rin = rinn;
         ips = ipsn;
         ipe = ipen;
+    }
+}
+
+// centres [lo, hi) of this block for the paired kernel: the XCD slices of
+// tile_range (block b on XCD b % 8), each split into contiguous runs among
+// the XCD's blocks, so that consecutive centres of a run can share a wave
+struct RunRange {
+    int lo, hi;
+};
+
+__device__ __forceinline__ RunRange run_range(int n)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    RunRange r;
+    if (G >= 8 && (G & 7) == 0) {
+        const int xcd = b & 7, local = b >> 3, per = G >> 3;
+        const int chunk = (n + 7) >> 3;
+        const int lo = min(n, xcd * chunk), hi = min(n, lo + chunk);
+        const long len = hi - lo;
+        r.lo = lo + (int)(len * local / per);
+        r.hi = lo + (int)(len * (local + 1) / per);
+    } else {
+        r.lo = (int)((long)n * b / G);
+        r.hi = (int)((long)n * (b + 1) / G);
+    }
+    return r;
+}
+
+// The fused forward + adjoint kernel, two centres to a wave.  Two
+// consecutive centres of a block's run whose neighbour lists both have at
+// most 32 entries share the pair phase: lanes 0..31 take the first centre's
+// entries and lanes 32..63 the second's, and the compacted neighbours sit at
+// slots 0.. and 32.. of the neighbour arrays.  Each centre's g2 row sums,
+// pair enumeration, triplet passes, row reduction and records then run over
+// all 64 lanes, in turn, in the same lanes and order as for a centre alone,
+// so a centre's descriptors and records are bitwise the same whichever
+// centre shares its wave, or none.  A centre with more than 32 entries, or
+// next to one, or last in its run, takes all 64 lanes alone.  The pair phase
+// is about a sixth of a centre's instructions and ran at the lane occupancy
+// of the neighbour count (~26 of 64 lanes); pairing halves it.
+// The triplet lanes are dealt 4-lane columns so that the row sums reduce as
+// in the forward kernel; the descriptors are accumulated from the same pair
+// and triplet work (the reference's gradient adds the forward into desc as
+// well, src/Descriptors.cpp:165-190).
+template <int P, int NSB, int NPB, int ZG, bool DC>
+__global__ __launch_bounds__(64, P == 0 ? 2 : 3)
+void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
+                  const int *__restrict__ offs, const int *__restrict__ nl,
+                  const int *__restrict__ species, const double *__restrict__ x,
+                  const double *__restrict__ dE, const int *__restrict__ inv_pos,
+                  double *__restrict__ edge, int n_edges, double *__restrict__ desc, int width,
+                  int maxn, DevCounts dc_in, int assign_rows)
+{
+    using T = typename Real<P>::T;
+    const Real<P> R;
+    const DevCounts dc = DC ? dc_in : DevCounts();
+    extern __shared__ double lds_base[];
+    const int s = threadIdx.x;
+    const int ns = NSB == 2 ? 2 : wt.n_species;  // resolved: two species
+    const Shared S = carve_shared(lds_base, ns);
+    const Grp L = carve_group(lds_base, ns, kPair);
+    stage_shared(S, wt, s);
+    const WaveParams &wp = *S.par;
+    const int nsq = __builtin_amdgcn_readfirstlane(wp.nsq);
+    const Fast F = P == 1 ? load_fast<NPB>(S, wp) : Fast();
+    if (dc.hdr) {
+        n_centers = dc.hdr[0];
+        n_edges = dc.hdr[2];
+    }
+    const RunRange rr = run_range(n_centers);
+    if (rr.lo >= rr.hi) return;
+    auto grad_of = [&](const Ctr &C) -> double {
+        if (!C.work || s >= width) return 0.0;
+        return dc.in_row ? dc.in_row[C.c][s] : dE[(size_t)C.c * width + s];
+    };
+    auto self_pos = [&](const Ctr &C) -> int { return C.live ? inv_pos[n_edges + C.c] : 0; };
+    const bool assign = dc.out_row != nullptr || assign_rows != 0;
+    auto row_in = [&](const Ctr &C) -> double {
+        return (!assign && C.work && s < width) ? desc[(size_t)C.c * width + s] : 0.0;
+    };
+    auto row_of = [&](const Ctr &C) -> double * {
+        return C.work ? (dc.out_row ? dc.out_row[C.c] : desc + (size_t)C.c * width) : nullptr;
+    };
+    // the unit at centre c: c alone, or c and c + 1 (returns true) when both
+    // lists fit 32 lanes; past the run, centres that are not live
+    auto load_unit = [&](int c, Ctr &A, Ctr &B) -> bool {
+        A = load_centre(c < rr.hi ? c : n_centers, n_centers, centers, offs, species, x, dc);
+        B = load_centre(c + 1 < rr.hi ? c + 1 : n_centers, n_centers, centers, offs, species, x, dc);
+        return c + 1 < rr.hi && A.N <= 32 && B.N <= 32;
+    };
+    // lane s's list entry of a unit: its centre (the second for s >= 32 of a
+    // pair) and the entry's index there
+    auto entry_of = [&](bool two, const Ctr &A, const Ctr &B, int &beg, int &e) -> bool {
+        const bool hb = two && s >= 32;
+        e = two ? (s & 31) : s;
+        beg = hb ? B.beg : A.beg;
+        return e < (hb ? B.N : A.N);
+    };
+    Ctr A, B;
+    int c = rr.lo;
+    bool two = load_unit(c, A, B);
+    double gA = grad_of(A), gB = two ? grad_of(B) : 0.0;
+    double rA = row_in(A), rB = two ? row_in(B) : 0.0;
+    int ipsA = self_pos(A), ipsB = two ? self_pos(B) : 0;
+    int ipe;
+    Nbr Bv;
+    {
+        int beg, e;
+        const bool in = entry_of(two, A, B, beg, e);
+        ipe = in ? inv_pos[beg + e] : 0;
+        Bv = load_nbr(in ? nl[beg + e] : -1, species, x);
+    }
+    while (c < rr.hi) {
+        const int cn = c + (two ? 2 : 1);
+        Ctr An, Bn;
+        const bool twon = load_unit(cn, An, Bn);
+        double *rowA = row_of(A);
+        double *rowB = two ? row_of(B) : nullptr;
+        if (A.work && s < width) L.g[s] = gA;
+        if (two && B.work && s < width) L.gB[s] = gB;
+        if (s < kWaveSlots) {
+            L.d[s] = rA;
+            L.dB[s] = rB;
+        }
+        // FP32-bulk: each centre's g2 row weights per species block (past the
+        // triplet weights in L.w)
+        float *gwA = reinterpret_cast<float *>(L.w) + 48;
+        float *gwB = gwA + 16;
+        if constexpr (P == 1) {
+            const int slot = wp.slots[s & 15];
+            const double gvA = __shfl(gA, slot < 0 ? 0 : slot);
+            const double gvB = __shfl(gB, slot < 0 ? 0 : slot);
+            if (s < 8 * NSB) {
+                gwA[s] = slot >= 0 ? (float)gvA : 0.0f;
+                gwB[s] = slot >= 0 ? (float)gvB : 0.0f;
+            }
+        }
+        __syncthreads();
+
+        // ---- pairs of both centres: radial part of dE/d r_j ----
+        const bool hb = two && s >= 32;
+        const int e = two ? (s & 31) : s;
+        const int XN = hb ? B.N : A.N, Xsi = hb ? B.si : A.si;
+        bool valid = false;
+        double d0 = 0, d1 = 0, d2 = 0, r2 = 0, r = 0;
+        const int sj = Bv.sj;
+        if (e < XN) {
+            d0 = Bv.p0 - (hb ? B.x0 : A.x0);
+            d1 = Bv.p1 - (hb ? B.x1 : A.x1);
+            d2 = Bv.p2 - (hb ? B.x2 : A.x2);
+            r2 = dist2(d0, d1, d2);
+            r = sqrt(r2);
+            valid = r < S.rc[pidx(Xsi, sj, ns)];
+            if (!valid) {
+                double *ej = edge + 3 * (size_t)ipe;
+                ej[0] = 0.0;
+                ej[1] = 0.0;
+                ej[2] = 0.0;
+            }
+        }
+        const unsigned long long m = __ballot(valid);
+        const unsigned long long mA = two ? (m & 0xffffffffull) : m, mB = m & ~mA;
+        const int MA = __popcll(mA), MB = __popcll(mB);
+        // two species blocks: species-0 neighbours first, list order within each
+        const unsigned long long m1 = NSB == 2 ? __ballot(valid && sj == 1) : 0ull;
+        const int n1A = NSB == 2 ? __popcll(m1 & mA) : 0, n1B = NSB == 2 ? __popcll(m1 & mB) : 0;
+        // the enumerations' first pair codes, a phase ahead
+        const int code0A = (wt.has3 && s < ((MA * (MA - 1)) >> 1)) ? pair_codes(MA)[s] : 0;
+        const int code0B = (wt.has3 && s < ((MB * (MB - 1)) >> 1)) ? pair_codes(MB)[s] : 0;
+        T *terms = reinterpret_cast<T *>(L.l0);  // the g2 terms, until the enumerations
+        if (valid) {
+            const unsigned long long mX = hb ? mB : mA, m1X = m1 & mX;
+            const int MX = hb ? MB : MA, n1X = hb ? n1B : n1A;
+            const int pos = (hb ? 32 : 0) + (NSB == 2 ? (sj ? (MX - n1X) + below(m1X, s)
+                                                             : below(mX & ~m1X, s))
+                                                      : below(mX, s));
+            const int k = pidx(Xsi, sj, ns);
+            T fc, dfc;
+            cut_pair(r, S.rc[k], S.rcinv[k], S.rcs[k], fc, dfc, R);
+            double sum = 0.0;
+            if constexpr (P == 1)
+                sum = (double)pair_rows_f<NSB, true, true>(wp, hb ? gwB : gwA, (float)r, fc, dfc,
+                                                           sj, reinterpret_cast<float *>(terms) + 8 * pos);
+            const double *gX = hb ? L.gB : L.g;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (P == 0) {
+                    double v = 0.0;
+                    if (q < wt.nr2) {  // the centre's dE/dzeta row needs a row of the set
+                        const int slot = wp.slots[(NSB == 1 ? 0 : sj) * 8 + q];
+                        const double dr = r - wp.rs2[q];
+                        const T ex = dexp(-wp.eta2[q] * dr * dr, R);
+                        const T dphi = ex * (T)(-2.0 * wp.eta2[q] * dr) * fc + ex * dfc;
+                        sum += gX[slot] * (double)dphi;
+                        v = (double)(ex * fc);  // the g2 term, as the forward kernel forms it
+                    }
+                    terms[8 * pos + q] = (T)v;
+                }
+            }
+            // 1/r: FP32 (one Newton step) in the FP32-bulk form
+            const double ir = P == 1 ? (double)rcp_refined((float)r) : 1.0 / r;
+            if constexpr (P == 1) {
+                double2 *D = reinterpret_cast<double2 *>(L.px);
+                float4 *V = reinterpret_cast<float4 *>(D + 2 * kW);
+                float2 *W = reinterpret_cast<float2 *>(V + kW);
+                const float r2f = (float)r2;
+                D[pos] = make_double2(d0, d1);
+                D[kW + pos] = make_double2(d2, r);
+                V[pos] = make_float4((float)ir, fc, dfc, __int_as_float(sj));
+                W[pos] = make_float2(__builtin_amdgcn_exp2f(-F.etal[0] * r2f),
+                                     __builtin_amdgcn_exp2f(-F.etal[1] * r2f));
+            } else {
+                L.px[pos] = Bv.p0;
+                L.py[pos] = Bv.p1;
+                L.pz[pos] = Bv.p2;
+                L.r2[pos] = r2;
+                L.r[pos] = r;
+                L.rinv[pos] = ir;
+                L.fc[pos] = (double)fc;
+                L.dfc[pos] = (double)dfc;
+                L.sp[pos] = sj;
+            }
+            L.ip[pos] = ipe;
+            L.fx[pos] = sum * d0 * ir;
+            L.fy[pos] = sum * d1 * ir;
+            L.fz[pos] = sum * d2 * ir;
+        }
+        // the next unit's list entries
+        int jn, ipen;
+        {
+            int beg, en;
+            const bool in = entry_of(twon, An, Bn, beg, en);
+            jn = in ? nl[beg + en] : -1;
+            ipen = in ? inv_pos[beg + en] : 0;
+        }
+        __syncthreads();
+        g2_rows_sum<NSB>(terms, MA, MA - n1A, s, wp.slots, L.d);
+        if (two) g2_rows_sum<NSB>(terms + 8 * 32, MB, MB - n1B, s, wp.slots, L.dB);
+        __syncthreads();
+
+        // ---- triplets of each centre in turn, lanes split over its blocks ----
+        Nbr Bvn;
+        if (wt.has3) {
+#pragma clang loop unroll(disable)
+            for (int h = 0; h < (two ? 2 : 1); ++h) {
+                const int off = h ? 32 : 0;
+                const int M = h ? MB : MA, n1 = h ? n1B : n1A;
+                const Ctr &X = h ? B : A;
+                double *drow = h ? L.dB : L.d;
+                const double *grow = h ? L.gB : L.g;
+                int cnt[3], base[3];
+                const int code0 = h ? code0B : code0A;
+                if (wt.rc_uniform)
+                    enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base, off);
+                else
+                    enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base, off);
+                if (h == 0) Bvn = load_nbr(jn, species, x);
+                // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda
+                if (s < 16 * NPB) {
+                    const int k = s & 15, z = k >> 2, l = (k >> 1) & 1;
+                    const int slot = wp.slots[16 + s];
+                    const double ww = (X.work && slot >= 0) ? grow[slot] * wp.coef[z] : 0.0;
+                    reinterpret_cast<T *>(L.w)[s] = (T)ww;
+                    reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
+                }
+                __syncthreads();
+                // columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48; column s & 15)
+                // dealt to the blocks in proportion to their active pairs, the
+                // fewest passes I with sum_p ceil(cnt_p / 4I) <= 16 columns
+                const int nall = cnt[0] + (NPB > 1 ? cnt[1] + cnt[2] : 0);
+                int passes = (nall + kW - 1) / kW;
+                int nc[3] = {16, 0, 0};
+                const int col = s & 15, rowq = s >> 4;
+                if (NPB > 1 && nall > 0) {
+                    const int c40 = (cnt[0] + 3) >> 2, c41 = (cnt[1] + 3) >> 2, c42 = (cnt[2] + 3) >> 2;
+                    auto cols_for = [&](int I) {
+                        nc[0] = ceil_div_small(c40, I);
+                        nc[1] = ceil_div_small(c41, I);
+                        nc[2] = ceil_div_small(c42, I);
+                        return nc[0] + nc[1] + nc[2];
+                    };
+                    while (cols_for(passes) > 16) ++passes;
+                }
+                const int pl = NPB == 1 ? 0 : (col < nc[0] ? 0 : (col < nc[0] + nc[1] ? 1 : 2));
+                const int cs = pl == 0 ? 0 : (pl == 1 ? nc[0] : nc[0] + nc[1]);
+                const int ncol = pl == 0 ? nc[0] : (pl == 1 ? nc[1] : nc[2]);
+                const int u = NPB == 1 ? s : (col - cs) + rowq * ncol;
+                const int stride = NPB == 1 ? kW : 4 * ncol;
+                const bool mine = NPB == 1 || col < cs + ncol;
+                const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
+                const unsigned short *lst =
+                    L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
+                const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
+                const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
+                const int kp = blk_pair(pl);
+                const float rcl = P == 1 ? (float)S.rc[kp] : 0.0f;
+                const double rc2l = P == 1 ? S.rc2[kp] : 0.0;
+                const float rcsl = P == 1 ? (float)S.rcs[kp] : 0.0f;
+                double accd[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) accd[k] = 0.0;
+                for (int i = 0; i < passes; ++i) {
+                    const int t = u + i * stride;
+                    if (mine && t < nb) {
+                        const int code = lst[t];
+                        const int a = code & 63, b = (code >> 6) & 63;
+                        if constexpr (P == 1) {
+                            adj_triplet_f<NPB, true, ZG>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
+                        } else {
+                            Trip<P> q;
+                            triplet<P, true>(S, wp, wt, L, ns, a, b, q);
+                            const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
+                            const T fc3 = q.fa * q.fb * q.fjk;
+                            double te[2];
+#pragma unroll
+                            for (int ee = 0; ee < 2; ++ee) te[ee] = (double)(q.E[ee] * fc3);
+                            T U[2] = {(T)0, (T)0}, V[2] = {(T)0, (T)0};
+#pragma unroll
+                            for (int l = 0; l < 2; ++l) {
+                                if (l < wt.nl) {
+                                    T pm[4];
+                                    if constexpr (ZG != 0)
+                                        grid_powers64_m1<ZG>(q.B[l], pm);
+                                    else
+                                        powers_m1<P>(q.B[l], nsq, wp, wt.nz, pm);
+#pragma unroll
+                                    for (int z = 0; z < 4; ++z) {
+                                        if (z < wt.nz) {
+                                            const T pm1 = pm[z];
+                                            const T pw = pm1 * q.B[l];
+#pragma unroll
+                                            for (int ee = 0; ee < 2; ++ee) {
+                                                U[ee] += w[(z * 2 + l) * 2 + ee] * pw;
+                                                V[ee] += wv[(z * 2 + l) * 2 + ee] * pm1;
+                                            }
+                                            const double pz = (double)((T)wp.coef[z] * pw);
+#pragma unroll
+                                            for (int ee = 0; ee < 2; ++ee)
+                                                if (ee < wt.ne)
+                                                    accd[(z * 2 + l) * 2 + ee] =
+                                                        fma(pz, te[ee], accd[(z * 2 + l) * 2 + ee]);
+                                        }
+                                    }
+                                }
+                            }
+                            T SA = (T)0, SB = (T)0, SC = (T)0;
+#pragma unroll
+                            for (int ee = 0; ee < 2; ++ee) {
+                                if (ee < wt.ne) {
+                                    SA += V[ee] * q.E[ee];
+                                    const T ue = U[ee] * q.E[ee];
+                                    SB += ue;
+                                    SC += (T)wp.eta3[ee] * ue;
+                                }
+                            }
+                            const T dcij = (T)q.nij * q.inv2 * (T)q.ria;
+                            const T dcik = (T)q.nik * q.inv2 * (T)q.rib;
+                            const T dcjk = -(T)q.rjk * (T)2 * q.inv2;
+                            const T Aij = fc3 * (dcij * SA - (T)(2 * q.ra) * SC) + q.fb * q.fjk * dfa * SB;
+                            const T Aik = fc3 * (dcik * SA - (T)(2 * q.rb) * SC) + q.fa * q.fjk * dfb * SB;
+                            const T Ajk = fc3 * (dcjk * SA - (T)(2 * q.rjk) * SC) + q.fa * q.fb * q.dfjk * SB;
+                            const double a1 = (double)Aij * q.ria, a2 = (double)Aik * q.rib,
+                                         a3 = (double)Ajk * q.rjkinv;
+                            atomicAdd(&L.fx[a], a1 * (L.px[a] - X.x0) - a3 * q.e0);
+                            atomicAdd(&L.fy[a], a1 * (L.py[a] - X.x1) - a3 * q.e1);
+                            atomicAdd(&L.fz[a], a1 * (L.pz[a] - X.x2) - a3 * q.e2);
+                            atomicAdd(&L.fx[b], a2 * (L.px[b] - X.x0) + a3 * q.e0);
+                            atomicAdd(&L.fy[b], a2 * (L.py[b] - X.x1) + a3 * q.e1);
+                            atomicAdd(&L.fz[b], a2 * (L.pz[b] - X.x2) + a3 * q.e2);
+                        }
+                    }
+                }
+                __syncthreads();
+                // the row sums: a column's 4 lanes with two register swaps, then
+                // the columns of a block through LDS (the pair list's region,
+                // now read), in order, all in FP64; lane s keeps values
+                // 4 (s >> 4) .. +3
+                double v4[4];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    double a = accd[q], b = accd[q + 8];
+                    xswap<32>(a, b);
+                    accd[q] = a + b;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double a = accd[q], b = accd[q + 4];
+                    xswap<16>(a, b);
+                    accd[q] = a + b;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v4[q] = accd[q];
+                double *buf = reinterpret_cast<double *>(L.l0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) buf[16 * (4 * rowq + q) + col] = v4[q];
+                __syncthreads();
+                if (rowq < NPB) {
+                    const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
+                    const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
+                    double sum = 0.0;
+                    for (int cc = c0; cc < c1; ++cc) sum += buf[16 * col + cc];
+                    const int slot = wp.slots[16 + s];
+                    // the FP32-bulk rows take their 2^(1 - zeta) here
+                    if (P == 1) sum *= wp.coef[col >> 2];
+                    if (c1 > c0 && slot >= 0) drow[slot] += sum;
+                }
+                // the column sums are read before the next enumeration reuses the region
+                __syncthreads();
+            }
+        } else {
+            Bvn = load_nbr(jn, species, x);
+        }
+        // the next unit's rows, loaded here rather than a unit ahead: held
+        // through the triplet passes they would cost registers there
+        const double gAn = grad_of(An), gBn = twon ? grad_of(Bn) : 0.0;
+        const double rAn = row_in(An), rBn = twon ? row_in(Bn) : 0.0;
+        const int ipsAn = self_pos(An), ipsBn = twon ? self_pos(Bn) : 0;
+
+        // ---- edge records and each centre's own record ----
+        {
+            double sv[4] = {0.0, 0.0, 0.0, 0.0};
+            if (s < MA) {
+                const double f0 = L.fx[s], f1 = L.fy[s], f2 = L.fz[s];
+                double *ej = edge + 3 * (size_t)L.ip[s];
+                ej[0] = f0;
+                ej[1] = f1;
+                ej[2] = f2;
+                sv[0] = -f0;
+                sv[1] = -f1;
+                sv[2] = -f2;
+            }
+            wave_reduce<4>(sv, s);
+            if ((s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ipsA + (s >> 4)] = sv[0];
+        }
+        if (two) {
+            double sv[4] = {0.0, 0.0, 0.0, 0.0};
+            if (s < MB) {
+                const double f0 = L.fx[32 + s], f1 = L.fy[32 + s], f2 = L.fz[32 + s];
+                double *ej = edge + 3 * (size_t)L.ip[32 + s];
+                ej[0] = f0;
+                ej[1] = f1;
+                ej[2] = f2;
+                sv[0] = -f0;
+                sv[1] = -f1;
+                sv[2] = -f2;
+            }
+            wave_reduce<4>(sv, s);
+            if ((s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ipsB + (s >> 4)] = sv[0];
+        }
+        __syncthreads();
+        if (rowA && s < width) rowA[s] = L.d[s];
+        if (rowB && s < width) rowB[s] = L.dB[s];
+        c = cn;
+        A = An;
+        B = Bn;
+        two = twon;
+        gA = gAn;
+        gB = gBn;
+        rA = rAn;
+        rB = rBn;
+        ipsA = ipsAn;
+        ipsB = ipsBn;
+        ipe = ipen;
+        Bv = Bvn;
     }
 }
 
@@ -1797,7 +2179,7 @@ int wave_group_size(int max_nn, int n_species)
 {
     // one wavefront per centre, one lane per neighbour-list entry
     if (max_nn > kW) return 0;
-    return lds_for(max_nn, kFused, n_species, 0) <= kWaveMaxLds ? kW : 0;
+    return lds_for(max_nn, kPair, n_species, 0) <= kWaveMaxLds ? kW : 0;
 }
 
 size_t wave_lds_bytes(int max_nn, bool adjoint, int n_species, int group)
@@ -1969,26 +2351,34 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
     const bool fused = desc != nullptr || dc.out_row != nullptr;
-    const size_t lds = lds_for(max_nn, fused ? kFused : kAdj, wt.n_species, precision);
+    const size_t lds = lds_for(max_nn, fused ? kPair : kAdj, wt.n_species, precision);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
     const bool use_dc = dc.hdr || dc.req_flag || dc.out_row || dc.in_row;
-#define LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, DCV)                                                    \
-    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, FWD, ZG, DCV>), grid, block, lds, s, wt,          \
-                       n_centers, centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, \
-                       width, maxn, dc, assign ? 1 : 0)
-#define LDSF_ADJ(P, NSB, NPB, FWD, ZG)                          \
-    do {                                                        \
-        if (use_dc) LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, true);    \
-        else LDSF_ADJ_DC(P, NSB, NPB, FWD, ZG, false);          \
+#define LDSF_ADJ_DC(P, NSB, NPB, ZG, DCV)                                                         \
+    hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, ZG, DCV>), grid, block, lds, s, wt, n_centers,    \
+                       centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, dc)
+#define LDSF_PAIR_DC(P, NSB, NPB, ZG, DCV)                                                       \
+    hipLaunchKernelGGL((k_fused_pair<P, NSB, NPB, ZG, DCV>), grid, block, lds, s, wt, n_centers, \
+                       centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, width,   \
+                       maxn, dc, assign ? 1 : 0)
+#define LDSF_ADJ(P, NSB, NPB, FUSED, ZG)                                   \
+    do {                                                                   \
+        if (FUSED) {                                                       \
+            if (use_dc) LDSF_PAIR_DC(P, NSB, NPB, ZG, true);               \
+            else LDSF_PAIR_DC(P, NSB, NPB, ZG, false);                     \
+        } else {                                                           \
+            if (use_dc) LDSF_ADJ_DC(P, NSB, NPB, ZG, true);                \
+            else LDSF_ADJ_DC(P, NSB, NPB, ZG, false);                      \
+        }                                                                  \
     } while (0)
-#define LDSF_ADJ_P(NSB, NPB, FWD)                                \
-    do {                                                         \
-        if (precision == 0 && wt.zgrid == 1) LDSF_ADJ(0, NSB, NPB, FWD, 1);       \
-        else if (precision == 0 && wt.zgrid == 2) LDSF_ADJ(0, NSB, NPB, FWD, 2);  \
-        else if (precision == 0) LDSF_ADJ(0, NSB, NPB, FWD, 0);                   \
-        else if (wt.zgrid == 1) LDSF_ADJ(1, NSB, NPB, FWD, 1);   \
-        else if (wt.zgrid == 2) LDSF_ADJ(1, NSB, NPB, FWD, 2);   \
-        else LDSF_ADJ(1, NSB, NPB, FWD, 0);                      \
+#define LDSF_ADJ_P(NSB, NPB, FUSED)                                                \
+    do {                                                                           \
+        if (precision == 0 && wt.zgrid == 1) LDSF_ADJ(0, NSB, NPB, FUSED, 1);      \
+        else if (precision == 0 && wt.zgrid == 2) LDSF_ADJ(0, NSB, NPB, FUSED, 2); \
+        else if (precision == 0) LDSF_ADJ(0, NSB, NPB, FUSED, 0);                  \
+        else if (wt.zgrid == 1) LDSF_ADJ(1, NSB, NPB, FUSED, 1);                   \
+        else if (wt.zgrid == 2) LDSF_ADJ(1, NSB, NPB, FUSED, 2);                   \
+        else LDSF_ADJ(1, NSB, NPB, FUSED, 0);                                      \
     } while (0)
     if (fused) {
         if (wt.nsb == 1) LDSF_ADJ_P(1, 1, true); else LDSF_ADJ_P(2, 3, true);
@@ -1997,6 +2387,7 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
     }
 #undef LDSF_ADJ_P
 #undef LDSF_ADJ
+#undef LDSF_PAIR_DC
 #undef LDSF_ADJ_DC
     return hipGetLastError();
 }

@@ -153,6 +153,44 @@ def test_fused_accumulates_into_desc(c1):
     assert rel_err(G, G0) <= TOL64
 
 
+def test_centre_same_in_a_shared_wave_and_alone():
+    """The fused kernel runs two consecutive centres of at most 32 neighbours
+    in one wave (a shared pair phase); a centre's descriptors and dE/dr are
+    bitwise those of the same centre run alone.  A 40,000-atom system (more
+    centres than the plan's 16,384 blocks, so runs hold pairs); dE/dzeta is
+    zero except on the centre, so the whole-system dE/dr holds that centre's
+    records only, summed with exact zeros; a plan over the one centre (centre
+    list [c]) runs it alone."""
+    from libdescriptor_amd import _capi
+    print("case: 40,000-atom two-species, paired vs alone", flush=True)
+    S = Setup(two_species(40000, seed=21))
+    L, n, W, d = S.L, S.n, S.W, S.x.device
+    nn = np.asarray(L["numneigh"])
+    off = np.r_[0, np.cumsum(nn)]
+    t = lambda a, dt=torch.int32: torch.tensor(np.asarray(a), dtype=dt, device=d)  # noqa: E731
+    sp = t(L["species"])
+    # centres whose neighbour and its own lists fit 32 lanes (most are), the
+    # first and the last
+    small = [c for c in range(1, n - 1) if nn[c] <= 32 and nn[c - 1] <= 32 and nn[c + 1] <= 32]
+    cases = [0, small[0], small[len(small) // 2], small[-1], n - 1]
+    for prec in (0, 1):
+        for c in cases:
+            g = torch.zeros((n, W), dtype=torch.float64, device=d)
+            g[c] = S.g[c]
+            desc = torch.zeros((n, W), dtype=torch.float64, device=d)
+            dx = torch.zeros_like(S.x)
+            S.plan.forward_gradient(S.x, g, desc, dx, prec)
+            one = _capi.Plan(S.ds, len(L["coords"]), sp, t(L["neighlist"][off[c]:off[c + 1]]),
+                             t(nn[c:c + 1]), centers=t([c]))
+            d1 = torch.zeros((1, W), dtype=torch.float64, device=d)
+            dx1 = torch.zeros_like(S.x)
+            one.forward_gradient(S.x, g[c:c + 1].contiguous(), d1, dx1, prec)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(d1.cpu().numpy()[0], desc.cpu().numpy()[c])
+            np.testing.assert_array_equal(dx1.cpu().numpy(), dx.cpu().numpy())
+            assert np.abs(dx1.cpu().numpy()).max() > 0
+
+
 @pytest.mark.parametrize("family", [1, 0])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_forward_gradient_assign(family, prec):
