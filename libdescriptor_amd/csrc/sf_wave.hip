@@ -386,6 +386,7 @@ __device__ __forceinline__ void powers_m1(typename Real<P>::T b, int nsq, const 
 struct Shared {
     WaveParams *par;
     double *rc, *rcinv, *rc2, *rcs;  // rc, 1/rc, rc^2, (pi/2)/rc per species pair (ns^2 each)
+    double *rct;                     // the least x with sqrt(x) >= rc (FP64, correctly rounded)
 };
 
 // the centre's arrays: 64 entries per neighbour array, then the pair list
@@ -402,7 +403,7 @@ struct Grp {
 __host__ __device__ constexpr size_t shared_doubles(int ns)
 {
     // an even count: the neighbour arrays that follow hold 16-byte vectors
-    return ((sizeof(WaveParams) + 15) / 16) * 2 + 4 * (size_t)ns * ns;
+    return ((sizeof(WaveParams) + 15) / 16) * 2 + 6 * (size_t)ns * ns;
 }
 
 // The per-neighbour arrays sit at fixed offsets (64 entries each) from one
@@ -458,7 +459,8 @@ __device__ __forceinline__ Shared carve_shared(double *base, int ns)
     S.rc = p; p += ns * ns;
     S.rcinv = p; p += ns * ns;
     S.rc2 = p; p += ns * ns;
-    S.rcs = p;
+    S.rcs = p; p += ns * ns;
+    S.rct = p;
     return S;
 }
 
@@ -510,6 +512,12 @@ __device__ __forceinline__ void stage_shared(const Shared &S, const WaveTab &wt,
         S.rcinv[k] = 1.0 / rc;
         S.rc2[k] = rc * rc;
         S.rcs[k] = 0.5 * kPi / rc;
+        // sqrt(x) < rc exactly when x < rct: FP64 sqrt is correctly rounded
+        // and so monotone, and rct is the least double whose root reaches rc
+        double y = rc * rc;
+        while (y > 0.0 && sqrt(y) >= rc) y = nextafter(y, 0.0);
+        while (sqrt(y) < rc) y = nextafter(y, __builtin_inf());
+        S.rct[k] = y;
     }
     const int *src = reinterpret_cast<const int *>(wt.par);
     int *dst = reinterpret_cast<int *>(S.par);
@@ -872,43 +880,46 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
         const int t = t0 + s;
         const int code = next;
         if (t0 + kW < T) next = t + kW < T ? tab[t + kW] : 0;
-        bool act = false;
-        int blk = 0;
-        if (t < T) {
-            const int a = (code & 63) + off, b = (code >> 6) + off;
-            int sa, sb;
-            if constexpr (P == 0) {
-                const double rjk2 =
-                    dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
-                sa = L.sp[a];
-                sb = L.sp[b];
-                act = sqrt(rjk2) < S.rc[UNI ? 0 : pidx(sa, sb, ns)];
-            } else {
-                // the triplet phase forms the same r_jk^2 from the same operands:
-                // an active pair has r_jk^2 < rc^2 there too.  (The reference
-                // tests r_jk < rc_jk; the two differ only within an ulp of rc_jk,
-                // where fc_jk vanishes quadratically.)
-                const double2 *D = reinterpret_cast<const double2 *>(L.px);
-                const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
-                double ex, ey, ez;
-                const double rjk2 = rjk2_of(D[a], D[kW + a], D[b], D[kW + b], ex, ey, ez);
-                sa = __float_as_int(V[a].w);
-                sb = __float_as_int(V[b].w);
-                const double rc2 = (UNI || ns == 1) ? F.rq0
-                                   : (NPB > 1 ? S.rc2[blk_pair(sa + sb)]
-                                              : S.rc2[pidx(sa, sb, ns)]);
-                act = rjk2 < rc2;
-            }
-            if (NPB > 1) blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
+        // no branch on t < T: past T the code is 0, whose reads (slot off)
+        // stay inside the arrays, and the test is masked after, so that the
+        // ballots below take the compare results as they are
+        const int a = (code & 63) + off, b = (code >> 6) + off;
+        int sa, sb;
+        bool act;
+        if constexpr (P == 0) {
+            const double rjk2 = dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
+            sa = L.sp[a];
+            sb = L.sp[b];
+            // sqrt(rjk2) < rc_jk as the reference tests it, without the root
+            act = rjk2 < S.rct[UNI ? 0 : pidx(sa, sb, ns)];
+        } else {
+            // the triplet phase forms the same r_jk^2 from the same operands:
+            // an active pair has r_jk^2 < rc^2 there too.  (The reference
+            // tests r_jk < rc_jk; the two differ only within an ulp of rc_jk,
+            // where fc_jk vanishes quadratically.)
+            const double2 *D = reinterpret_cast<const double2 *>(L.px);
+            const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
+            double ex, ey, ez;
+            const double rjk2 = rjk2_of(D[a], D[kW + a], D[b], D[kW + b], ex, ey, ez);
+            sa = __float_as_int(V[a].w);
+            sb = __float_as_int(V[b].w);
+            const double rc2 = (UNI || ns == 1) ? F.rq0
+                               : (NPB > 1 ? S.rc2[blk_pair(sa + sb)] : S.rc2[pidx(sa, sb, ns)]);
+            act = rjk2 < rc2;
         }
+        act = act && t < T;
         if (NPB == 1) {
             const unsigned long long m = __builtin_amdgcn_ballot_w64(act);
             if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)(code + 65 * off);
             cnt[0] += __popcll(m);
         } else {
-            const unsigned long long m0 = __builtin_amdgcn_ballot_w64(act && blk == 0);
-            const unsigned long long m1 = __builtin_amdgcn_ballot_w64(act && blk == 1);
-            const unsigned long long m2 = __builtin_amdgcn_ballot_w64(act && blk == 2);
+            const int blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
+            // the block masks as scalar ANDs of plain ballots (a ballot of
+            // act && blk == p would be rebuilt from a per-lane boolean)
+            const unsigned long long ma = __builtin_amdgcn_ballot_w64(act);
+            const unsigned long long b0 = __builtin_amdgcn_ballot_w64(blk == 0);
+            const unsigned long long b1 = __builtin_amdgcn_ballot_w64(blk == 1);
+            const unsigned long long m0 = ma & b0, m1 = ma & b1, m2 = ma & ~(b0 | b1);
             if (act) {
                 const int p0 = base[0] + cnt[0] + below(m0, s);
                 const int p1 = base[1] + cnt[1] + below(m1, s);
