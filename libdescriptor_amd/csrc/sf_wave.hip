@@ -391,7 +391,7 @@ struct Shared {
 
 // the centre's arrays: 64 entries per neighbour array, then the pair list
 struct Grp {
-    double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates
+    double *px, *py, *pz, *r2, *r, *rinv, *fc;  // absolute neighbour coordinates (r2: unused slot)
     double *dfc, *fx, *fy, *fz, *w, *wv, *g;    // adjoint only (w, wv: row weights of up to
                                                 // 3 blocks; g: the centre's dE/dzeta row)
     double *d;                                  // forward only: the centre's descriptor row
@@ -933,7 +933,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
     }
 }
 
-// shared quantities of one active triplet (a, b)
+// shared quantities of one active triplet (a, b) of the FP64 form
 template <int P>
 struct Trip {
     using T = typename Real<P>::T;
@@ -945,12 +945,17 @@ struct Trip {
     double nij, nik;  // r_ij^2 - r_ik^2 + r_jk^2 and r_ik^2 - r_ij^2 + r_jk^2
 };
 
+// sym_g4's own arithmetic and operation order in FP64
+// (src/SymmetryFunctions.cpp:101-127).  The FP64 form keeps it, the cutoff
+// 0.5 (cos(pi r_jk / rc) + 1) included: near rc that value carries ~1e-16
+// absolute rounding, and a species block whose value one such triplet
+// dominates must reproduce it to stay within 1e-12 of the reference.
 template <int P, bool ADJ>
 __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, const WaveTab &wt,
                                         const Grp &L, int ns, int a, int b, Trip<P> &t)
 {
+    static_assert(P == 0, "the FP32-bulk form takes adj_triplet_f and the forward's own path");
     using T = typename Real<P>::T;
-    const Real<P> R;
     const int k = pidx(L.sp[a], L.sp[b], ns);
     const double rc = S.rc[k], rcinv = S.rcinv[k];
     t.ra = L.r[a];
@@ -966,55 +971,24 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
     const double rjk2 = dist2(t.e0, t.e1, t.e2);
     t.B[0] = t.B[1] = (T)0;
     t.E[0] = t.E[1] = (T)0;
-    if (P == 0) {
-        // sym_g4's own arithmetic and operation order (src/SymmetryFunctions.cpp:112-125)
-        t.rjk = sqrt(rjk2);
-        const RefTrip rt = ref_trip(t.ra, t.rb, t.rjk);
+    t.rjk = sqrt(rjk2);
+    const RefTrip rt = ref_trip(t.ra, t.rb, t.rjk);
 #pragma unroll
-        for (int l = 0; l < 2; ++l)
-            if (l < wt.nl) t.B[l] = (T)ref_base(wp.lam[l], rt.cosv);
+    for (int l = 0; l < 2; ++l)
+        if (l < wt.nl) t.B[l] = (T)ref_base(wp.lam[l], rt.cosv);
 #pragma unroll
-        for (int e = 0; e < 2; ++e)
-            if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * rt.s2);
-        double sn, cs;
-        sincos(ref_arg(t.rjk, rc), &sn, &cs);
-        t.fjk = (T)(0.5 * (cs + 1.0));
-        t.dfjk = (T)(-0.5 * sn * (kPi * rcinv));
-        const double h = 0.5 * t.ria * t.rib;
-        t.inv2 = (T)h;
-        if (ADJ) {
-            t.rjkinv = 1.0 / t.rjk;
-            t.nij = rt.rijsq - rt.riksq + rt.rjksq;
-            t.nik = rt.riksq - rt.rijsq + rt.rjksq;
-        }
-    } else {
-        const double r2a = L.r2[a], r2b = L.r2[b];
-        const float rjkf = __builtin_amdgcn_sqrtf((float)rjk2);
-        t.rjk = rjkf;
-        const double h = 0.5 * t.ria * t.rib;
-        const float inv = (float)h;
-        // 1 + lambda cos = ((1 + lambda)/2 (r_ij + r_ik)^2 - (1 - lambda)/2 (r_ij - r_ik)^2
-        //                   - lambda r_jk^2) / (2 r_ij r_ik): for lambda = +1 / -1 the
-        // cancellation-free forms ((r_ij + r_ik)^2 - r_jk^2) and (r_jk^2 - (r_ij - r_ik)^2)
-        const double sp = (t.ra + t.rb) * (t.ra + t.rb), sm = (t.ra - t.rb) * (t.ra - t.rb);
-#pragma unroll
-        for (int l = 0; l < 2; ++l)
-            t.B[l] = (float)(wp.lp[l] * sp - wp.lm[l] * sm - wp.lam[l] * rjk2) * inv;
-        const double S2 = r2a + r2b + rjk2;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) t.E[e] = dexp(-wp.eta3[e] * S2, R);
-        // rc - r_jk = (rc^2 - r_jk^2) / (rc + r_jk) keeps its relative accuracy near rc
-        const float u = (float)(S.rc2[k] - rjk2) * __builtin_amdgcn_rcpf((float)rc + rjkf);
-        float sn, cs;
-        sincos_quarter(u * (float)S.rcs[k], sn, cs);
-        t.fjk = sn * sn;
-        t.dfjk = -(float)(2.0 * S.rcs[k]) * sn * cs;
-        t.inv2 = inv;
-        if (ADJ) {
-            t.rjkinv = (double)rcp_refined(rjkf);
-            t.nij = r2a - r2b + rjk2;
-            t.nik = r2b - r2a + rjk2;
-        }
+    for (int e = 0; e < 2; ++e)
+        if (e < wt.ne) t.E[e] = (T)exp(-wp.eta3[e] * rt.s2);
+    double sn, cs;
+    sincos(ref_arg(t.rjk, rc), &sn, &cs);
+    t.fjk = (T)(0.5 * (cs + 1.0));
+    t.dfjk = (T)(-0.5 * sn * (kPi * rcinv));
+    const double h = 0.5 * t.ria * t.rib;
+    t.inv2 = (T)h;
+    if (ADJ) {
+        t.rjkinv = 1.0 / t.rjk;
+        t.nij = rt.rijsq - rt.riksq + rt.rjksq;
+        t.nik = rt.riksq - rt.rijsq + rt.rjksq;
     }
 }
 
@@ -1217,7 +1191,6 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 L.px[pos] = B.p0;
                 L.py[pos] = B.p1;
                 L.pz[pos] = B.p2;
-                L.r2[pos] = r2;
                 L.r[pos] = r;
                 L.rinv[pos] = 1.0 / r;
                 L.fc[pos] = (double)fc;
@@ -1436,7 +1409,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
 }
 
 // The adjoint kernel (the fused forward + adjoint is k_fused_pair).
-// Occupancy floors (waves per SIMD): FP64 1, FP32-bulk 4 (128 VGPRs).
+// Occupancy floors (waves per SIMD): FP64 none (168 VGPRs: 3), FP32-bulk 4 (128 VGPRs).
 template <int P, int NSB, int NPB, int ZG, bool DC>
 __global__ __launch_bounds__(64, P == 0 ? 1 : 4)
 void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
@@ -1500,7 +1473,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             d2 = B.p2 - C.x2;
             r2 = dist2(d0, d1, d2);
             r = sqrt(r2);
-            valid = r < S.rc[pidx(C.si, sj, ns)];
+            valid = r < S.rc[pidx(C.si, sj, ns)];  // r > rc is skipped; r == rc adds exactly 0
             if (!valid) {
                 double *ej = edge + 3 * (size_t)ipe;
                 ej[0] = 0.0;
@@ -1550,7 +1523,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 L.px[pos] = B.p0;
                 L.py[pos] = B.p1;
                 L.pz[pos] = B.p2;
-                L.r2[pos] = r2;
                 L.r[pos] = r;
                 L.rinv[pos] = ir;
                 L.fc[pos] = (double)fc;
@@ -1855,7 +1827,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
             d2 = Bv.p2 - (hb ? B.x2 : A.x2);
             r2 = dist2(d0, d1, d2);
             r = sqrt(r2);
-            valid = r < S.rc[pidx(Xsi, sj, ns)];
+            valid = r < S.rc[pidx(Xsi, sj, ns)];  // r > rc is skipped; r == rc adds exactly 0
             if (!valid) {
                 double *ej = edge + 3 * (size_t)ipe;
                 ej[0] = 0.0;
@@ -1918,7 +1890,6 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 L.px[pos] = Bv.p0;
                 L.py[pos] = Bv.p1;
                 L.pz[pos] = Bv.p2;
-                L.r2[pos] = r2;
                 L.r[pos] = r;
                 L.rinv[pos] = ir;
                 L.fc[pos] = (double)fc;
