@@ -235,6 +235,56 @@ __device__ __forceinline__ void g2_rows_sum(const T *terms, int M, int n0, int s
     }
 }
 
+// The same sums over 32 lanes, for a centre of at most 32 compacted
+// neighbours: each half-wave serves one centre (its terms at `terms`, M
+// neighbours, n0 of species 0, descriptor row `lrow`; on == false: a half
+// without a centre), lane s & 31 adding row s & 7 over one part of its block
+// (2 parts with two blocks, 4 with one), the parts then added across lane
+// bits 3 and 4 only.  A centre's sums are the same in either half.
+template <int NSB, typename T>
+__device__ __forceinline__ void g2_rows_sum_half(const T *terms, int M, int n0, int s,
+                                                 const int *slots, double *lrow, bool on)
+{
+    const int ls = s & 31, q = ls & 7;
+    int lo = 0, hi = M, part;
+    if constexpr (NSB == 2) {
+        const bool b1 = (ls & 8) != 0;
+        lo = b1 ? n0 : 0;
+        hi = b1 ? M : n0;
+        part = ls >> 4;
+    } else {
+        part = ls >> 3;
+    }
+    constexpr int kParts = NSB == 2 ? 2 : 4;
+    const int len = hi - lo;
+    const int a0 = lo + ((len * part) / kParts), a1 = lo + ((len * (part + 1)) / kParts);
+    double sum = 0.0;
+    for (int a = a0; a < a1; a += 4) {
+        T t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = a + k < a1 ? terms[8 * (a + k) + q] : (T)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sum += (double)t[k];
+    }
+    if constexpr (NSB == 1) {
+        // lane ^ 8 within its 16-lane row: DPP row_ror:8
+        const unsigned long long u = __double_as_longlong(sum);
+        const int lo32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x128, 0xF, 0xF, false);
+        const int hi32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false);
+        sum += __longlong_as_double((long long)(((unsigned long long)(unsigned)hi32 << 32) |
+                                                (unsigned)lo32));
+    }
+    {
+        double a = sum, b = sum;
+        xswap<16>(a, b);
+        sum = a + b;
+    }
+    if (on && ls < (NSB == 2 ? 16 : 8)) {
+        const int slot = slots[NSB == 2 ? (ls & 15) : q];
+        if (slot >= 0) lrow[slot] += sum;
+    }
+}
+
 // species-pair index a * ns + b (ns is small: v_mul_u32_u24, or a shift
 // where ns is a compile-time 2)
 __device__ __forceinline__ int pidx(int a, int b, int ns)
@@ -1910,8 +1960,16 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
             ipen = in ? inv_pos[beg + en] : 0;
         }
         __syncthreads();
-        g2_rows_sum<NSB>(terms, MA, MA - n1A, s, wp.slots, L.d);
-        if (two) g2_rows_sum<NSB>(terms + 8 * 32, MB, MB - n1B, s, wp.slots, L.dB);
+        // a centre of at most 32 compacted neighbours sums its g2 rows (and
+        // its own record, below) over 32 lanes, one half-wave per centre; a
+        // larger one, never paired, over all 64
+        const bool halves = two || MA <= 32;
+        if (halves)
+            g2_rows_sum_half<NSB>(terms + (hb ? 8 * 32 : 0), hb ? MB : MA,
+                                  hb ? MB - n1B : MA - n1A, s, wp.slots, hb ? L.dB : L.d,
+                                  s < 32 || two);
+        else
+            g2_rows_sum<NSB>(terms, MA, MA - n1A, s, wp.slots, L.d);
         __syncthreads();
 
         // ---- triplets of each centre in turn, lanes split over its blocks ----
@@ -2093,9 +2151,10 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         const int ipsAn = self_pos(An), ipsBn = twon ? self_pos(Bn) : 0;
 
         // ---- edge records and each centre's own record ----
+        // (lane s holds slot s: the first centre's slots 0.., the second's 32..)
         {
             double sv[4] = {0.0, 0.0, 0.0, 0.0};
-            if (s < MA) {
+            if (s < 32 ? s < MA : (halves ? (two && s - 32 < MB) : s < MA)) {
                 const double f0 = L.fx[s], f1 = L.fy[s], f2 = L.fz[s];
                 double *ej = edge + 3 * (size_t)L.ip[s];
                 ej[0] = f0;
@@ -2105,23 +2164,15 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 sv[1] = -f1;
                 sv[2] = -f2;
             }
-            wave_reduce<4>(sv, s);
-            if ((s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ipsA + (s >> 4)] = sv[0];
-        }
-        if (two) {
-            double sv[4] = {0.0, 0.0, 0.0, 0.0};
-            if (s < MB) {
-                const double f0 = L.fx[32 + s], f1 = L.fy[32 + s], f2 = L.fz[32 + s];
-                double *ej = edge + 3 * (size_t)L.ip[32 + s];
-                ej[0] = f0;
-                ej[1] = f1;
-                ej[2] = f2;
-                sv[0] = -f0;
-                sv[1] = -f1;
-                sv[2] = -f2;
+            if (halves) {
+                Halve<4, 2, kW / 4>::run(sv, s);  // each half-wave's total of value (s >> 3) & 3
+                const int v = (s >> 3) & 3;
+                if ((s & 7) == 0 && v < 3 && (s < 32 || two))
+                    edge[3 * (size_t)(s < 32 ? ipsA : ipsB) + v] = sv[0];
+            } else {
+                wave_reduce<4>(sv, s);
+                if ((s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ipsA + (s >> 4)] = sv[0];
             }
-            wave_reduce<4>(sv, s);
-            if ((s & 15) == 0 && (s >> 4) < 3) edge[3 * (size_t)ipsB + (s >> 4)] = sv[0];
         }
         __syncthreads();
         if (rowA && s < width) rowA[s] = L.d[s];
