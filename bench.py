@@ -229,7 +229,7 @@ def main():
     dom_flop = fwd_flop + adj_flop
     dom_ms = r32["fused_ms"]
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
-    dom_kernel = "k_adj_wave" if plan.kernel_family == 1 else "k_fwd_general+k_adj_general"
+    dom_kernel = "k_fused_pair" if plan.kernel_family == 1 else "k_fwd_general+k_adj_general"
     traffic = measured_traffic(dom_kernel, n) if args.n == 262144 else None
     # the same kernel in the FP64 form, against the FP64 vector peak
     dom64_ms = res["fp64"]["fused_ms"]
