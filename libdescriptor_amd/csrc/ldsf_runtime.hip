@@ -161,6 +161,7 @@ struct ldsf_plan {
     int *in_start = nullptr;
     int *in_edges = nullptr;
     int *inv_pos = nullptr;
+    int *nlp = nullptr;            // wave kernels: the list with each neighbour's species code
     double *edge = nullptr;
     bool edge_owned = true;        // false: a caller's buffer (ldsf_plan_set_records)
     ldsf::WaveParams *wave_par = nullptr;
@@ -173,7 +174,7 @@ struct ldsf_plan {
         (void)hipSetDevice(device);
         if (!borrowed)
             for (void *p : {(void *)centers, (void *)offs, (void *)in_start, (void *)in_edges,
-                            (void *)inv_pos, (void *)wave_par})
+                            (void *)inv_pos, (void *)nlp, (void *)wave_par})
                 if (p) (void)hipFree(p);
         if (edge_owned && edge) (void)hipFree(edge);
         (void)hipSetDevice(cur);
@@ -184,7 +185,7 @@ struct ldsf_plan {
 // host-pointer calls build one plan per call, in one arena per descriptor
 // and device, which keeps the largest sizes asked of it.
 struct PlanArena {
-    ldsf::Grow<int> centers, offs, flags, count, in_start, in_edges, inv_pos, stage;
+    ldsf::Grow<int> centers, offs, flags, count, in_start, in_edges, inv_pos, stage, nlp;
     ldsf::Grow<char> tmp;
     ldsf::Grow<double> edge;
     ldsf::Grow<ldsf::WaveParams> wpar;
@@ -319,11 +320,28 @@ int plan_build(ldsf_plan **out, const ldsf_sf *sf, int n_centers, int *owned_cen
         plan->wave.rc = tab.rc;
         plan->wave.par = plan->wave_par;
         plan->wave_ok = true;
+        // each entry with its neighbour's species code: the wave kernels skip
+        // the per-neighbour species gather (the list is validated above)
+        if (n_total <= ldsf::kNlIndexMask + 1 && plan->n_edges > 0) {
+            if (arena) {
+                HIP_TRY(arena->nlp.reserve(plan->n_edges), "hipMalloc(packed list)");
+                plan->nlp = arena->nlp.p;
+            } else {
+                HIP_TRY(hipMalloc((void **)&plan->nlp, sizeof(int) * plan->n_edges),
+                        "hipMalloc(packed list)");
+            }
+            HIP_TRY(ldsf::launch_pack_species(plan->n_edges, d_nl, d_species, plan->nlp, s),
+                    "packed list");
+            plan->wave.nl_species = 1;
+        }
     }
     plan->family = plan->wave_ok ? 1 : 0;
     *out = guard.release();
     return LDSF_OK;
 }
+
+// the neighbour list the wave kernels read: the packed one when the plan has it
+const int *wave_nl(const ldsf_plan *plan) { return plan->nlp ? plan->nlp : plan->nl; }
 
 int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, hipStream_t s)
 {
@@ -332,7 +350,7 @@ int plan_compute(ldsf_plan *plan, const double *x, double *desc, int precision, 
     hipError_t e;
     if (plan->family == 1 && plan->wave_ok && ldsf::wave_runs(plan->wave, precision))
         e = ldsf::launch_fwd_wave(plan->wave, precision, plan->n_centers, plan->centers,
-                                  plan->offs, plan->nl, plan->species, x, desc, tab.width,
+                                  plan->offs, wave_nl(plan), plan->species, x, desc, tab.width,
                                   plan->max_nn, plan->wave_group, s);
     else
         e = ldsf::launch_fwd_general(tab, precision, plan->n_centers, plan->centers, plan->offs,
@@ -353,7 +371,7 @@ int plan_adjoint_edges(ldsf_plan *plan, const double *x, const double *dE, int p
     hipError_t e;
     if (plan->family == 1 && plan->wave_ok && ldsf::wave_runs(plan->wave, precision))
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
-                                  plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
+                                  plan->offs, wave_nl(plan), plan->species, x, dE, plan->inv_pos,
                                   plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,
                                   plan->wave_group, s);
     else
@@ -378,7 +396,7 @@ int plan_forward_adjoint_edges(ldsf_plan *plan, const double *x, const double *d
     hipError_t e;
     if (plan->family == 1 && plan->wave_ok && ldsf::wave_runs(plan->wave, precision)) {
         e = ldsf::launch_adj_wave(plan->wave, precision, plan->n_centers, plan->centers,
-                                  plan->offs, plan->nl, plan->species, x, dE, plan->inv_pos,
+                                  plan->offs, wave_nl(plan), plan->species, x, dE, plan->inv_pos,
                                   plan->edge, (int)plan->n_edges, tab.width, plan->max_nn,
                                   plan->wave_group, s, ldsf::DevCounts(), desc, assign);
     } else {

@@ -76,11 +76,17 @@ struct WaveTab {
     int zgrid;              // zeta grid (1, 2, 4, 16) = 1, (1, 2, 4, 8) = 2, else 0: the FP32-bulk
                             // kernels' compile-time squaring chains
     int fp32_exp_ok;        // every exponent the FP32-bulk form takes in FP32 is <= kFp32ExpMax
+    int nl_species;         // the neighbour-list entries carry the neighbour's species code
+                            // (entry = row | species << kNlSpeciesShift, a plan's packed list)
     const double *rc;       // device, n_species^2
     const WaveParams *par;  // device
 };
 constexpr int kWaveSlots = 64;
 constexpr int kMaxWaveSpecies = 8;
+// A plan's packed neighbour list for the wave kernels: the row in the low 28
+// bits, the species code (< kMaxWaveSpecies) above, so entries stay positive.
+constexpr int kNlSpeciesShift = 28;
+constexpr int kNlIndexMask = (1 << kNlSpeciesShift) - 1;
 // Largest exponent eta (r - Rs)^2 (g2) or eta (r_ij^2 + r_ik^2 + r_jk^2) (g4)
 // that the FP32-bulk wave kernels form in FP32; sets that reach beyond it
 // over the cutoff sphere run the FP32-bulk general kernels, which split the
@@ -149,6 +155,9 @@ hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, 
 hipError_t launch_gather(int n_total, const int *in_start, const double *edge, double *dcoords,
                          hipStream_t s, const int *skip = nullptr, bool assign = false);
 hipError_t launch_max_int(int n, const int *v, int *out, hipStream_t s);
+// out[e] = nl[e] | species[nl[e]] << kNlSpeciesShift for validated entries
+hipError_t launch_pack_species(long n_edges, const int *nl, const int *species, int *out,
+                               hipStream_t s);
 // v[0 .. n) = 0 by a kernel (no memset node where a caller captures a graph)
 hipError_t launch_zero_f64(long n, double *v, hipStream_t s);
 

@@ -268,6 +268,29 @@ hipError_t launch_sort_inverse(int n_total, const int *in_start, int *in_edges, 
     return hipGetLastError();
 }
 
+// out[e] = nl[e] | species[nl[e]] << kNlSpeciesShift: the wave kernels read a
+// neighbour's species from its list entry instead of gathering it (indices
+// validated before, in [0, n_total) with n_total <= kNlIndexMask + 1)
+__global__ void k_pack_species(long n_edges, const int *__restrict__ nl,
+                               const int *__restrict__ species, int *__restrict__ out)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n_edges;
+         e += (long)gridDim.x * blockDim.x) {
+        const int j = nl[e];
+        out[e] = j | (species[j] << kNlSpeciesShift);
+    }
+}
+
+hipError_t launch_pack_species(long n_edges, const int *nl, const int *species, int *out,
+                               hipStream_t s)
+{
+    if (n_edges <= 0) return hipSuccess;
+    const long blocks = std::min<long>((n_edges + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_pack_species, dim3((unsigned)blocks), dim3(kBlock), 0, s, n_edges, nl,
+                       species, out);
+    return hipGetLastError();
+}
+
 __global__ void k_zero_f64(long n, double *__restrict__ v)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)

@@ -619,13 +619,20 @@ __device__ __forceinline__ int load_index(const Ctr &C, int s, const int *nl)
     return s < C.N ? nl[C.beg + s] : -1;
 }
 
-__device__ __forceinline__ Nbr load_nbr(int j, const int *species, const double *x)
+// (packed: the entry j carries the neighbour's species code above its row,
+// a plan's list; else the species array is read)
+__device__ __forceinline__ Nbr load_nbr(int j, const int *species, const double *x, bool packed)
 {
     Nbr B;
     B.sj = 0;
     B.p0 = B.p1 = B.p2 = 0.0;
     if (j >= 0) {
-        B.sj = species[j];
+        if (packed) {
+            B.sj = j >> kNlSpeciesShift;
+            j &= kNlIndexMask;
+        } else {
+            B.sj = species[j];
+        }
         B.p0 = x[3 * (size_t)j];
         B.p1 = x[3 * (size_t)j + 1];
         B.p2 = x[3 * (size_t)j + 2];
@@ -1190,7 +1197,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
     // the first centre's loads, in order
     Ctr C = load_centre(tr.first, n_centers, centers, offs, species, x, dc);
     double rin = row_in(C);
-    Nbr B = load_nbr(load_index(C, s, nl), species, x);
+    Nbr B = load_nbr(load_index(C, s, nl), species, x, wt.nl_species != 0);
     for (int c = tr.first; c < tr.hi; c += tr.step) {
         const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
         const double rinn = row_in(Cn);
@@ -1269,7 +1276,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base);
             else
                 enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base);
-            Bn = load_nbr(jn, species, x);
+            Bn = load_nbr(jn, species, x, wt.nl_species != 0);
             __syncthreads();
             // the blocks are dealt columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48;
             // column s & 15) in proportion to their active pairs: the fewest
@@ -1448,7 +1455,7 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                 }
             }
         } else {
-            Bn = load_nbr(jn, species, x);
+            Bn = load_nbr(jn, species, x, wt.nl_species != 0);
         }
         __syncthreads();
         if (row && s < width) row[s] = L.d[s];
@@ -1496,7 +1503,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
     double gs = grad_of(C);
     int ips = self_pos(C);
     int ipe = s < C.N ? inv_pos[C.beg + s] : 0;  // record position of lane s's edge
-    Nbr B = load_nbr(load_index(C, s, nl), species, x);
+    Nbr B = load_nbr(load_index(C, s, nl), species, x, wt.nl_species != 0);
     for (int c = tr.first; c < tr.hi; c += tr.step) {
         const Ctr Cn = load_centre(c + tr.step, n_centers, centers, offs, species, x, dc);
         const double gsn = grad_of(Cn);
@@ -1596,7 +1603,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base);
             else
                 enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base);
-            Bn = load_nbr(jn, species, x);
+            Bn = load_nbr(jn, species, x, wt.nl_species != 0);
             // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda,
             // kept in the compute type (a loop-invariant copy costs half the registers)
             if (s < 16 * NPB) {
@@ -1702,7 +1709,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             }
             __syncthreads();
         } else {
-            Bn = load_nbr(jn, species, x);
+            Bn = load_nbr(jn, species, x, wt.nl_species != 0);
         }
 
         // ---- edge records and the centre's own record ----
@@ -1835,7 +1842,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         int beg, e;
         const bool in = entry_of(two, A, B, beg, e);
         ipe = in ? inv_pos[beg + e] : 0;
-        Bv = load_nbr(in ? nl[beg + e] : -1, species, x);
+        Bv = load_nbr(in ? nl[beg + e] : -1, species, x, wt.nl_species != 0);
     }
     while (c < rr.hi) {
         const int cn = c + (two ? 2 : 1);
@@ -1988,7 +1995,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                     enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base, off);
                 else
                     enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base, off);
-                if (h == 0) Bvn = load_nbr(jn, species, x);
+                if (h == 0) Bvn = load_nbr(jn, species, x, wt.nl_species != 0);
                 // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda
                 if (s < 16 * NPB) {
                     const int k = s & 15, z = k >> 2, l = (k >> 1) & 1;
@@ -2142,7 +2149,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 __syncthreads();
             }
         } else {
-            Bvn = load_nbr(jn, species, x);
+            Bvn = load_nbr(jn, species, x, wt.nl_species != 0);
         }
         // the next unit's rows, loaded here rather than a unit ahead: held
         // through the triplet passes they would cost registers there
