@@ -2074,7 +2074,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                                                 U[ee] += w[(z * 2 + l) * 2 + ee] * pw;
                                                 V[ee] += wv[(z * 2 + l) * 2 + ee] * pm1;
                                             }
-                                            const double pz = (double)((T)wp.coef[z] * pw);
+                                            const double pz = (double)pw;  // 2^(1 - zeta) at the row sum
 #pragma unroll
                                             for (int ee = 0; ee < 2; ++ee)
                                                 if (ee < wt.ne)
@@ -2141,8 +2141,9 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                     double sum = 0.0;
                     for (int cc = c0; cc < c1; ++cc) sum += buf[16 * col + cc];
                     const int slot = wp.slots[16 + s];
-                    // the FP32-bulk rows take their 2^(1 - zeta) here
-                    if (P == 1) sum *= wp.coef[col >> 2];
+                    // each row takes its 2^(1 - zeta) here (a power of two for the integer
+                    // zeta of the grids: the same bits as scaling every term)
+                    sum *= wp.coef[col >> 2];
                     if (c1 > c0 && slot >= 0) drow[slot] += sum;
                 }
                 // the column sums are read before the next enumeration reuses the region
