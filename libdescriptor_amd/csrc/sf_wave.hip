@@ -466,9 +466,22 @@ __host__ __device__ constexpr size_t shared_doubles(int ns)
 // one per-lane address.  The fused kind holds two centres at a time: the
 // second centre's neighbours at slots 32..63, its dE/dzeta and descriptor
 // rows (gB, dB) after the neighbour arrays.
+// The fused kind's three pair blocks' 16 row weights sit kWBlk elements
+// apart: its columns of lanes are dealt to the blocks across each 16-lane row,
+// so the lanes of one LDS read serve several blocks, each reading the same
+// element of its own block, and with a stride of 16 elements those addresses
+// would share banks (FP32: blocks 0 and 2; FP64: all three).  With 20 they
+// fall on banks 0, 20, 8 (FP32) and 0, 8, 16 (FP64); 80 and 160 bytes keep
+// each block 16-byte aligned.  The adjoint kind deals runs of consecutive
+// lanes to the blocks (only the 16-lane rows at a run's end mix two blocks)
+// and keeps them 16 apart.
+constexpr int kWBlk = 20;
+constexpr int kWEnd = 2 * kWBlk + 16;                          // doubles of w and of wv (fused)
+constexpr int kWBlkAdj = 16;
+constexpr int kWEndAdj = 3 * kWBlkAdj;                         // (adjoint)
 constexpr int kStr = kW + 1;
-constexpr int kHeadAdj = 48 + 48 + kWaveSlots;                 // w, wv, g
-constexpr int kHeadPair = kHeadAdj + kWaveSlots;               // ... and d
+constexpr int kHeadAdj = kWEndAdj + kWEndAdj + kWaveSlots;     // w, wv, g
+constexpr int kHeadPair = kWEnd + kWEnd + 2 * kWaveSlots;      // w, wv, g, d
 constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
 constexpr int kAdjDoubles = kHeadAdj + 11 * kStr + 1;          // w, wv, g, px..fz, pad
 constexpr int kPairRows = kHeadPair + 11 * kStr + 1;           // w, wv, g, d, px..fz, pad
@@ -482,9 +495,16 @@ __host__ __device__ constexpr int kind_doubles(int kind)
     return kind == kFwd ? kFwdDoubles : (kind == kAdj ? kAdjDoubles : kPairDoubles);
 }
 
+// The column sums of the triplet passes: 16 values of 16 columns, value k of
+// column c at kColStr k + c.  The stores (16 lanes: one value, 16 columns)
+// and the row sums (16 lanes: 16 values, one column) each fall on 16
+// different bank pairs with a stride of 17 doubles; 16 would put a row sum's
+// 16 reads on one bank.
+constexpr int kColStr = 17;
+
 // The paired kind's pair-list region also holds the g2 terms of both centres
-// (slots 0..63) and, after a centre's triplet passes, its 16 x 16 column sums.
-constexpr size_t kPairListMin = 16 * 16 * sizeof(double);
+// (slots 0..63) and, after a centre's triplet passes, its column sums.
+constexpr size_t kPairListMin = 16 * kColStr * sizeof(double);
 
 // The pair list's region also holds, between the pair phase and the pair
 // enumeration, the g2 terms of every compacted neighbour (8 per neighbour,
@@ -521,10 +541,11 @@ __device__ __forceinline__ Grp carve_group(double *base, int ns, int kind)
     Grp L;
     double *q = p;
     if (adjoint) {
+        const int wend = kind == kPair ? kWEnd : kWEndAdj;
         L.w = p;  // read as float4 / double2
-        L.wv = p + 48;
-        L.g = p + 96;
-        L.d = kind == kPair ? p + kHeadAdj : nullptr;
+        L.wv = p + wend;
+        L.g = p + 2 * wend;
+        L.d = kind == kPair ? p + 2 * wend + kWaveSlots : nullptr;
         q = p + (kind == kPair ? kHeadPair : kHeadAdj);  // even: 16-byte aligned
     }
     L.gB = kind == kPair ? p + kPairRows : nullptr;
@@ -1436,18 +1457,17 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
                     for (int q = 0; q < 4; ++q) v4[q] = acc3[q];
                 }
                 // the neighbour arrays are dead here: value k of column c at
-                // buf[20 k + c] (20 doubles a value: the 64 lanes' stores fall on
-                // two bank sets), then lane (p, k) = (s >> 4, s & 15) adds its
+                // buf[kColStr k + c], then lane (p, k) = (s >> 4, s & 15) adds its
                 // block's columns in order
                 double *buf = L.px;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) buf[20 * (4 * rowq + q) + col] = v4[q];
+                for (int q = 0; q < 4; ++q) buf[kColStr * (4 * rowq + q) + col] = v4[q];
                 __syncthreads();
                 if (rowq < NPB) {
                     const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
                     const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
                     double sum = 0.0;
-                    for (int c = c0; c < c1; ++c) sum += buf[20 * col + c];
+                    for (int c = c0; c < c1; ++c) sum += buf[kColStr * col + c];
                     const int slot = wp.slots[16 + s];
                     // the FP32-bulk rows take their 2^(1 - zeta) here
                     if (P == 1) sum *= wp.coef[col >> 2];
@@ -1512,7 +1532,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         if (C.work && s < width) L.g[s] = gs;
         // FP32-bulk: the g2 rows' weights per species block, from the lanes
         // that hold them (past the triplet weights in L.w)
-        float *gw = reinterpret_cast<float *>(L.w) + 48;
+        float *gw = reinterpret_cast<float *>(L.w) + kWEndAdj;
         if constexpr (P == 1) {
             const int slot = wp.slots[s & 15];
             const double gv = __shfl(gs, slot < 0 ? 0 : slot);
@@ -1636,8 +1656,8 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
             const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
             const unsigned short *lst =
                 L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
-            const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
-            const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
+            const T *w = reinterpret_cast<const T *>(L.w) + kWBlkAdj * pl;
+            const T *wv = reinterpret_cast<const T *>(L.wv) + kWBlkAdj * pl;
             // this lane's species-pair cutoff (FP32-bulk)
             const int kp = blk_pair(pl);
             const float rcl = P == 1 ? (float)S.rc[kp] : 0.0f;
@@ -1858,7 +1878,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         }
         // FP32-bulk: each centre's g2 row weights per species block (past the
         // triplet weights in L.w)
-        float *gwA = reinterpret_cast<float *>(L.w) + 48;
+        float *gwA = reinterpret_cast<float *>(L.w) + kWEnd;
         float *gwB = gwA + 16;
         if constexpr (P == 1) {
             const int slot = wp.slots[s & 15];
@@ -2001,8 +2021,9 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                     const int k = s & 15, z = k >> 2, l = (k >> 1) & 1;
                     const int slot = wp.slots[16 + s];
                     const double ww = (X.work && slot >= 0) ? grow[slot] * wp.coef[z] : 0.0;
-                    reinterpret_cast<T *>(L.w)[s] = (T)ww;
-                    reinterpret_cast<T *>(L.wv)[s] = (T)(ww * wp.zeta[z] * wp.lam[l]);
+                    const int at = kWBlk * (s >> 4) + k;
+                    reinterpret_cast<T *>(L.w)[at] = (T)ww;
+                    reinterpret_cast<T *>(L.wv)[at] = (T)(ww * wp.zeta[z] * wp.lam[l]);
                 }
                 __syncthreads();
                 // columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48; column s & 15)
@@ -2031,8 +2052,8 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const int nb = NPB == 1 ? cnt[0] : (pl == 0 ? cnt[0] : (pl == 1 ? cnt[1] : cnt[2]));
                 const unsigned short *lst =
                     L.l0 + (NPB == 1 ? 0 : (pl == 0 ? base[0] : (pl == 1 ? base[1] : base[2])));
-                const T *w = reinterpret_cast<const T *>(L.w) + 16 * pl;
-                const T *wv = reinterpret_cast<const T *>(L.wv) + 16 * pl;
+                const T *w = reinterpret_cast<const T *>(L.w) + kWBlk * pl;
+                const T *wv = reinterpret_cast<const T *>(L.wv) + kWBlk * pl;
                 const int kp = blk_pair(pl);
                 const float rcl = P == 1 ? (float)S.rc[kp] : 0.0f;
                 const double rc2l = P == 1 ? S.rc2[kp] : 0.0;
@@ -2133,13 +2154,13 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 for (int q = 0; q < 4; ++q) v4[q] = accd[q];
                 double *buf = reinterpret_cast<double *>(L.l0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) buf[16 * (4 * rowq + q) + col] = v4[q];
+                for (int q = 0; q < 4; ++q) buf[kColStr * (4 * rowq + q) + col] = v4[q];
                 __syncthreads();
                 if (rowq < NPB) {
                     const int c0 = rowq == 0 ? 0 : (rowq == 1 ? nc[0] : nc[0] + nc[1]);
                     const int c1 = c0 + (rowq == 0 ? nc[0] : (rowq == 1 ? nc[1] : nc[2]));
                     double sum = 0.0;
-                    for (int cc = c0; cc < c1; ++cc) sum += buf[16 * col + cc];
+                    for (int cc = c0; cc < c1; ++cc) sum += buf[kColStr * col + cc];
                     const int slot = wp.slots[16 + s];
                     // each row takes its 2^(1 - zeta) here (a power of two for the integer
                     // zeta of the grids: the same bits as scaling every term)
