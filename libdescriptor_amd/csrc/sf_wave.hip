@@ -7,8 +7,8 @@
 //      compacted (list order kept) into LDS arrays; each lane adds its g2
 //      rows into FP64 accumulators.
 //   2. triplet enumeration: the lanes walk all pairs (a < b) of compacted
-//      neighbours along diagonals b - a = 1, 2, ... and compact the pairs with
-//      r_jk < rc_jk into one LDS region per species-pair block.
+//      neighbours in folded-diagonal order (pair_of) and compact the pairs
+//      with r_jk < rc_jk into one LDS region per species-pair block.
 //   3. triplet phase: the lanes walk the active pairs only.  cos(theta),
 //      fc_jk, one exp per distinct g4 eta and the zeta powers of each lambda's
 //      base serve all rows of a triplet; each row adds one FP64 fma.
@@ -292,34 +292,25 @@ __device__ __forceinline__ int pidx(int a, int b, int ns)
     return (int)__umul24(a, ns) + b;
 }
 
-// Codes a | b << 6 of all pairs a < b of M compacted neighbours (M = 2..64)
-// in diagonal order: diagonal d = b - a = 1, 2, ... holds the M - d pairs
-// (a, a + d), a ascending.  The pairs of M start at C(M, 3) = M(M-1)(M-2)/6.
-// Along a diagonal the a's and the b's are all distinct, so the lanes of one
-// instruction touch distinct neighbours: conflict-free LDS reads, and the
-// adjoint's LDS accumulations into a and b rarely collide.  The table is
-// built at compile time (87 KB of read-only device memory); a lane reads its
-// pair's code with one coalesced load, issued a pass ahead.
-constexpr int kPairTabSize = 64 * 63 * 62 / 6 + 64 * 63 / 2;  // C(65, 3)
-struct PairTab {
-    unsigned short v[kPairTabSize];
-};
-
-constexpr PairTab make_pair_tab()
+// The pairs a < b of M compacted neighbours (M = 2..64) in folded-diagonal
+// order: the T = M(M-1)/2 candidates t run in rows of M, row r holding the
+// M - 1 - r pairs of diagonal b - a = r + 1 (a ascending) and then the r + 1
+// pairs of diagonal M - 1 - r (for even M the last row holds diagonal M / 2
+// alone, whose M / 2 pairs end at T).  Along each part of a row the a's and
+// the b's are distinct, so the lanes of one instruction mostly touch distinct
+// neighbours: few LDS bank conflicts, and the adjoint's LDS accumulations into
+// a and b rarely collide.  A lane computes its pair from t, with no table
+// load (whose latency a pass of this length cannot hide): r = floor(t / M)
+// from the float (t + 0.5) / M, which lies at least 0.5 / M from an integer,
+// far beyond the rounding of the product and of inv_m = 1 / M (within an ulp).
+__device__ __forceinline__ void pair_of(int t, int M, float inv_m, int &a, int &b)
 {
-    PairTab t{};
-    int o = 0;
-    for (int M = 2; M <= 64; ++M)
-        for (int d = 1; d < M; ++d)
-            for (int a = 0; a + d < M; ++a) t.v[o++] = (unsigned short)(a | ((a + d) << 6));
-    return t;
-}
-
-__device__ const PairTab kPairTab = make_pair_tab();
-
-__device__ __forceinline__ const unsigned short *pair_codes(int M)
-{
-    return kPairTab.v + (M * (M - 1) * (M - 2)) / 6;
+    const int r = (int)fmaf((float)t, inv_m, 0.5f * inv_m);
+    const int i = t - r * M;
+    const int l1 = M - 1 - r;  // the pairs of row r's first diagonal
+    const bool first = i < l1;
+    a = first ? i : i - l1;
+    b = a + (first ? r + 1 : l1);
 }
 
 // centres of this block: blocks are dealt to the 8 XCDs round-robin (block b
@@ -933,7 +924,7 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
 }
 
 // Phase 2: the active pairs (a < b, r_jk < rc_jk) of the M compacted
-// neighbours, walked in diagonal order, compacted into one region of
+// neighbours, walked in folded-diagonal order, compacted into one region of
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
@@ -943,8 +934,8 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
 // codes carry the absolute slots.
 template <int P, int NPB, bool UNI>
 __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, const Fast &F,
-                                                int ns, int M, int n1, int s, int code0,
-                                                int (&cnt)[3], int (&base)[3], int off = 0)
+                                                int ns, int M, int n1, int s, int (&cnt)[3],
+                                                int (&base)[3], int off = 0)
 {
     const int n0 = M - n1;
     base[0] = 0;
@@ -952,16 +943,17 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
     base[2] = base[1] + n0 * n1;
     cnt[0] = cnt[1] = cnt[2] = 0;
     const int T = (M * (M - 1)) >> 1;
-    const unsigned short *tab = pair_codes(M);
-    int next = code0;  // this lane's code of the first pass, loaded by the caller
+    const float inv_m = __builtin_amdgcn_rcpf((float)(M > 1 ? M : 2));
     for (int t0 = 0; t0 < T; t0 += kW) {
         const int t = t0 + s;
-        const int code = next;
-        if (t0 + kW < T) next = t + kW < T ? tab[t + kW] : 0;
-        // no branch on t < T: past T the code is 0, whose reads (slot off)
-        // stay inside the arrays, and the test is masked after, so that the
+        // no branch on t < T: past T the lane takes pair 0, whose reads stay
+        // inside the arrays, and the test is masked after, so that the
         // ballots below take the compare results as they are
-        const int a = (code & 63) + off, b = (code >> 6) + off;
+        int a, b;
+        pair_of(t < T ? t : 0, M, inv_m, a, b);
+        a += off;
+        b += off;
+        const int code = a | (b << 6);
         int sa, sb;
         bool act;
         if constexpr (P == 0) {
@@ -988,7 +980,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
         act = act && t < T;
         if (NPB == 1) {
             const unsigned long long m = __builtin_amdgcn_ballot_w64(act);
-            if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)(code + 65 * off);
+            if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
             cnt[0] += __popcll(m);
         } else {
             const int blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
@@ -1002,7 +994,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                 const int p0 = base[0] + cnt[0] + below(m0, s);
                 const int p1 = base[1] + cnt[1] + below(m1, s);
                 const int p2 = base[2] + cnt[2] + below(m2, s);
-                L.l0[blk == 0 ? p0 : (blk == 1 ? p1 : p2)] = (unsigned short)(code + 65 * off);
+                L.l0[blk == 0 ? p0 : (blk == 1 ? p1 : p2)] = (unsigned short)code;
             }
             cnt[0] += __popcll(m0);
             cnt[1] += __popcll(m1);
@@ -1244,8 +1236,6 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         // two species blocks: species-0 neighbours first, list order within each
         const unsigned long long m1 = NSB == 2 ? __ballot(valid && sj == 1) : 0ull;
         const int n1 = NSB == 2 ? __popcll(m1) : 0;
-        // the enumeration's first pair codes, a phase ahead
-        const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
         T *terms = reinterpret_cast<T *>(L.l0);  // the g2 terms, until the enumeration
         if (valid) {
             const int pos = NSB == 2 ? (sj ? (M - n1) + below(m1, s) : below(m & ~m1, s))
@@ -1294,9 +1284,9 @@ __global__ __launch_bounds__(64, P == 0 ? 3 : 5) void k_fwd_wave(WaveTab wt, int
         if (wt.has3) {
             int cnt[3], base[3];
             if (wt.rc_uniform)
-                enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base);
+                enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, cnt, base);
             else
-                enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base);
+                enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x, wt.nl_species != 0);
             __syncthreads();
             // the blocks are dealt columns of 4 lanes (s, s ^ 16, s ^ 32, s ^ 48;
@@ -1563,8 +1553,6 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         // two species blocks: species-0 neighbours first, list order within each
         const unsigned long long m1 = NSB == 2 ? __ballot(valid && sj == 1) : 0ull;
         const int n1 = NSB == 2 ? __popcll(m1) : 0;
-        // the enumeration's first pair codes, a phase ahead
-        const int code0 = (wt.has3 && s < ((M * (M - 1)) >> 1)) ? pair_codes(M)[s] : 0;
         if (valid) {
             const int pos = NSB == 2 ? (sj ? (M - n1) + below(m1, s) : below(m & ~m1, s))
                                      : below(m, s);
@@ -1620,9 +1608,9 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
         if (wt.has3) {
             int cnt[3], base[3];
             if (wt.rc_uniform)
-                enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base);
+                enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, cnt, base);
             else
-                enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base);
+                enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, cnt, base);
             Bn = load_nbr(jn, species, x, wt.nl_species != 0);
             // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda,
             // kept in the compute type (a loop-invariant copy costs half the registers)
@@ -1918,9 +1906,6 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         // two species blocks: species-0 neighbours first, list order within each
         const unsigned long long m1 = NSB == 2 ? __ballot(valid && sj == 1) : 0ull;
         const int n1A = NSB == 2 ? __popcll(m1 & mA) : 0, n1B = NSB == 2 ? __popcll(m1 & mB) : 0;
-        // the enumerations' first pair codes, a phase ahead
-        const int code0A = (wt.has3 && s < ((MA * (MA - 1)) >> 1)) ? pair_codes(MA)[s] : 0;
-        const int code0B = (wt.has3 && s < ((MB * (MB - 1)) >> 1)) ? pair_codes(MB)[s] : 0;
         T *terms = reinterpret_cast<T *>(L.l0);  // the g2 terms, until the enumerations
         if (valid) {
             const unsigned long long mX = hb ? mB : mA, m1X = m1 & mX;
@@ -2010,11 +1995,10 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 double *drow = h ? L.dB : L.d;
                 const double *grow = h ? L.gB : L.g;
                 int cnt[3], base[3];
-                const int code0 = h ? code0B : code0A;
                 if (wt.rc_uniform)
-                    enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, code0, cnt, base, off);
+                    enumerate_pairs<P, NPB, true>(S, L, F, ns, M, n1, s, cnt, base, off);
                 else
-                    enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, code0, cnt, base, off);
+                    enumerate_pairs<P, NPB, false>(S, L, F, ns, M, n1, s, cnt, base, off);
                 if (h == 0) Bvn = load_nbr(jn, species, x, wt.nl_species != 0);
                 // row weights of every block: dE/dzeta 2^(1-zeta), and times zeta lambda
                 if (s < 16 * NPB) {
