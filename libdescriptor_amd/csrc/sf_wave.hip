@@ -1853,6 +1853,14 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         Bv = load_nbr(in ? nl[beg + e] : -1, species, x, wt.nl_species != 0);
     }
     while (c < rr.hi) {
+        // the lane index, opaque to the compiler within each unit (an empty asm
+        // that claims to change it): the values derived from it are formed in
+        // the unit where they are used, not hoisted out of the loop and held
+        // (or spilled) through all of it
+        int s_unit = s;
+        asm volatile("" : "+v"(s_unit));
+        {
+        const int s = s_unit;
         const int cn = c + (two ? 2 : 1);
         Ctr An, Bn;
         const bool twon = load_unit(cn, An, Bn);
@@ -2202,6 +2210,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         ipsB = ipsBn;
         ipe = ipen;
         Bv = Bvn;
+        }
     }
 }
 
