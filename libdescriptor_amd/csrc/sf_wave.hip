@@ -306,7 +306,7 @@ __device__ __forceinline__ int pidx(int a, int b, int ns)
 __device__ __forceinline__ void pair_of(int t, int M, float inv_m, int &a, int &b)
 {
     const int r = (int)fmaf((float)t, inv_m, 0.5f * inv_m);
-    const int i = t - r * M;
+    const int i = t - (int)__umul24(r, M);
     const int l1 = M - 1 - r;  // the pairs of row r's first diagonal
     const bool first = i < l1;
     a = first ? i : i - l1;
@@ -1164,12 +1164,14 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     const float Aik = fmaf(fc3, fmaf(dcik, SA, -2.0f * rb * SC), fa * fjk * dfb * SB);
     const float Ajk = fmaf(fc3, fmaf(dcjk, SA, -2.0f * g.rjk * SC), fa * fb * dfjk * SB);
     const double a1c = (double)(Aij * ria), a2c = (double)(Aik * rib), a3c = (double)(Ajk * g.rjkinv);
-    atomicAdd(&L.fx[a], fma(a1c, a0.x, -a3c * g.ex));
-    atomicAdd(&L.fy[a], fma(a1c, a0.y, -a3c * g.ey));
-    atomicAdd(&L.fz[a], fma(a1c, a1.x, -a3c * g.ez));
-    atomicAdd(&L.fx[b], fma(a2c, b0.x, a3c * g.ex));
-    atomicAdd(&L.fy[b], fma(a2c, b0.y, a3c * g.ey));
-    atomicAdd(&L.fz[b], fma(a2c, b1.x, a3c * g.ez));
+    // the r_jk term once, added to b and (negated, exactly) to a
+    const double jx = a3c * g.ex, jy = a3c * g.ey, jz = a3c * g.ez;
+    atomicAdd(&L.fx[a], fma(a1c, a0.x, -jx));
+    atomicAdd(&L.fy[a], fma(a1c, a0.y, -jy));
+    atomicAdd(&L.fz[a], fma(a1c, a1.x, -jz));
+    atomicAdd(&L.fx[b], fma(a2c, b0.x, jx));
+    atomicAdd(&L.fy[b], fma(a2c, b0.y, jy));
+    atomicAdd(&L.fz[b], fma(a2c, b1.x, jz));
 }
 
 // ZG: the zeta grid of the FP32-bulk form (ZGrid)
