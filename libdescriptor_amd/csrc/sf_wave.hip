@@ -7,7 +7,7 @@
 //      compacted (list order kept) into LDS arrays; each lane adds its g2
 //      rows into FP64 accumulators.
 //   2. triplet enumeration: the lanes walk all pairs (a < b) of compacted
-//      neighbours in folded-diagonal order (pair_of) and compact the pairs
+//      neighbours in rotational order (pair_of) and compact the pairs
 //      with r_jk < rc_jk into one LDS region per species-pair block.
 //   3. triplet phase: the lanes walk the active pairs only.  cos(theta),
 //      fc_jk, one exp per distinct g4 eta and the zeta powers of each lambda's
@@ -292,25 +292,29 @@ __device__ __forceinline__ int pidx(int a, int b, int ns)
     return (int)__umul24(a, ns) + b;
 }
 
-// The pairs a < b of M compacted neighbours (M = 2..64) in folded-diagonal
-// order: the T = M(M-1)/2 candidates t run in rows of M, row r holding the
-// M - 1 - r pairs of diagonal b - a = r + 1 (a ascending) and then the r + 1
-// pairs of diagonal M - 1 - r (for even M the last row holds diagonal M / 2
-// alone, whose M / 2 pairs end at T).  Along each part of a row the a's and
-// the b's are distinct, so the lanes of one instruction mostly touch distinct
-// neighbours: few LDS bank conflicts, and the adjoint's LDS accumulations into
-// a and b rarely collide.  A lane computes its pair from t, with no table
-// load (whose latency a pass of this length cannot hide): r = floor(t / M)
-// from the float (t + 0.5) / M, which lies at least 0.5 / M from an integer,
-// far beyond the rounding of the product and of inv_m = 1 / M (within an ulp).
+// The pairs of M compacted neighbours (M = 2..64) in rotational order: the
+// T = M(M-1)/2 candidates t run in rows of M, row r pairing neighbour i with
+// (i + r + 1) mod M (for even M the last row, r = M/2 - 1, holds only its M/2
+// pairs i < M/2, which end at T): every unordered pair once.  Along a row the
+// lanes of one instruction touch distinct neighbours: few LDS bank conflicts,
+// and the adjoint's LDS accumulations into a and b rarely collide.  A lane
+// computes its pair from t, with no table load (whose latency a pass of this
+// length cannot hide): r = floor(t / M) from the float (t + 0.5) / M, which
+// lies at least 0.5 / M from an integer, far beyond the rounding of the
+// product and of inv_m = 1 / M (within an ulp).  ORDERED: a < b, as the list
+// order of the single-block kernels needs for an asymmetric cutoff matrix
+// (rcut(j, k) with j first, src/SymmetryFunctions.cpp:206); the species-
+// resolved blocks have a symmetric one, and a triplet's terms are symmetric
+// in (a, b) up to exactly commuted operations.
+template <bool ORDERED>
 __device__ __forceinline__ void pair_of(int t, int M, float inv_m, int &a, int &b)
 {
     const int r = (int)fmaf((float)t, inv_m, 0.5f * inv_m);
     const int i = t - (int)__umul24(r, M);
-    const int l1 = M - 1 - r;  // the pairs of row r's first diagonal
-    const bool first = i < l1;
-    a = first ? i : i - l1;
-    b = a + (first ? r + 1 : l1);
+    const unsigned c = (unsigned)(i + r + 1);
+    const int j = (int)min(c, c - (unsigned)M);  // (i + r + 1) mod M
+    a = ORDERED ? min(i, j) : i;
+    b = ORDERED ? max(i, j) : j;
 }
 
 // centres of this block: blocks are dealt to the 8 XCDs round-robin (block b
@@ -924,7 +928,7 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
 }
 
 // Phase 2: the active pairs (a < b, r_jk < rc_jk) of the M compacted
-// neighbours, walked in folded-diagonal order, compacted into one region of
+// neighbours, walked in rotational order, compacted into one region of
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
@@ -950,7 +954,7 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
         // inside the arrays, and the test is masked after, so that the
         // ballots below take the compare results as they are
         int a, b;
-        pair_of(t < T ? t : 0, M, inv_m, a, b);
+        pair_of<NPB == 1>(t < T ? t : 0, M, inv_m, a, b);
         a += off;
         b += off;
         const int code = a | (b << 6);
@@ -977,16 +981,20 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
                                : (NPB > 1 ? S.rc2[blk_pair(sa + sb)] : S.rc2[pidx(sa, sb, ns)]);
             act = rjk2 < rc2;
         }
+        // the lanes past the end of the list, as a scalar mask (a ballot of
+        // act && t < T would be rebuilt from a per-lane boolean)
+        const unsigned long long live = T - t0 >= kW ? ~0ull : (1ull << (T - t0)) - 1ull;
+        const unsigned long long mt = __builtin_amdgcn_ballot_w64(act) & live;
         act = act && t < T;
         if (NPB == 1) {
-            const unsigned long long m = __builtin_amdgcn_ballot_w64(act);
+            const unsigned long long m = mt;
             if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
             cnt[0] += __popcll(m);
         } else {
             const int blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
             // the block masks as scalar ANDs of plain ballots (a ballot of
             // act && blk == p would be rebuilt from a per-lane boolean)
-            const unsigned long long ma = __builtin_amdgcn_ballot_w64(act);
+            const unsigned long long ma = mt;
             const unsigned long long b0 = __builtin_amdgcn_ballot_w64(blk == 0);
             const unsigned long long b1 = __builtin_amdgcn_ballot_w64(blk == 1);
             const unsigned long long m0 = ma & b0, m1 = ma & b1, m2 = ma & ~(b0 | b1);
