@@ -25,8 +25,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["sf_general.hip", "sf_wave.hip", "sf_plan.hip", "ldsf_runtime.hip",
-               "sf_server.hip", "sf_layout.hip", "ldsf_noalloc.hip", "sf_stage.hip"]
+HIP_SOURCES = ["sf_general.hip", "sf_wave.hip", "sf_wave_fp64.hip", "sf_wave_fp64_fused.hip",
+               "sf_plan.hip", "ldsf_runtime.hip", "sf_server.hip", "sf_layout.hip",
+               "ldsf_noalloc.hip", "sf_stage.hip"]
+# per-source options: the FP64 fused wave kernel takes the iterative ILP
+# scheduler (1.5 % faster on the 262,144-atom system, no scratch)
+HIP_EXTRA = {"sf_wave_fp64_fused.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
 CXX_SOURCES = ["sf_params.cpp", "neighbor_list.cpp", "descriptors.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.
@@ -63,7 +67,8 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = []
     for s in HIP_SOURCES:
-        jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", _obj(s)])
+        jobs.append([HIPCC] + HIP_FLAGS + HIP_EXTRA.get(s, []) +
+                    ["-c", os.path.join(CSRC, s), "-o", _obj(s)])
     for s in CXX_SOURCES:
         jobs.append([CXX] + CXX_FLAGS + ["-c", os.path.join(CSRC, s), "-o", _obj(s)])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:

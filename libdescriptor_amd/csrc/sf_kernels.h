@@ -123,6 +123,23 @@ hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, cons
                            int width, int max_nn, int group, hipStream_t s,
                            DevCounts dc = DevCounts(), double *desc = nullptr,
                            bool assign = false);
+// the FP64 form's launches, in units of their own for their compiler options
+// (sf_wave_fp64.hip: forward and adjoint; sf_wave_fp64_fused.hip: the fused
+// kernel); launch_fwd_wave and launch_adj_wave call them for precision 0
+hipError_t launch_fwd_wave_fp64(const WaveTab &wt, int n_centers, const int *centers,
+                                const int *offs, const int *nl, const int *species, const double *x,
+                                double *desc, int width, int max_nn, int group, hipStream_t s,
+                                DevCounts dc);
+hipError_t launch_adj_wave_fp64(const WaveTab &wt, int n_centers, const int *centers,
+                                const int *offs, const int *nl, const int *species, const double *x,
+                                const double *dE, const int *inv_pos, double *edge, int n_edges,
+                                int width, int max_nn, int group, hipStream_t s, DevCounts dc,
+                                double *desc, bool assign);
+hipError_t launch_fused_wave_fp64(const WaveTab &wt, int n_centers, const int *centers,
+                                  const int *offs, const int *nl, const int *species,
+                                  const double *x, const double *dE, const int *inv_pos,
+                                  double *edge, int n_edges, int width, int max_nn, int group,
+                                  hipStream_t s, DevCounts dc, double *desc, bool assign);
 
 // sf_plan.hip: neighbour-list plan building and the adjoint gather
 hipError_t launch_plan_validate(int n_centers, const int *centers, int n_total, const int *numneigh,

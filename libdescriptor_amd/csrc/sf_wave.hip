@@ -2240,6 +2240,7 @@ size_t lds_for(int max_nn, int kind, int ns, int precision)
 
 }  // namespace
 
+#ifndef LDSF_WAVE_FP64_UNIT
 int wave_group_size(int max_nn, int n_species)
 {
     // one wavefront per centre, one lane per neighbour-list entry
@@ -2371,33 +2372,37 @@ bool wave_build(const ldsf_sf *sf, WaveTab *wt, WaveParams *wp)
     return true;
 }
 
-hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
-                           const int *offs, const int *nl, const int *species, const double *x,
-                           double *desc, int width, int max_nn, int group, hipStream_t s,
-                           DevCounts dc)
+#endif  // LDSF_WAVE_FP64_UNIT
+
+namespace {
+
+// the launches of one precision form P (the FP64 instances are compiled in
+// units of their own, for their compiler options: sf_wave_fp64.hip, the
+// forward and adjoint kernels, and sf_wave_fp64_fused.hip, the fused kernel)
+template <int P>
+hipError_t launch_fwd_form(const WaveTab &wt, int n_centers, const int *centers, const int *offs,
+                           const int *nl, const int *species, const double *x, double *desc,
+                           int width, int max_nn, int group, hipStream_t s, DevCounts dc)
 {
     if (n_centers <= 0) return hipSuccess;
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const size_t lds = lds_for(max_nn, kFwd, wt.n_species, precision);
+    const size_t lds = lds_for(max_nn, kFwd, wt.n_species, P);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
     const bool use_dc = dc.hdr || dc.req_flag || dc.out_row || dc.in_row;
-#define LDSF_FWD_DC(P, NSB, NPB, ZG, DCV)                                                     \
+#define LDSF_FWD_DC(NSB, NPB, ZG, DCV)                                                        \
     hipLaunchKernelGGL((k_fwd_wave<P, NSB, NPB, ZG, DCV>), grid, block, lds, s, wt, n_centers, \
                        centers, offs, nl, species, x, desc, width, maxn, dc)
-#define LDSF_FWD(P, NSB, NPB, ZG)                  \
-    do {                                           \
-        if (use_dc) LDSF_FWD_DC(P, NSB, NPB, ZG, true);  \
-        else LDSF_FWD_DC(P, NSB, NPB, ZG, false);        \
+#define LDSF_FWD(NSB, NPB, ZG)                        \
+    do {                                              \
+        if (use_dc) LDSF_FWD_DC(NSB, NPB, ZG, true);  \
+        else LDSF_FWD_DC(NSB, NPB, ZG, false);        \
     } while (0)
-#define LDSF_FWD_P(NSB, NPB)                                                        \
-    do {                                                                            \
-        if (precision == 0 && wt.zgrid == 1) LDSF_FWD(0, NSB, NPB, 1);             \
-        else if (precision == 0 && wt.zgrid == 2) LDSF_FWD(0, NSB, NPB, 2);        \
-        else if (precision == 0) LDSF_FWD(0, NSB, NPB, 0);                         \
-        else if (wt.zgrid == 1) LDSF_FWD(1, NSB, NPB, 1);                          \
-        else if (wt.zgrid == 2) LDSF_FWD(1, NSB, NPB, 2);                          \
-        else LDSF_FWD(1, NSB, NPB, 0);                                             \
+#define LDSF_FWD_P(NSB, NPB)                           \
+    do {                                               \
+        if (wt.zgrid == 1) LDSF_FWD(NSB, NPB, 1);      \
+        else if (wt.zgrid == 2) LDSF_FWD(NSB, NPB, 2); \
+        else LDSF_FWD(NSB, NPB, 0);                    \
     } while (0)
     if (wt.nsb == 1) LDSF_FWD_P(1, 1); else LDSF_FWD_P(2, 3);
 #undef LDSF_FWD_P
@@ -2406,55 +2411,109 @@ hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, cons
     return hipGetLastError();
 }
 
-hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
-                           const int *offs, const int *nl, const int *species, const double *x,
-                           const double *dE, const int *inv_pos, double *edge, int n_edges,
-                           int width, int max_nn, int group, hipStream_t s, DevCounts dc,
-                           double *desc, bool assign)
+// the adjoint (FUSED false) or the fused forward + adjoint (FUSED true)
+template <int P, bool FUSED>
+hipError_t launch_adj_form(const WaveTab &wt, int n_centers, const int *centers, const int *offs,
+                           const int *nl, const int *species, const double *x, const double *dE,
+                           const int *inv_pos, double *edge, int n_edges, int width, int max_nn,
+                           int group, hipStream_t s, DevCounts dc, double *desc, bool assign)
 {
     if (n_centers <= 0) return hipSuccess;
     if (group != kW || max_nn > kW) return hipErrorInvalidValue;
     const int maxn = std::max(max_nn, 1);
-    const bool fused = desc != nullptr || dc.out_row != nullptr;
-    const size_t lds = lds_for(max_nn, fused ? kPair : kAdj, wt.n_species, precision);
+    const size_t lds = lds_for(max_nn, FUSED ? kPair : kAdj, wt.n_species, P);
     const dim3 grid(grid_for(n_centers, dc.max_blocks)), block(kW);
     const bool use_dc = dc.hdr || dc.req_flag || dc.out_row || dc.in_row;
-#define LDSF_ADJ_DC(P, NSB, NPB, ZG, DCV)                                                         \
+#define LDSF_ADJ_DC(NSB, NPB, ZG, DCV)                                                            \
     hipLaunchKernelGGL((k_adj_wave<P, NSB, NPB, ZG, DCV>), grid, block, lds, s, wt, n_centers,    \
                        centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, width, maxn, dc)
-#define LDSF_PAIR_DC(P, NSB, NPB, ZG, DCV)                                                       \
+#define LDSF_PAIR_DC(NSB, NPB, ZG, DCV)                                                          \
     hipLaunchKernelGGL((k_fused_pair<P, NSB, NPB, ZG, DCV>), grid, block, lds, s, wt, n_centers, \
                        centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, desc, width,   \
                        maxn, dc, assign ? 1 : 0)
-#define LDSF_ADJ(P, NSB, NPB, FUSED, ZG)                                   \
-    do {                                                                   \
-        if (FUSED) {                                                       \
-            if (use_dc) LDSF_PAIR_DC(P, NSB, NPB, ZG, true);               \
-            else LDSF_PAIR_DC(P, NSB, NPB, ZG, false);                     \
-        } else {                                                           \
-            if (use_dc) LDSF_ADJ_DC(P, NSB, NPB, ZG, true);                \
-            else LDSF_ADJ_DC(P, NSB, NPB, ZG, false);                      \
-        }                                                                  \
+#define LDSF_ADJ(NSB, NPB, ZG)                                          \
+    do {                                                                \
+        if constexpr (FUSED) {                                          \
+            if (use_dc) LDSF_PAIR_DC(NSB, NPB, ZG, true);               \
+            else LDSF_PAIR_DC(NSB, NPB, ZG, false);                     \
+        } else {                                                        \
+            if (use_dc) LDSF_ADJ_DC(NSB, NPB, ZG, true);                \
+            else LDSF_ADJ_DC(NSB, NPB, ZG, false);                      \
+        }                                                               \
     } while (0)
-#define LDSF_ADJ_P(NSB, NPB, FUSED)                                                \
-    do {                                                                           \
-        if (precision == 0 && wt.zgrid == 1) LDSF_ADJ(0, NSB, NPB, FUSED, 1);      \
-        else if (precision == 0 && wt.zgrid == 2) LDSF_ADJ(0, NSB, NPB, FUSED, 2); \
-        else if (precision == 0) LDSF_ADJ(0, NSB, NPB, FUSED, 0);                  \
-        else if (wt.zgrid == 1) LDSF_ADJ(1, NSB, NPB, FUSED, 1);                   \
-        else if (wt.zgrid == 2) LDSF_ADJ(1, NSB, NPB, FUSED, 2);                   \
-        else LDSF_ADJ(1, NSB, NPB, FUSED, 0);                                      \
+#define LDSF_ADJ_P(NSB, NPB)                           \
+    do {                                               \
+        if (wt.zgrid == 1) LDSF_ADJ(NSB, NPB, 1);      \
+        else if (wt.zgrid == 2) LDSF_ADJ(NSB, NPB, 2); \
+        else LDSF_ADJ(NSB, NPB, 0);                    \
     } while (0)
-    if (fused) {
-        if (wt.nsb == 1) LDSF_ADJ_P(1, 1, true); else LDSF_ADJ_P(2, 3, true);
-    } else {
-        if (wt.nsb == 1) LDSF_ADJ_P(1, 1, false); else LDSF_ADJ_P(2, 3, false);
-    }
+    if (wt.nsb == 1) LDSF_ADJ_P(1, 1); else LDSF_ADJ_P(2, 3);
 #undef LDSF_ADJ_P
 #undef LDSF_ADJ
 #undef LDSF_PAIR_DC
 #undef LDSF_ADJ_DC
     return hipGetLastError();
 }
+
+}  // namespace
+
+#if !defined(LDSF_WAVE_FP64_UNIT)
+hipError_t launch_fwd_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
+                           const int *offs, const int *nl, const int *species, const double *x,
+                           double *desc, int width, int max_nn, int group, hipStream_t s,
+                           DevCounts dc)
+{
+    if (precision == 0)
+        return launch_fwd_wave_fp64(wt, n_centers, centers, offs, nl, species, x, desc, width,
+                                    max_nn, group, s, dc);
+    return launch_fwd_form<1>(wt, n_centers, centers, offs, nl, species, x, desc, width, max_nn,
+                              group, s, dc);
+}
+
+hipError_t launch_adj_wave(const WaveTab &wt, int precision, int n_centers, const int *centers,
+                           const int *offs, const int *nl, const int *species, const double *x,
+                           const double *dE, const int *inv_pos, double *edge, int n_edges,
+                           int width, int max_nn, int group, hipStream_t s, DevCounts dc,
+                           double *desc, bool assign)
+{
+    const bool fused = desc != nullptr || dc.out_row != nullptr;
+    if (precision == 0)
+        return (fused ? launch_fused_wave_fp64 : launch_adj_wave_fp64)(
+            wt, n_centers, centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, width,
+            max_nn, group, s, dc, desc, assign);
+    return (fused ? launch_adj_form<1, true> : launch_adj_form<1, false>)(
+        wt, n_centers, centers, offs, nl, species, x, dE, inv_pos, edge, n_edges, width, max_nn,
+        group, s, dc, desc, assign);
+}
+#elif LDSF_WAVE_FP64_UNIT == 1
+hipError_t launch_fwd_wave_fp64(const WaveTab &wt, int n_centers, const int *centers,
+                                const int *offs, const int *nl, const int *species, const double *x,
+                                double *desc, int width, int max_nn, int group, hipStream_t s,
+                                DevCounts dc)
+{
+    return launch_fwd_form<0>(wt, n_centers, centers, offs, nl, species, x, desc, width, max_nn,
+                              group, s, dc);
+}
+
+hipError_t launch_adj_wave_fp64(const WaveTab &wt, int n_centers, const int *centers,
+                                const int *offs, const int *nl, const int *species, const double *x,
+                                const double *dE, const int *inv_pos, double *edge, int n_edges,
+                                int width, int max_nn, int group, hipStream_t s, DevCounts dc,
+                                double *desc, bool assign)
+{
+    return launch_adj_form<0, false>(wt, n_centers, centers, offs, nl, species, x, dE, inv_pos,
+                                     edge, n_edges, width, max_nn, group, s, dc, desc, assign);
+}
+#elif LDSF_WAVE_FP64_UNIT == 2
+hipError_t launch_fused_wave_fp64(const WaveTab &wt, int n_centers, const int *centers,
+                                  const int *offs, const int *nl, const int *species,
+                                  const double *x, const double *dE, const int *inv_pos,
+                                  double *edge, int n_edges, int width, int max_nn, int group,
+                                  hipStream_t s, DevCounts dc, double *desc, bool assign)
+{
+    return launch_adj_form<0, true>(wt, n_centers, centers, offs, nl, species, x, dE, inv_pos,
+                                    edge, n_edges, width, max_nn, group, s, dc, desc, assign);
+}
+#endif  // LDSF_WAVE_FP64_UNIT
 
 }  // namespace ldsf
