@@ -2007,6 +2007,12 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
         if (wt.has3) {
 #pragma clang loop unroll(disable)
             for (int h = 0; h < (two ? 2 : 1); ++h) {
+                // the lane index opaque again for each centre (as for each unit
+                // above): its derived values are formed per centre
+                int s_cen = s;
+                asm volatile("" : "+v"(s_cen));
+                {
+                const int s = s_cen;
                 const int off = h ? 32 : 0;
                 const int M = h ? MB : MA, n1 = h ? n1B : n1A;
                 const Ctr &X = h ? B : A;
@@ -2171,6 +2177,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 }
                 // the column sums are read before the next enumeration reuses the region
                 __syncthreads();
+                }
             }
         } else {
             Bvn = load_nbr(jn, species, x, wt.nl_species != 0);
