@@ -114,6 +114,36 @@ def test_fused_two_species_against_oracle(family):
     np.testing.assert_array_equal(G2, G32)
 
 
+def test_neighbourhoods_of_up_to_64():
+    """A dense cluster whose largest neighbourhood has 63 entries (49 centres
+    over 32, each a wave of its own, beside paired ones): the wave kernels'
+    largest pair enumerations (up to 1,953 candidates, 31 rotational rows),
+    fused and separate, both precisions, against the oracle."""
+    from test_oracle import cluster, full_list
+    print("case: neighbourhoods of up to 64 entries", flush=True)
+    rc = 4.8
+    pos = cluster(110, 4, box=9.0, dmin=1.2)
+    nn, nl = full_list(pos, rc)
+    assert 57 <= nn.max() <= 64 and (nn > 32).sum() >= 10 and (nn <= 32).sum() >= 10
+    L = dict(n_contrib=len(pos), coords=pos, species=np.zeros(len(pos), np.int32), neighlist=nl,
+             numneigh=nn)
+    S = Setup(L, species=("Si",), rc=rc, resolved=False)
+    assert S.plan.kernel_family == 1
+    d0, G0 = S.oracle()
+    d, G = S.fused(0)
+    assert elem_rel_err(d, d0, 1e-300) <= TOL64
+    assert rel_err(G, G0) <= TOL64
+    ds_, Gs = S.split(0)
+    assert elem_rel_err(ds_, d0, 1e-300) <= TOL64
+    assert rel_err(Gs, G0) <= TOL64
+    d32, G32 = S.fused(1)
+    assert signal_rel(d32, d0, [(0, 8), (8, 24)]) <= TOL32
+    assert rel_err(G32, G0) <= TOL32
+    ds32, Gs32 = S.split(1)
+    assert signal_rel(ds32, d0, [(0, 8), (8, 24)]) <= TOL32
+    assert rel_err(Gs32, G0) <= TOL32
+
+
 @pytest.mark.parametrize("resolved,rc,family", [
     (True, [[5.0, 4.6], [4.6, 4.2]], 1),    # three pair blocks, one cutoff each
     (False, [[5.0, 4.4], [4.7, 4.1]], 1),   # one block, the cutoff looked up per species pair
