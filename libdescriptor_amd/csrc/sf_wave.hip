@@ -86,15 +86,18 @@ __device__ __forceinline__ void xswap(double &a, double &b)
 
 // M <= 8: the partner lane's value through DPP: lane ^ 1 and lane ^ 2 (quad
 // permutations), the mirror lane in its 8-lane half row (differs in bit 2)
-// and in its 16-lane row (differs in bit 3).
+// and in its 16-lane row (differs in bit 3).  (These DPP calls, and the row
+// rotations below, take 0 for a source lane they cannot read, whether as the
+// old value or, with bound_ctrl set, as the bound value; bound_ctrl spares the
+// register copy of the old value.)
 template <int M>
 __device__ __forceinline__ double dpp_partner(double v)
 {
     constexpr int ctrl = M == 1 ? 0xB1 : M == 2 ? 0x4E : M == 4 ? 0x141 : 0x140;
     static_assert(M == 1 || M == 2 || M == 4 || M == 8, "DPP partner within a row");
     const unsigned long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, ctrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
                                             (unsigned)lo));
 }
@@ -214,8 +217,8 @@ __device__ __forceinline__ void g2_rows_sum(const T *terms, int M, int n0, int s
     if constexpr (NSB == 1) {
         // lane ^ 8 within its 16-lane row: DPP row_ror:8
         const unsigned long long u = __double_as_longlong(sum);
-        const int lo32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x128, 0xF, 0xF, false);
-        const int hi32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false);
+        const int lo32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x128, 0xF, 0xF, true);
+        const int hi32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, true);
         sum += __longlong_as_double((long long)(((unsigned long long)(unsigned)hi32 << 32) |
                                                 (unsigned)lo32));
     }
@@ -269,8 +272,8 @@ __device__ __forceinline__ void g2_rows_sum_half(const T *terms, int M, int n0, 
     if constexpr (NSB == 1) {
         // lane ^ 8 within its 16-lane row: DPP row_ror:8
         const unsigned long long u = __double_as_longlong(sum);
-        const int lo32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x128, 0xF, 0xF, false);
-        const int hi32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false);
+        const int lo32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x128, 0xF, 0xF, true);
+        const int hi32 = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, true);
         sum += __longlong_as_double((long long)(((unsigned long long)(unsigned)hi32 << 32) |
                                                 (unsigned)lo32));
     }
