@@ -92,7 +92,11 @@ constexpr int kGridStride = 2048;
 constexpr int kBuckets = 9;        // wave kernels for max_nn <= 8, 16, ..., 64; general kernels
                                    // (max_nn > 64, or a set the wave kernels do not cover)
 constexpr int kGeneralBucket = 8;
-constexpr int kWaveBlocks = 4096;  // persistent grid of the wave kernels (~16 waves per CU)
+// the wave kernels' grid: 32 one-wave blocks per CU, each a run of a few
+// centres of the batch (a batch of ~60,000 centres: runs of ~7), so that the
+// last runs of a batch leave little of the GPU idle (measured against 4,096
+// and 12,288 blocks: 2 % more offline throughput, a 6 % lower median latency)
+constexpr int kWaveBlocks = 8192;
 
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ long gstride() { return (long)gridDim.x * blockDim.x; }
