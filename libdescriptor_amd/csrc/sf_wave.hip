@@ -935,10 +935,12 @@ __device__ __forceinline__ float pair_rows_f(const WaveParams &wp, const float *
 // L.l0 per species-pair block (codes a | b << 6).  With two species and n1
 // neighbours of species 1, block 0 (0-0), 1 (0-1) and 2 (1-1) hold at most
 // n0(n0-1)/2, n0 n1 and n1(n1-1)/2 pairs, so the regions start at 0,
-// n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.  UNI: every species pair has the cutoff
-// of pair 0 (no per-pair choice).  The neighbours sit at slots off .. off +
-// M - 1 (off = 0, or 32 for the second centre of a paired wave); the stored
-// codes carry the absolute slots.
+// n0(n0-1)/2 and n0(n0-1)/2 + n0 n1.  Two species blocks come with the slots
+// sorted by species (species 0 at 0 .. n0 - 1), so a pair's block follows
+// from its two indices.  UNI: every species pair has the cutoff of pair 0
+// (no per-pair choice).  The neighbours sit at slots off .. off + M - 1
+// (off = 0, or 32 for the second centre of a paired wave); the stored codes
+// carry the slots relative to off.
 template <int P, int NPB, bool UNI>
 __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, const Fast &F,
                                                 int ns, int M, int n1, int s, int (&cnt)[3],
@@ -958,15 +960,16 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
         // ballots below take the compare results as they are
         int a, b;
         pair_of<NPB == 1>(t < T ? t : 0, M, inv_m, a, b);
-        a += off;
-        b += off;
         const int code = a | (b << 6);
+        // species of the two slots: from the indices with two blocks
+        const bool xa = a >= n0, xb = b >= n0;
         int sa, sb;
         bool act;
         if constexpr (P == 0) {
-            const double rjk2 = dist2(L.px[b] - L.px[a], L.py[b] - L.py[a], L.pz[b] - L.pz[a]);
-            sa = L.sp[a];
-            sb = L.sp[b];
+            const double *px = L.px + off, *py = L.py + off, *pz = L.pz + off;
+            const double rjk2 = dist2(px[b] - px[a], py[b] - py[a], pz[b] - pz[a]);
+            sa = NPB > 1 ? (int)xa : L.sp[off + a];
+            sb = NPB > 1 ? (int)xb : L.sp[off + b];
             // sqrt(rjk2) < rc_jk as the reference tests it, without the root
             act = rjk2 < S.rct[UNI ? 0 : pidx(sa, sb, ns)];
         } else {
@@ -974,12 +977,13 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
             // an active pair has r_jk^2 < rc^2 there too.  (The reference
             // tests r_jk < rc_jk; the two differ only within an ulp of rc_jk,
             // where fc_jk vanishes quadratically.)
-            const double2 *D = reinterpret_cast<const double2 *>(L.px);
-            const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
+            const double2 *D0 = reinterpret_cast<const double2 *>(L.px);
+            const float4 *V = reinterpret_cast<const float4 *>(D0 + 2 * kW) + off;
+            const double2 *D = D0 + off;
             double ex, ey, ez;
             const double rjk2 = rjk2_of(D[a], D[kW + a], D[b], D[kW + b], ex, ey, ez);
-            sa = __float_as_int(V[a].w);
-            sb = __float_as_int(V[b].w);
+            sa = NPB > 1 ? (int)xa : __float_as_int(V[a].w);
+            sb = NPB > 1 ? (int)xb : __float_as_int(V[b].w);
             const double rc2 = (UNI || ns == 1) ? F.rq0
                                : (NPB > 1 ? S.rc2[blk_pair(sa + sb)] : S.rc2[pidx(sa, sb, ns)]);
             act = rjk2 < rc2;
@@ -994,18 +998,18 @@ __device__ __forceinline__ void enumerate_pairs(const Shared &S, const Grp &L, c
             if (act) L.l0[cnt[0] + below(m, s)] = (unsigned short)code;
             cnt[0] += __popcll(m);
         } else {
-            const int blk = sa + sb;  // two species: 0-0, 0-1, 1-1 = pair_index(2, sa, sb)
-            // the block masks as scalar ANDs of plain ballots (a ballot of
-            // act && blk == p would be rebuilt from a per-lane boolean)
+            // two species: blocks 0-0, 0-1, 1-1 = pair_index(2, sa, sb); the
+            // block masks as scalar logic on the two index compares' ballots (a
+            // ballot of act && blk == p would be rebuilt from a per-lane boolean)
             const unsigned long long ma = mt;
-            const unsigned long long b0 = __builtin_amdgcn_ballot_w64(blk == 0);
-            const unsigned long long b1 = __builtin_amdgcn_ballot_w64(blk == 1);
-            const unsigned long long m0 = ma & b0, m1 = ma & b1, m2 = ma & ~(b0 | b1);
+            const unsigned long long ba = __builtin_amdgcn_ballot_w64(xa);
+            const unsigned long long bb = __builtin_amdgcn_ballot_w64(xb);
+            const unsigned long long m0 = ma & ~(ba | bb), m1 = ma & (ba ^ bb), m2 = ma & ba & bb;
             if (act) {
                 const int p0 = base[0] + cnt[0] + below(m0, s);
                 const int p1 = base[1] + cnt[1] + below(m1, s);
                 const int p2 = base[2] + cnt[2] + below(m2, s);
-                L.l0[blk == 0 ? p0 : (blk == 1 ? p1 : p2)] = (unsigned short)code;
+                L.l0[xa != xb ? p1 : (xa ? p2 : p0)] = (unsigned short)code;
             }
             cnt[0] += __popcll(m0);
             cnt[1] += __popcll(m1);
@@ -1078,15 +1082,19 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 // into the two neighbours' FP64 LDS accumulators.  w / wv: the block's row
 // weights dE/dzeta 2^(1-zeta) and those times zeta lambda.  FWD: the triplet's
 // descriptor terms B^zeta exp(..) fc fc fc (without 2^(1-zeta)) are also added
-// into the lane's row accumulators `acc`, from the same powers.
+// into the lane's row accumulators `acc`, from the same powers.  a and b are
+// slots relative to off (the centre's first slot).
 template <int NPB, bool FWD, int ZG>
 __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, const Grp &L, int ns,
-                                              float rc, double rc2, float rcs, int a, int b,
+                                              float rc, double rc2, float rcs, int off, int a, int b,
                                               const float *w, const float *wv, double (&acc)[16])
 {
-    const double2 *D = reinterpret_cast<const double2 *>(L.px);
-    const float4 *V = reinterpret_cast<const float4 *>(D + 2 * kW);
-    const float2 *W = reinterpret_cast<const float2 *>(V + kW);
+    const double2 *D0 = reinterpret_cast<const double2 *>(L.px);
+    const float4 *V0 = reinterpret_cast<const float4 *>(D0 + 2 * kW);
+    const float2 *W = reinterpret_cast<const float2 *>(V0 + kW) + off;
+    const double2 *D = D0 + off;
+    const float4 *V = V0 + off;
+    double *fx = L.fx + off, *fy = L.fy + off, *fz = L.fz + off;
     const double2 a0 = D[a], a1 = D[kW + a], b0 = D[b], b1 = D[kW + b];
     const float4 va = V[a], vb = V[b];
     const float2 ea = W[a], eb = W[b];
@@ -1177,12 +1185,12 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     const double a1c = (double)(Aij * ria), a2c = (double)(Aik * rib), a3c = (double)(Ajk * g.rjkinv);
     // the r_jk term once, added to b and (negated, exactly) to a
     const double jx = a3c * g.ex, jy = a3c * g.ey, jz = a3c * g.ez;
-    atomicAdd(&L.fx[a], fma(a1c, a0.x, -jx));
-    atomicAdd(&L.fy[a], fma(a1c, a0.y, -jy));
-    atomicAdd(&L.fz[a], fma(a1c, a1.x, -jz));
-    atomicAdd(&L.fx[b], fma(a2c, b0.x, jx));
-    atomicAdd(&L.fy[b], fma(a2c, b0.y, jy));
-    atomicAdd(&L.fz[b], fma(a2c, b1.x, jz));
+    atomicAdd(&fx[a], fma(a1c, a0.x, -jx));
+    atomicAdd(&fy[a], fma(a1c, a0.y, -jy));
+    atomicAdd(&fz[a], fma(a1c, a1.x, -jz));
+    atomicAdd(&fx[b], fma(a2c, b0.x, jx));
+    atomicAdd(&fy[b], fma(a2c, b0.y, jy));
+    atomicAdd(&fz[b], fma(a2c, b1.x, jz));
 }
 
 // ZG: the zeta grid of the FP32-bulk form (ZGrid)
@@ -1671,7 +1679,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                     const int code = lst[t];
                     const int a = code & 63, b = (code >> 6) & 63;
                     if constexpr (P == 1) {
-                        adj_triplet_f<NPB, false, ZG>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
+                        adj_triplet_f<NPB, false, ZG>(S, F, L, ns, rcl, rc2l, rcsl, 0, a, b, w, wv, accd);
                     } else {
                         Trip<P> q;
                         triplet<P, true>(S, wp, wt, L, ns, a, b, q);
@@ -2076,10 +2084,13 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                     const int t = u + i * stride;
                     if (mine && t < nb) {
                         const int code = lst[t];
-                        const int a = code & 63, b = (code >> 6) & 63;
+                        // slots relative to the centre's first (off)
+                        const int ra = code & 63, rb = (code >> 6) & 63;
                         if constexpr (P == 1) {
-                            adj_triplet_f<NPB, true, ZG>(S, F, L, ns, rcl, rc2l, rcsl, a, b, w, wv, accd);
+                            adj_triplet_f<NPB, true, ZG>(S, F, L, ns, rcl, rc2l, rcsl, off, ra, rb, w, wv,
+                                                         accd);
                         } else {
+                            const int a = ra + off, b = rb + off;
                             Trip<P> q;
                             triplet<P, true>(S, wp, wt, L, ns, a, b, q);
                             const T dfa = (T)L.dfc[a], dfb = (T)L.dfc[b];
