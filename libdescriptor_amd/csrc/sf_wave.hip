@@ -484,11 +484,6 @@ constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
 constexpr int kAdjDoubles = kHeadAdj + 11 * kStr + 1;          // w, wv, g, px..fz, pad
 constexpr int kPairRows = kHeadPair + 11 * kStr + 1;           // w, wv, g, d, px..fz, pad
 constexpr int kPairDoubles = kPairRows + 2 * kWaveSlots;       // ..., gB, dB
-// The FP32-bulk adjoint kinds keep, in place of px..fc, per compacted
-// neighbour: D = two double2 (d_x, d_y) and (d_z, r); V = float4 (1/r, fc,
-// dfc, and the species with one block, r in FP32 with two blocks, whose
-// sorted slots give the species); W = float2 (the two exp(-eta_e r^2) of the
-// triplet sets).
 
 // kernel kinds: the forward, the adjoint, the fused forward + adjoint
 enum { kFwd = 0, kAdj = 1, kPair = 2 };
@@ -1089,10 +1084,10 @@ __device__ __forceinline__ void triplet(const Shared &S, const WaveParams &wp, c
 // descriptor terms B^zeta exp(..) fc fc fc (without 2^(1-zeta)) are also added
 // into the lane's row accumulators `acc`, from the same powers.  a and b are
 // slots relative to off (the centre's first slot).
-template <int NPB, bool FWD, int ZG, typename A>
+template <int NPB, bool FWD, int ZG>
 __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, const Grp &L, int ns,
                                               float rc, double rc2, float rcs, int off, int a, int b,
-                                              const float *w, const float *wv, A (&acc)[16])
+                                              const float *w, const float *wv, double (&acc)[16])
 {
     const double2 *D0 = reinterpret_cast<const double2 *>(L.px);
     const float4 *V0 = reinterpret_cast<const float4 *>(D0 + 2 * kW);
@@ -1113,8 +1108,7 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     geom_f(a0, a1, b0, b1, rc, rc2, rcs, g);
     const float cs = cos_q(g.y * g.y);
     const float fjk = g.sn * g.sn, dfjk = -2.0f * rcs * g.sn * cs;
-    const float ra = NPB > 1 ? va.w : (float)a1.y, rb = NPB > 1 ? vb.w : (float)b1.y;
-    const float ria = va.x, rib = vb.x;
+    const float ra = (float)a1.y, rb = (float)b1.y, ria = va.x, rib = vb.x;
     const float fa = va.y, fb = vb.y, dfa = va.z, dfb = vb.z;
     const float inv1 = ria * rib;  // 1 / (r_ij r_ik)
     const double rr = a1.y * b1.y, dot = dot_ab(a0, a1, b0, b1);
@@ -1141,13 +1135,11 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     E[0] = ea.x * eb.x * __builtin_amdgcn_exp2f(-F.etal[0] * g.rjk2);
     E[1] = ea.y * eb.y * __builtin_amdgcn_exp2f(-F.etal[1] * g.rjk2);
     float U[2] = {0.0f, 0.0f}, Vs[2] = {0.0f, 0.0f};
-    // FWD: the FP32 terms, summed in the type of acc (FP32 in the fused kernel:
-    // one lane's few terms of a centre, before the FP64 sums across lanes)
-    float te[2] = {0.0f, 0.0f};
+    double te[2] = {0.0, 0.0};  // FWD: FP32 terms, summed in FP64
     if (FWD) {
         const float fc3f = fa * fb * fjk;
-        te[0] = E[0] * fc3f;
-        te[1] = E[1] * fc3f;
+        te[0] = (double)(E[0] * fc3f);
+        te[1] = (double)(E[1] * fc3f);
     }
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
@@ -1174,7 +1166,7 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
                 const int k = (z * 2 + l) * 2 + e;
                 U[e] = fmaf(FWD ? w[k] : wl[k], pw, U[e]);
                 Vs[e] = fmaf(FWD ? wv[k] : wvl[k], pm, Vs[e]);
-                if (FWD) acc[(z * 2 + l) * 2 + e] = fmaf(pw, te[e], acc[(z * 2 + l) * 2 + e]);
+                if (FWD) acc[(z * 2 + l) * 2 + e] = fma((double)pw, te[e], acc[(z * 2 + l) * 2 + e]);
             }
         }
     }
@@ -1610,8 +1602,7 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const float r2f = (float)r2;
                 D[pos] = make_double2(d0, d1);
                 D[kW + pos] = make_double2(d2, r);
-                // the species (one block) or, with the blocks' sorted slots, r in FP32
-                V[pos] = make_float4((float)ir, fc, dfc, NPB > 1 ? (float)r : __int_as_float(sj));
+                V[pos] = make_float4((float)ir, fc, dfc, __int_as_float(sj));
                 W[pos] = make_float2(__builtin_amdgcn_exp2f(-F.etal[0] * r2f),
                                      __builtin_amdgcn_exp2f(-F.etal[1] * r2f));
             } else {
@@ -1983,8 +1974,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const float r2f = (float)r2;
                 D[pos] = make_double2(d0, d1);
                 D[kW + pos] = make_double2(d2, r);
-                // the species (one block) or, with the blocks' sorted slots, r in FP32
-                V[pos] = make_float4((float)ir, fc, dfc, NPB > 1 ? (float)r : __int_as_float(sj));
+                V[pos] = make_float4((float)ir, fc, dfc, __int_as_float(sj));
                 W[pos] = make_float2(__builtin_amdgcn_exp2f(-F.etal[0] * r2f),
                                      __builtin_amdgcn_exp2f(-F.etal[1] * r2f));
             } else {
@@ -2088,9 +2078,8 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const double rc2l = P == 1 ? S.rc2[kp] : 0.0;
                 const float rcsl = P == 1 ? (float)S.rcs[kp] : 0.0f;
                 double accd[16];
-                float accf[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) accd[k] = 0.0, accf[k] = 0.0f;
+                for (int k = 0; k < 16; ++k) accd[k] = 0.0;
                 for (int i = 0; i < passes; ++i) {
                     const int t = u + i * stride;
                     if (mine && t < nb) {
@@ -2099,7 +2088,7 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                         const int ra = code & 63, rb = (code >> 6) & 63;
                         if constexpr (P == 1) {
                             adj_triplet_f<NPB, true, ZG>(S, F, L, ns, rcl, rc2l, rcsl, off, ra, rb, w, wv,
-                                                         accf);
+                                                         accd);
                         } else {
                             const int a = ra + off, b = rb + off;
                             Trip<P> q;
@@ -2166,16 +2155,10 @@ void k_fused_pair(WaveTab wt, int n_centers, const int *__restrict__ centers,
                     }
                 }
                 __syncthreads();
-                // the row sums: each lane's own sums (FP32-bulk: FP32 over its one
-                // term per pass, 2 or 3 passes on a bulk system, about 32 for a
-                // 64-neighbour list) taken to FP64, then a column's 4 lanes with two
-                // register swaps, then the columns of a block through LDS (the pair
-                // list's region, now read), in order, all in FP64; lane s keeps
-                // values 4 (s >> 4) .. +3
-                if constexpr (P == 1) {
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) accd[k] = (double)accf[k];
-                }
+                // the row sums: a column's 4 lanes with two register swaps, then
+                // the columns of a block through LDS (the pair list's region,
+                // now read), in order, all in FP64; lane s keeps values
+                // 4 (s >> 4) .. +3
                 double v4[4];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
