@@ -484,6 +484,11 @@ constexpr int kFwdDoubles = 8 * kStr;                          // px..fc, d
 constexpr int kAdjDoubles = kHeadAdj + 11 * kStr + 1;          // w, wv, g, px..fz, pad
 constexpr int kPairRows = kHeadPair + 11 * kStr + 1;           // w, wv, g, d, px..fz, pad
 constexpr int kPairDoubles = kPairRows + 2 * kWaveSlots;       // ..., gB, dB
+// The FP32-bulk adjoint kinds keep, in place of px..fc, per compacted
+// neighbour: D = two double2 (d_x, d_y) and (d_z, r); V = float4 (1/r, fc,
+// dfc, and the species, or in the adjoint kind with two blocks, whose sorted
+// slots give the species, r in FP32); W = float2 (the two exp(-eta_e r^2) of
+// the triplet sets).
 
 // kernel kinds: the forward, the adjoint, the fused forward + adjoint
 enum { kFwd = 0, kAdj = 1, kPair = 2 };
@@ -1108,7 +1113,11 @@ __device__ __forceinline__ void adj_triplet_f(const Shared &S, const Fast &F, co
     geom_f(a0, a1, b0, b1, rc, rc2, rcs, g);
     const float cs = cos_q(g.y * g.y);
     const float fjk = g.sn * g.sn, dfjk = -2.0f * rcs * g.sn * cs;
-    const float ra = (float)a1.y, rb = (float)b1.y, ria = va.x, rib = vb.x;
+    // r in FP32: beside 1/r in the adjoint kernel's two-block lists; the fused
+    // kernel, short of registers, converts it at its use
+    const bool vr = NPB > 1 && !FWD;
+    const float ra = vr ? va.w : (float)a1.y, rb = vr ? vb.w : (float)b1.y;
+    const float ria = va.x, rib = vb.x;
     const float fa = va.y, fb = vb.y, dfa = va.z, dfb = vb.z;
     const float inv1 = ria * rib;  // 1 / (r_ij r_ik)
     const double rr = a1.y * b1.y, dot = dot_ab(a0, a1, b0, b1);
@@ -1602,7 +1611,8 @@ void k_adj_wave(WaveTab wt, int n_centers, const int *__restrict__ centers,
                 const float r2f = (float)r2;
                 D[pos] = make_double2(d0, d1);
                 D[kW + pos] = make_double2(d2, r);
-                V[pos] = make_float4((float)ir, fc, dfc, __int_as_float(sj));
+                // the species (one block) or, with the blocks' sorted slots, r in FP32
+                V[pos] = make_float4((float)ir, fc, dfc, NPB > 1 ? (float)r : __int_as_float(sj));
                 W[pos] = make_float2(__builtin_amdgcn_exp2f(-F.etal[0] * r2f),
                                      __builtin_amdgcn_exp2f(-F.etal[1] * r2f));
             } else {
